@@ -1,0 +1,256 @@
+"""CPU oracle — test infrastructure only.
+
+ctypes bindings for the plain-C FP64 restatement of the reference's fermion
+hot path (oracle/oracle.c).  Only tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg import this package, and only as the checker;
+the product package (yahmc_amd) never does.
+
+All arrays use the reference's host layouts (see oracle.c's header):
+spinor fields are float64 arrays of shape (V, 8*DR), fundamental links
+(V*4, 2*NC*NC), represented links (V*4, 2*DR*DR).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBDIR = os.path.join(HERE, "lib")
+
+BC_PERIODIC, BC_ANTIPERIODIC, BC_OPEN, BC_SF = 0, 1, 2, 3
+MVM_DPHI, MVM_HPHI, MVM_HPHI_SQ = 0, 1, 2
+
+MODELS = {
+    "su3_fund": (3, 0),
+    "su2_fund": (2, 0),
+    "su2_adj": (2, 1),
+    "su3_adj": (3, 1),
+    "su3_sym": (3, 2),
+    "su3_asym": (3, 3),
+    "su4_asym": (4, 3),
+}
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+
+
+def build(quiet=True):
+    """Compile every oracle model with gcc (oracle/Makefile)."""
+    out = subprocess.run(["make", "-C", HERE, "-j8"], capture_output=True, text=True)
+    if out.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + out.stdout + out.stderr)
+    if not quiet:
+        print(out.stdout)
+
+
+def _p(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_D)
+
+
+class Oracle:
+    """One (N, representation) model of the oracle.  The C side keeps global
+    state (like the reference), so one lattice is live per model at a time."""
+
+    _cache = {}
+
+    def __init__(self, model="su3_fund"):
+        path = os.path.join(LIBDIR, f"liboracle_{model}.so")
+        if not os.path.exists(path):
+            build()
+        if path not in Oracle._cache:
+            Oracle._cache[path] = ctypes.CDLL(path)
+        self.lib = lib = Oracle._cache[path]
+        self.model = model
+        self.nc = lib.or_nc()
+        self.dr = lib.or_dr()
+        self.repr = lib.or_repr()
+        lib.or_init.argtypes = [ctypes.c_int] * 5 + [ctypes.c_double, ctypes.c_double]
+        for name in ("or_Dphi_oe", "or_Dphi_eo"):
+            getattr(lib, name).argtypes = [_D, _D]
+        for name in ("or_Dphi_ee", "or_Dphi_oo", "or_Dphi_oo_inv", "or_Dphi", "or_Hphi", "or_Hphi_sq"):
+            getattr(lib, name).argtypes = [ctypes.c_double, _D, _D]
+        lib.or_mvm.argtypes = [ctypes.c_int, ctypes.c_double, _D, _D]
+        lib.or_sqnorm.argtypes = [_D]
+        lib.or_sqnorm.restype = ctypes.c_double
+        lib.or_product.argtypes = [_D, _D, _D]
+        lib.or_product_re.argtypes = [_D, _D]
+        lib.or_product_re.restype = ctypes.c_double
+        lib.or_cg.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _D, _D, _D]
+        lib.or_bicgstab.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _D, _D, _D]
+        lib.or_auto_inverter.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _D, _D]
+        lib.or_cg_mshift.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, _D, _D, _D]
+        lib.or_clover_logdet.argtypes = [ctypes.c_double]
+        lib.or_clover_logdet.restype = ctypes.c_double
+        lib.or_compute_ldl_decomp.argtypes = [ctypes.c_double]
+        lib.or_represent_link.argtypes = [_D, _D]
+        lib.or_get_generators.argtypes = [_D, _D]
+        for name in ("or_set_gauge", "or_get_gauge", "or_get_repr", "or_set_repr", "or_get_clover",
+                     "or_get_ldl", "or_get_clover_force", "or_get_momenta", "or_set_momenta",
+                     "or_apply_bcs_on_spinor", "or_g5"):
+            getattr(lib, name).argtypes = [_D]
+        lib.or_coords.argtypes = [ctypes.c_int, _I]
+
+    # ---- lattice -------------------------------------------------------
+    def init(self, dims, bc=BC_ANTIPERIODIC, csw=1.0, cf=1.0):
+        self.dims = tuple(int(d) for d in dims)
+        self.bc, self.csw, self.cf = bc, csw, cf
+        rc = self.lib.or_init(*self.dims, bc, csw, cf)
+        if rc != 0:
+            raise ValueError(f"unsupported lattice {dims}")
+        self.V = int(np.prod(self.dims))
+        self.VH = self.V // 2
+        return self
+
+    def index(self, t, x, y, z):
+        return self.lib.or_index(t, x, y, z)
+
+    def coords(self):
+        out = np.zeros((self.V, 4), dtype=np.int32)
+        c = (ctypes.c_int * 4)()
+        for i in range(self.V):
+            self.lib.or_coords(i, c)
+            out[i] = list(c)
+        return out
+
+    def neighbour(self, i, mu, d):
+        return self.lib.or_neighbour(i, mu, d)
+
+    # ---- fields --------------------------------------------------------
+    @property
+    def nsp(self):
+        return 8 * self.dr
+
+    def spinor(self):
+        return np.zeros((self.V, self.nsp))
+
+    def set_gauge(self, links):
+        links = np.ascontiguousarray(links, dtype=np.float64)
+        assert links.size == self.V * 4 * 2 * self.nc ** 2
+        self.lib.or_set_gauge(_p(links))
+
+    def get_gauge(self):
+        out = np.zeros((self.V * 4, 2 * self.nc ** 2))
+        self.lib.or_get_gauge(_p(out))
+        return out
+
+    def apply_bcs_on_gauge(self):
+        self.lib.or_apply_bcs_on_gauge_field()
+
+    def represent(self):
+        self.lib.or_represent()
+
+    def get_repr(self):
+        out = np.zeros((self.V * 4, 2 * self.dr ** 2))
+        self.lib.or_get_repr(_p(out))
+        return out
+
+    def set_repr(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        self.lib.or_set_repr(_p(a))
+
+    def get_clover(self):
+        out = np.zeros((self.V * 4, 2 * self.dr ** 2))
+        self.lib.or_get_clover(_p(out))
+        return out
+
+    def get_ldl(self):
+        n = self.dr * (2 * self.dr + 1)
+        out = np.zeros((self.V, 4 * n))
+        self.lib.or_get_ldl(_p(out))
+        flat = out.reshape(self.V, 2, n, 2)
+        return flat[..., 0] + 1j * flat[..., 1]  # (V, {upper,lower}, n)
+
+    def compute_ldl(self, sigma):
+        self.lib.or_compute_ldl_decomp(sigma)
+
+    def represent_link(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        out = np.zeros(2 * self.dr ** 2)
+        self.lib.or_represent_link(_p(out), _p(u))
+        return out
+
+    def generators(self):
+        ng = self.nc ** 2 - 1
+        f = np.zeros((ng, 2 * self.nc ** 2))
+        r = np.zeros((ng, 2 * self.dr ** 2))
+        self.lib.or_get_generators(_p(f), _p(r))
+        return f, r
+
+    # ---- operators (dirac.h) ------------------------------------------
+    def Dphi_oe(self, d, s):
+        self.lib.or_Dphi_oe(_p(d), _p(s))
+
+    def Dphi_eo(self, d, s):
+        self.lib.or_Dphi_eo(_p(d), _p(s))
+
+    def Dphi_ee(self, m, d, s):
+        self.lib.or_Dphi_ee(m, _p(d), _p(s))
+
+    def Dphi_oo(self, m, d, s):
+        self.lib.or_Dphi_oo(m, _p(d), _p(s))
+
+    def Dphi_oo_inv(self, m, d, s):
+        self.lib.or_Dphi_oo_inv(m, _p(d), _p(s))
+
+    def Dphi(self, m, d, s):
+        self.lib.or_Dphi(m, _p(d), _p(s))
+
+    def Hphi(self, m, d, s):
+        self.lib.or_Hphi(m, _p(d), _p(s))
+
+    def Hphi_sq(self, m, d, s):
+        self.lib.or_Hphi_sq(m, _p(d), _p(s))
+
+    def mvm(self, which, m, d, s):
+        self.lib.or_mvm(which, m, _p(d), _p(s))
+
+    def mvm_reset(self):
+        self.lib.or_mvm_reset()
+
+    def mvm_get(self):
+        return self.lib.or_mvm_get()
+
+    # ---- linalg / solvers ---------------------------------------------
+    def sqnorm(self, a):
+        return self.lib.or_sqnorm(_p(a))
+
+    def product(self, a, b):
+        out = np.zeros(2)
+        self.lib.or_product(_p(a), _p(b), _p(out))
+        return complex(out[0], out[1])
+
+    def product_re(self, a, b):
+        return self.lib.or_product_re(_p(a), _p(b))
+
+    def g5(self, s):
+        self.lib.or_g5(_p(s))
+
+    def cg(self, mvm, mass, prec, d, s):
+        r = np.zeros(1)
+        it = self.lib.or_cg(mvm, mass, prec, _p(d), _p(s), _p(r))
+        return it, r[0]
+
+    def bicgstab(self, mvm, mass, prec, d, s):
+        r = np.zeros(1)
+        it = self.lib.or_bicgstab(mvm, mass, prec, _p(d), _p(s), _p(r))
+        return it, r[0]
+
+    def auto_inverter(self, mvm, mass, prec, d, s):
+        return self.lib.or_auto_inverter(mvm, mass, prec, _p(d), _p(s))
+
+    def cg_mshift(self, mvm, mass, prec, shifts, s):
+        shifts = np.ascontiguousarray(shifts, dtype=np.float64)
+        out = np.zeros((len(shifts),) + s.shape)
+        it = self.lib.or_cg_mshift(mvm, mass, prec, len(shifts), _p(shifts), _p(out), _p(s))
+        return it, out
+
+    def clover_logdet(self, mass):
+        return self.lib.or_clover_logdet(mass)
+
+    def apply_bcs_on_spinor(self, s):
+        self.lib.or_apply_bcs_on_spinor(_p(s))
+
+    def set_threads(self, n):
+        self.lib.or_set_threads(n)

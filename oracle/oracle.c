@@ -1,0 +1,1087 @@
+/*
+ * CPU ORACLE — test infrastructure only.
+ *
+ * A plain-C, FP64 restatement of the reference's fermion hot path, used by
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as the
+ * checker.  Nothing in the product (yahmc_amd/) links, loads or calls it.
+ *
+ * Every function cites the reference file:line it restates (paths relative
+ * to the reference repository root).  Data layouts are the reference's own
+ * host layouts so that arrays can be compared element by element:
+ *
+ *   Spinor   re[DR][4] then im[DR][4]             (include/spinor.h:7-9)
+ *   suNg     re[NC*NC] then im[NC*NC], row-major   (include/suN.h:26-28)
+ *   suNf     re[DR*DR] then im[DR*DR], row-major   (include/suN.h:57-59)
+ *   links    gfield[4*id+mu]                       (include/gaugefield.h:6-10)
+ *   clover   cl_term[4*id+k], k = A,B,C,D          (include/clover.h:12-22)
+ *   LDL      cl_ldl[id].{upper,lower}[i(i+1)/2+j]  (include/clover.h:24-33)
+ *   site id  (parity*V + lex)/2, lex = ((t*X+x)*Y+y)*Z+z
+ *                                                  (library/geometry.cpp:756-878)
+ *
+ * Compile-time model, as in the reference (programs/Makefile:6-11):
+ *   OR_NC   = number of colours (2 or 3)
+ *   OR_REPR = 0 fundamental, 1 adjoint, 2 symmetric, 3 antisymmetric
+ *             (the reference's REPR_ID values, include/repr.h:9-39)
+ * Boundary conditions and c_sw are run-time parameters here.
+ *
+ * Pinning (what checks each function, see tests/test_oracle_*.py):
+ *   hop            dense gamma-matrix Wilson formula on the full lattice,
+ *                  free-field plane-wave eigenvalues, gauge covariance,
+ *                  gamma5-hermiticity of the Schur operator
+ *   clover term    textbook clover leaf + sigma_{mu nu} construction in dense
+ *                  4*DR form, hermiticity, covariance
+ *   LDL / inverse  dense inverse and determinant (numpy.linalg)
+ *   represent      homomorphism R(UV)=R(U)R(V), R(U^+)=R(U)^+, generator
+ *                  orthogonality, and the reference's explicit per-entry
+ *                  formulas transcribed independently in the test
+ *   solvers        true residuals, the reference's self-check tolerances
+ *   forces         finite differences of the action
+ */
+#include <complex.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdio.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#ifndef OR_NC
+#define OR_NC 3
+#endif
+#ifndef OR_REPR
+#define OR_REPR 0
+#endif
+
+#define NC OR_NC
+#define NG (NC * NC - 1)
+#define TF 2.0
+
+#if OR_REPR == 0
+#define DR NC
+#elif OR_REPR == 1
+#define DR (NC * NC - 1)
+#elif OR_REPR == 2
+#define DR (NC * (NC + 1) / 2)
+#elif OR_REPR == 3
+#define DR (NC * (NC - 1) / 2)
+#else
+#error unknown representation
+#endif
+
+#define NSP (8 * DR)      /* doubles per Spinor */
+#define NLF (2 * DR * DR) /* doubles per suNf */
+#define NLG (2 * NC * NC) /* doubles per suNg */
+#define NB (2 * DR)       /* size of one chiral clover block */
+#define NLDL (DR * (2 * DR + 1))
+
+typedef double complex cplx;
+
+enum { BC_PERIODIC = 0, BC_ANTIPERIODIC = 1, BC_OPEN = 2, BC_SF = 3 };
+enum { PART_EVEN = 1, PART_ODD = 2, PART_BOTH = 3 };
+enum { MVM_DPHI = 0, MVM_HPHI = 1, MVM_HPHI_SQ = 2 };
+
+/* ------------------------------------------------------------------ */
+/* global state (the reference keeps the same fields as globals:       */
+/* include/gaugefield.h:15-18, include/clover.h:35-37)                */
+/* ------------------------------------------------------------------ */
+static int L[4];
+static int V, VH;
+static int *nb_fw, *nb_bk, *crd;
+static int g_bc;
+static double g_csw, g_cf, g_Ta = -1, g_Tb = -1;
+static double *gfund, *grepr, *clterm, *clforce, *mom;
+static cplx *ldl_up, *ldl_lo;
+static double cl_sigma = 0xF00F;
+static int mvm_counter;
+static double *tmp_a, *tmp_b, *tmp_c;
+
+/* generators and representation basis (library/repr.cpp:13-105) */
+static cplx iTfund[NG][NC][NC];
+static cplx iTrepr[NG][DR][DR];
+static cplx basis[DR][NC][NC];
+
+#define SRE(p, c, s) (p)[(c) * 4 + (s)]
+#define SIM(p, c, s) (p)[4 * DR + (c) * 4 + (s)]
+
+static inline cplx sget(const double *p, int c, int s) { return SRE(p, c, s) + I * SIM(p, c, s); }
+static inline void sset(double *p, int c, int s, cplx v) { SRE(p, c, s) = creal(v); SIM(p, c, s) = cimag(v); }
+static inline cplx fget(const double *m, int i, int j) { return m[i * DR + j] + I * m[DR * DR + i * DR + j]; }
+static inline void fset(double *m, int i, int j, cplx v) { m[i * DR + j] = creal(v); m[DR * DR + i * DR + j] = cimag(v); }
+static inline cplx gget(const double *m, int i, int j) { return m[i * NC + j] + I * m[NC * NC + i * NC + j]; }
+
+int or_nc(void) { return NC; }
+int or_dr(void) { return DR; }
+int or_repr(void) { return OR_REPR; }
+int or_volume(void) { return V; }
+
+void or_set_threads(int n)
+{
+#ifdef _OPENMP
+	omp_set_num_threads(n);
+#else
+	(void)n;
+#endif
+}
+
+int or_get_threads(void)
+{
+#ifdef _OPENMP
+	return omp_get_max_threads();
+#else
+	return 1;
+#endif
+}
+
+/* ------------------------------------------------------------------ */
+/* geometry: library/geometry.cpp:756-878 (EO branch 763-768, no halo) */
+/* and neighbours 1161-1187                                            */
+/* ------------------------------------------------------------------ */
+int or_index(int t, int x, int y, int z)
+{
+	int lex = ((t * L[1] + x) * L[2] + y) * L[3] + z;
+	int par = (t + x + y + z) & 1;
+	return (par * V + lex) / 2;
+}
+
+static int wrap(int a, int n) { return (a % n + n) % n; }
+
+int or_global_time(int id) { return crd[4 * id]; } /* geometry.cpp:1001-1007 */
+
+void or_coords(int id, int *out)
+{
+	for (int k = 0; k < 4; k++) out[k] = crd[4 * id + k];
+}
+
+int or_neighbour(int id, int mu, int dir) { return dir > 0 ? nb_fw[4 * id + mu] : nb_bk[4 * id + mu]; }
+
+/* ------------------------------------------------------------------ */
+/* SU(N) helpers                                                       */
+/* ------------------------------------------------------------------ */
+typedef struct { cplx m[NC][NC]; } gmat;
+
+static gmat gload(const double *p)
+{
+	gmat r;
+	for (int i = 0; i < NC; i++)
+		for (int j = 0; j < NC; j++) r.m[i][j] = gget(p, i, j);
+	return r;
+}
+static gmat gmul(gmat a, gmat b)
+{
+	gmat r;
+	for (int i = 0; i < NC; i++)
+		for (int j = 0; j < NC; j++) {
+			cplx s = 0;
+			for (int k = 0; k < NC; k++) s += a.m[i][k] * b.m[k][j];
+			r.m[i][j] = s;
+		}
+	return r;
+}
+static gmat gdag(gmat a)
+{
+	gmat r;
+	for (int i = 0; i < NC; i++)
+		for (int j = 0; j < NC; j++) r.m[i][j] = conj(a.m[j][i]);
+	return r;
+}
+
+/* suNg(suNa) algebra -> anti-Hermitian matrix: library/suNg.cpp:15-58 */
+static gmat alg_to_mat(const double *av)
+{
+	gmat r;
+	memset(&r, 0, sizeof(r));
+	int n = 0;
+	for (int i = 0; i < NC; i++)
+		for (int j = 0; j < NC; j++) {
+			if (i > j) {
+				r.m[i][j] += I * av[n];
+				r.m[j][i] += I * av[n];
+				n++;
+			} else if (i < j) {
+				r.m[i][j] += av[n];
+				r.m[j][i] += -av[n];
+				n++;
+			} else if (i > 0) {
+				double v = sqrt(2.0 / (i * (i + 1)));
+				for (int k = 0; k < i; k++) r.m[k][k] += I * v * av[n];
+				r.m[i][i] -= I * (i * v) * av[n];
+				n++;
+			}
+		}
+	return r;
+}
+
+/* ------------------------------------------------------------------ */
+/* representation: library/repr.cpp:17-263                             */
+/* The map is written as the action on the representation basis        */
+/*   fund:      R(U) = U                                               */
+/*   sym/asym:  R(U)_{nm} = sum_ab conj(b_n)_{ab} (U b_m U^T)_{ab}       */
+/*   adjoint:   R(U)_{nm} = Tr(b_n U b_m U^+)                          */
+/* with the reference's bases (repr.cpp:25-61).  Its explicit per-entry */
+/* expansion (repr.cpp:107-263) is checked against this in the tests.   */
+/* ------------------------------------------------------------------ */
+static void repr_init(void)
+{
+	memset(basis, 0, sizeof(basis));
+	const double s2 = 1.0 / sqrt(2.0);
+	(void)s2;
+#if OR_REPR == 2
+	for (int i = 0, n = 0; i < NC; i++)
+		for (int j = 0; j <= i; j++, n++) {
+			if (i == j) basis[n][i][i] = 1;
+			else { basis[n][i][j] = s2; basis[n][j][i] = s2; }
+		}
+#elif OR_REPR == 3
+	for (int i = 0, n = 0; i < NC; i++)
+		for (int j = 0; j < i; j++, n++) {
+			basis[n][j][i] = s2;
+			basis[n][i][j] = -s2;
+		}
+#elif OR_REPR == 1
+	for (int n = 0; n < NG; n++) {
+		double av[NG] = {0};
+		av[n] = 1;
+		gmat g = alg_to_mat(av);
+		for (int i = 0; i < NC; i++)
+			for (int j = 0; j < NC; j++) basis[n][i][j] = (-I * s2) * g.m[i][j];
+	}
+#endif
+	for (int k = 0; k < NG; k++) {
+		double av[NG] = {0};
+		av[k] = 1;
+		gmat g = alg_to_mat(av);
+		memcpy(iTfund[k], g.m, sizeof(g.m));
+	}
+	/* iT^R_k = dR/dt (exp(t iT_k)) at t = 0 (repr.cpp:71-103) */
+	for (int k = 0; k < NG; k++) {
+		const cplx(*G)[NC] = iTfund[k];
+		for (int n = 0; n < DR; n++)
+			for (int m = 0; m < DR; m++) {
+#if OR_REPR == 0
+				iTrepr[k][n][m] = G[n][m];
+#elif OR_REPR == 1
+				cplx s = 0; /* Tr(b_n [G, b_m]) */
+				for (int a = 0; a < NC; a++)
+					for (int b = 0; b < NC; b++)
+						for (int c = 0; c < NC; c++)
+							s += basis[n][a][b] * (G[b][c] * basis[m][c][a] - basis[m][b][c] * G[c][a]);
+				iTrepr[k][n][m] = s;
+#else
+				cplx s = 0; /* sum conj(b_n)_{ab} (G b_m + b_m G^T)_{ab} */
+				for (int a = 0; a < NC; a++)
+					for (int b = 0; b < NC; b++) {
+						cplx v = 0;
+						for (int c = 0; c < NC; c++) v += G[a][c] * basis[m][c][b] + basis[m][a][c] * G[b][c];
+						s += conj(basis[n][a][b]) * v;
+					}
+				iTrepr[k][n][m] = s;
+#endif
+			}
+	}
+}
+
+static void represent_one(double *out, const double *in)
+{
+	gmat U = gload(in);
+#if OR_REPR == 0
+	memcpy(out, in, sizeof(double) * NLG);
+	(void)U;
+#else
+	for (int n = 0; n < DR; n++)
+		for (int m = 0; m < DR; m++) {
+			cplx s = 0;
+#if OR_REPR == 1
+			/* Tr(b_n U b_m U^+) */
+			for (int a = 0; a < NC; a++)
+				for (int b = 0; b < NC; b++) {
+					cplx v = 0;
+					for (int c = 0; c < NC; c++)
+						for (int d = 0; d < NC; d++) v += U.m[a][c] * basis[m][c][d] * conj(U.m[b][d]);
+					s += basis[n][b][a] * v;
+				}
+#else
+			for (int a = 0; a < NC; a++)
+				for (int b = 0; b < NC; b++) {
+					cplx v = 0;
+					for (int c = 0; c < NC; c++)
+						for (int d = 0; d < NC; d++) v += U.m[a][c] * basis[m][c][d] * U.m[b][d];
+					s += conj(basis[n][a][b]) * v;
+				}
+#endif
+			fset(out, n, m, s);
+		}
+#endif
+}
+
+void or_represent_link(double *out, const double *in) { represent_one(out, in); }
+
+void or_get_generators(double *fund_out, double *repr_out)
+{
+	for (int k = 0; k < NG; k++) {
+		for (int i = 0; i < NC; i++)
+			for (int j = 0; j < NC; j++) {
+				fund_out[k * NLG + i * NC + j] = creal(iTfund[k][i][j]);
+				fund_out[k * NLG + NC * NC + i * NC + j] = cimag(iTfund[k][i][j]);
+			}
+		for (int i = 0; i < DR; i++)
+			for (int j = 0; j < DR; j++) fset(repr_out + k * NLF, i, j, iTrepr[k][i][j]);
+	}
+}
+
+/* ------------------------------------------------------------------ */
+/* init / free                                                          */
+/* ------------------------------------------------------------------ */
+void or_free(void)
+{
+	free(nb_fw); free(nb_bk); free(crd);
+	free(gfund); free(grepr); free(clterm); free(clforce); free(mom);
+	free(ldl_up); free(ldl_lo);
+	free(tmp_a); free(tmp_b); free(tmp_c);
+	nb_fw = nb_bk = crd = NULL;
+	gfund = grepr = clterm = clforce = mom = NULL;
+	ldl_up = ldl_lo = NULL;
+	tmp_a = tmp_b = tmp_c = NULL;
+}
+
+/* bcs_init: library/bcs.cpp:11-137 (the parts that reach the fermion path) */
+int or_init(int T, int X, int Y, int Z, int bc, double csw, double cf)
+{
+	if (T < 2 || X < 2 || Y < 2 || Z < 2 || (Z & 1) || (T & 1) || (X & 1) || (Y & 1)) return -1;
+	or_free();
+	L[0] = T; L[1] = X; L[2] = Y; L[3] = Z;
+	V = T * X * Y * Z;
+	VH = V / 2;
+	g_bc = bc;
+	g_csw = csw;
+	g_cf = cf;
+	g_Ta = g_Tb = -1;
+	if (bc == BC_OPEN) { g_Ta = 1; g_Tb = T - 2; }
+	if (bc == BC_SF) { g_Ta = 1; g_Tb = T - 3; }
+	nb_fw = malloc(sizeof(int) * 4 * V);
+	nb_bk = malloc(sizeof(int) * 4 * V);
+	crd = malloc(sizeof(int) * 4 * V);
+	for (int t = 0; t < T; t++)
+		for (int x = 0; x < X; x++)
+			for (int y = 0; y < Y; y++)
+				for (int z = 0; z < Z; z++) {
+					int id = or_index(t, x, y, z);
+					int c[4] = {t, x, y, z};
+					for (int mu = 0; mu < 4; mu++) {
+						crd[4 * id + mu] = c[mu];
+						int cf_[4] = {t, x, y, z}, cb_[4] = {t, x, y, z};
+						cf_[mu] = wrap(c[mu] + 1, L[mu]);
+						cb_[mu] = wrap(c[mu] - 1, L[mu]);
+						nb_fw[4 * id + mu] = or_index(cf_[0], cf_[1], cf_[2], cf_[3]);
+						nb_bk[4 * id + mu] = or_index(cb_[0], cb_[1], cb_[2], cb_[3]);
+					}
+				}
+	gfund = calloc((size_t)4 * V * NLG, sizeof(double));
+	grepr = calloc((size_t)4 * V * NLF, sizeof(double));
+	clterm = calloc((size_t)4 * V * NLF, sizeof(double));
+	clforce = calloc((size_t)6 * V * NLF, sizeof(double));
+	mom = calloc((size_t)4 * V * NG, sizeof(double));
+	ldl_up = calloc((size_t)V * NLDL, sizeof(cplx));
+	ldl_lo = calloc((size_t)V * NLDL, sizeof(cplx));
+	tmp_a = calloc((size_t)V * NSP, sizeof(double));
+	tmp_b = calloc((size_t)V * NSP, sizeof(double));
+	tmp_c = calloc((size_t)V * NSP, sizeof(double));
+	cl_sigma = 0xF00F;
+	mvm_counter = 0;
+	repr_init();
+	return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* gauge field access                                                   */
+/* ------------------------------------------------------------------ */
+void or_set_gauge(const double *links) { memcpy(gfund, links, sizeof(double) * 4 * (size_t)V * NLG); }
+void or_get_gauge(double *links) { memcpy(links, gfund, sizeof(double) * 4 * (size_t)V * NLG); }
+void or_get_repr(double *out) { memcpy(out, grepr, sizeof(double) * 4 * (size_t)V * NLF); }
+void or_set_repr(const double *in) { memcpy(grepr, in, sizeof(double) * 4 * (size_t)V * NLF); }
+void or_get_clover(double *out) { memcpy(out, clterm, sizeof(double) * 4 * (size_t)V * NLF); }
+void or_get_clover_force(double *out) { memcpy(out, clforce, sizeof(double) * 6 * (size_t)V * NLF); }
+void or_get_momenta(double *out) { memcpy(out, mom, sizeof(double) * 4 * (size_t)V * NG); }
+void or_set_momenta(const double *in) { memcpy(mom, in, sizeof(double) * 4 * (size_t)V * NG); }
+
+/* ldl entries as (re, im) pairs, per site NLDL complex upper then lower */
+void or_get_ldl(double *out)
+{
+	for (int id = 0; id < V; id++)
+		for (int k = 0; k < NLDL; k++) {
+			out[(size_t)id * 4 * NLDL + 2 * k] = creal(ldl_up[(size_t)id * NLDL + k]);
+			out[(size_t)id * 4 * NLDL + 2 * k + 1] = cimag(ldl_up[(size_t)id * NLDL + k]);
+			out[(size_t)id * 4 * NLDL + 2 * NLDL + 2 * k] = creal(ldl_lo[(size_t)id * NLDL + k]);
+			out[(size_t)id * 4 * NLDL + 2 * NLDL + 2 * k + 1] = cimag(ldl_lo[(size_t)id * NLDL + k]);
+		}
+}
+
+/* apply_bcs_on_gauge_field: library/bcs.cpp:201-262 */
+void or_apply_bcs_on_gauge_field(void)
+{
+	if (g_bc == BC_OPEN) {
+		for (int id = 0; id < V; id++)
+			if (crd[4 * id] == L[0] - 1) memset(gfund + (4 * (size_t)id + 0) * NLG, 0, sizeof(double) * NLG);
+	}
+	if (g_bc == BC_SF && NC <= 3) {
+		double v1[4] = {0}, v2[4] = {0}, Ls;
+		if (NC == 2) { v1[0] = -M_PI; v1[1] = M_PI; v2[0] = -3 * M_PI; v2[1] = 3 * M_PI; Ls = 4.0 * L[1]; }
+		else { v1[0] = -M_PI; v1[1] = 0; v1[2] = M_PI; v2[0] = -5 * M_PI; v2[1] = 2 * M_PI; v2[2] = 3 * M_PI; Ls = 6.0 * L[1]; }
+		double C1[NLG], C2[NLG];
+		memset(C1, 0, sizeof(C1));
+		memset(C2, 0, sizeof(C2));
+		for (int i = 0; i < NC; i++) {
+			/* exp of a diagonal anti-Hermitian matrix (the reference's Taylor
+			 * series + reunitarisation converges to this to 1e-16) */
+			C1[i * NC + i] = cos(v1[i] / Ls); C1[NC * NC + i * NC + i] = sin(v1[i] / Ls);
+			C2[i * NC + i] = cos(v2[i] / Ls); C2[NC * NC + i * NC + i] = sin(v2[i] / Ls);
+		}
+		for (int id = 0; id < V; id++) {
+			int t = crd[4 * id];
+			double *l = gfund + 4 * (size_t)id * NLG;
+			if (t == 0)
+				for (int mu = 1; mu < 4; mu++) memcpy(l + mu * NLG, C1, sizeof(C1));
+			if (t == L[0] - 2) {
+				memset(l, 0, sizeof(double) * NLG);
+				for (int mu = 1; mu < 4; mu++) memcpy(l + mu * NLG, C2, sizeof(C2));
+			}
+			if (t == L[0] - 1) memset(l, 0, sizeof(double) * 4 * NLG);
+		}
+	}
+}
+
+/* ------------------------------------------------------------------ */
+/* clover term: library/wilsonloop.cpp:5-31 + library/clover.cpp:93-168 */
+/* ------------------------------------------------------------------ */
+static gmat LNK(int id, int mu) { return gload(gfund + (4 * (size_t)id + mu) * NLG); }
+
+static void clover_repr(int id, int mu, int nu, cplx out[DR][DR])
+{
+	gmat g[4];
+	int a, b, c;
+	a = nb_fw[4 * id + mu]; b = nb_fw[4 * id + nu];
+	g[0] = gmul(gmul(LNK(id, mu), LNK(a, nu)), gdag(gmul(LNK(id, nu), LNK(b, mu))));
+	a = nb_bk[4 * id + mu]; b = nb_fw[4 * a + nu];
+	g[1] = gmul(gmul(LNK(id, nu), gdag(gmul(LNK(a, nu), LNK(b, mu)))), LNK(a, mu));
+	a = nb_bk[4 * id + mu]; b = nb_bk[4 * id + nu]; c = nb_bk[4 * a + nu];
+	g[2] = gmul(gmul(gdag(gmul(LNK(c, nu), LNK(a, mu))), LNK(c, mu)), LNK(b, nu));
+	a = nb_bk[4 * id + nu]; b = nb_fw[4 * a + mu];
+	g[3] = gmul(gmul(gmul(gdag(LNK(a, nu)), LNK(a, mu)), LNK(b, nu)), gdag(LNK(id, mu)));
+	for (int i = 0; i < DR; i++)
+		for (int j = 0; j < DR; j++) out[i][j] = 0;
+	for (int k = 0; k < 4; k++) {
+		double in[NLG], f[NLF];
+		for (int i = 0; i < NC; i++)
+			for (int j = 0; j < NC; j++) { in[i * NC + j] = creal(g[k].m[i][j]); in[NC * NC + i * NC + j] = cimag(g[k].m[i][j]); }
+		represent_one(f, in);
+		for (int i = 0; i < DR; i++)
+			for (int j = 0; j < DR; j++) out[i][j] += fget(f, i, j);
+	}
+}
+
+static void compute_clover_term_site(int id)
+{
+	double *cl = clterm + 4 * (size_t)id * NLF;
+	int t = crd[4 * id];
+	int cond = 0;
+	if (g_bc == BC_SF) cond = (t == 0 || t == L[0] - 1 || t == L[0] - 2);
+	if (g_bc == BC_OPEN) cond = (t == 0 || t == L[0] - 1);
+	if (cond || g_csw == 0.0) {
+		memset(cl, 0, sizeof(double) * 4 * NLF);
+		return;
+	}
+	cplx q[6][DR][DR];
+	clover_repr(id, 0, 1, q[0]);
+	clover_repr(id, 0, 2, q[1]);
+	clover_repr(id, 0, 3, q[2]);
+	clover_repr(id, 1, 2, q[3]);
+	clover_repr(id, 1, 3, q[4]);
+	clover_repr(id, 2, 3, q[5]);
+	double c = -g_csw / 16.0;
+	for (int i = 0; i < DR; i++)
+		for (int j = 0; j < DR; j++) {
+			/* clover.cpp:149-165 */
+			double are = cimag(q[2][i][j]) + cimag(q[2][j][i]);
+			double aim = creal(q[2][j][i]) - creal(q[2][i][j]);
+			double bre = cimag(q[3][i][j]) + cimag(q[3][j][i]);
+			double bim = creal(q[3][i][j]) - creal(q[3][j][i]);
+			double cre = cimag(q[0][i][j]) + cimag(q[0][j][i]) - creal(q[1][i][j]) + creal(q[1][j][i]);
+			double cim = creal(q[0][j][i]) - creal(q[0][i][j]) - cimag(q[1][i][j]) - cimag(q[1][j][i]);
+			double dre = creal(q[4][i][j]) - creal(q[4][j][i]) + cimag(q[5][i][j]) + cimag(q[5][j][i]);
+			double dim = cimag(q[4][i][j]) + cimag(q[4][j][i]) - creal(q[5][i][j]) + creal(q[5][j][i]);
+			fset(cl + 0 * NLF, i, j, c * (are - bre) + I * (c * (aim + bim)));
+			fset(cl + 1 * NLF, i, j, c * (cre - dre) + I * (c * (cim - dim)));
+			fset(cl + 2 * NLF, i, j, -c * (are + bre) + I * (c * (bim - aim)));
+			fset(cl + 3 * NLF, i, j, -c * (cre + dre) + I * (-c * (cim + dim)));
+		}
+}
+
+/* represent_gauge_field: library/repr.cpp:265-280, then
+ * apply_bcs_on_represented_gauge_field: library/bcs.cpp:264-272 */
+void or_represent(void)
+{
+#pragma omp parallel for
+	for (int id = 0; id < V; id++)
+		for (int mu = 0; mu < 4; mu++) represent_one(grepr + (4 * (size_t)id + mu) * NLF, gfund + (4 * (size_t)id + mu) * NLG);
+	cl_sigma = 0xF00F; /* clover.cpp:337 */
+#pragma omp parallel for
+	for (int id = 0; id < V; id++) compute_clover_term_site(id);
+	if (g_bc == BC_ANTIPERIODIC)
+		for (int id = 0; id < V; id++)
+			if (crd[4 * id] == 0) {
+				double *l = grepr + (4 * (size_t)id + 0) * NLF;
+				for (int k = 0; k < NLF; k++) l[k] = -l[k];
+			}
+}
+
+/* ------------------------------------------------------------------ */
+/* LDL: library/solver.cpp:97-117 and library/clover.cpp:25-91, 307-328 */
+/* ------------------------------------------------------------------ */
+static void ldl_factor(int n, cplx *A)
+{
+	for (int i = 0; i < n; i++) {
+		double d = creal(A[i * n + i]);
+		for (int k = 0; k < i; k++) {
+			cplx a = A[i * n + k];
+			d -= (creal(a) * creal(a) + cimag(a) * cimag(a)) * creal(A[k * n + k]);
+		}
+		A[i * n + i] = d;
+		for (int j = i + 1; j < n; j++) {
+			for (int k = 0; k < i; k++) A[j * n + i] -= A[j * n + k] * conj(A[i * n + k]) * creal(A[k * n + k]);
+			A[j * n + i] /= creal(A[i * n + i]);
+		}
+	}
+}
+
+static double bc_cf(int id)
+{
+	int t = crd[4 * id];
+	if ((g_bc == BC_OPEN || g_bc == BC_SF) && (t == g_Ta || t == g_Tb)) return g_cf - 1.0;
+	return 0.0;
+}
+
+static void compute_ldl_site(int id)
+{
+	cplx A[NB * NB], B[NB * NB];
+	memset(A, 0, sizeof(A));
+	memset(B, 0, sizeof(B));
+	const double *cl = clterm + 4 * (size_t)id * NLF;
+	double cf = bc_cf(id);
+	for (int i = 0; i < DR; i++) {
+		int o1 = NB * i, o2 = NB * (i + DR), o3 = o2 + DR;
+		for (int j = 0; j < DR; j++) {
+			A[o2 + j] = conj(fget(cl + 1 * NLF, j, i));
+			B[o2 + j] = conj(fget(cl + 3 * NLF, j, i));
+			if (i >= j) {
+				A[o1 + j] = fget(cl + 0 * NLF, i, j);
+				A[o3 + j] = -fget(cl + 0 * NLF, i, j);
+				B[o1 + j] = fget(cl + 2 * NLF, i, j);
+				B[o3 + j] = -fget(cl + 2 * NLF, i, j);
+				if (i == j) {
+					A[o1 + j] += cl_sigma + cf;
+					A[o3 + j] += cl_sigma + cf;
+					B[o1 + j] += cl_sigma + cf;
+					B[o3 + j] += cl_sigma + cf;
+				}
+			}
+		}
+	}
+	ldl_factor(NB, A);
+	ldl_factor(NB, B);
+	for (int i = 0; i < NB; i++)
+		for (int j = 0; j <= i; j++) {
+			ldl_up[(size_t)id * NLDL + i * (i + 1) / 2 + j] = A[i * NB + j];
+			ldl_lo[(size_t)id * NLDL + i * (i + 1) / 2 + j] = B[i * NB + j];
+		}
+}
+
+void or_compute_ldl_decomp(double sigma)
+{
+	if (sigma == cl_sigma) return;
+	cl_sigma = sigma;
+#pragma omp parallel for
+	for (int id = VH; id < V; id++) compute_ldl_site(id);
+}
+
+/* clover_logdet: library/clover.cpp:281-305 */
+double or_clover_logdet(double mass)
+{
+	double det = 0;
+	or_compute_ldl_decomp(4.0 + mass);
+#pragma omp parallel for reduction(+ : det)
+	for (int id = VH; id < V; id++) {
+		double prod = 1;
+		for (int n = 0; n < NB; n++) {
+			prod *= creal(ldl_up[(size_t)id * NLDL + n * (n + 1) / 2 + n]);
+			prod *= creal(ldl_lo[(size_t)id * NLDL + n * (n + 1) / 2 + n]);
+		}
+		det += log(prod);
+	}
+	return det;
+}
+
+/* ------------------------------------------------------------------ */
+/* boundary condition on spinors: library/bcs.cpp:139-164              */
+/* ------------------------------------------------------------------ */
+static void apply_bcs_on_spinor(double *s)
+{
+	if (g_bc != BC_OPEN && g_bc != BC_SF) return;
+	for (int id = 0; id < V; id++) {
+		int t = crd[4 * id];
+		int z = 0;
+		if (g_bc == BC_OPEN) z = (t == 0 || t == L[0] - 1);
+		if (g_bc == BC_SF) z = (t == 0 || t == L[0] - 2 || t == L[0] - 1);
+		if (z) memset(s + (size_t)id * NSP, 0, sizeof(double) * NSP);
+	}
+}
+
+void or_apply_bcs_on_spinor(double *s) { apply_bcs_on_spinor(s); }
+
+static void part_range(int part, int *lo, int *hi)
+{
+	*lo = (part == PART_ODD) ? VH : 0;
+	*hi = (part == PART_EVEN) ? VH : V;
+}
+
+/* ------------------------------------------------------------------ */
+/* clover_kernel: library/dirac.cpp:25-107                             */
+/* ------------------------------------------------------------------ */
+static void clover_kernel(double mass, double *dst, const double *src, int part, int assign)
+{
+	int lo, hi;
+	part_range(part, &lo, &hi);
+	mass = 4.0 + mass;
+#pragma omp parallel for
+	for (int id = lo; id < hi; id++) {
+		const double *s = src + (size_t)id * NSP;
+		const double *cl = clterm + 4 * (size_t)id * NLF;
+		double m = mass + bc_cf(id);
+		cplx out[DR][4];
+		for (int i = 0; i < DR; i++)
+			for (int sp = 0; sp < 4; sp++) out[i][sp] = m * sget(s, i, sp);
+		for (int i = 0; i < DR; i++)
+			for (int j = 0; j < DR; j++) {
+				cplx a = fget(cl, i, j), b = fget(cl + NLF, i, j), bt = conj(fget(cl + NLF, j, i));
+				cplx c = fget(cl + 2 * NLF, i, j), d = fget(cl + 3 * NLF, i, j), dt = conj(fget(cl + 3 * NLF, j, i));
+				out[i][0] += a * sget(s, j, 0) + b * sget(s, j, 1);
+				out[i][1] += bt * sget(s, j, 0) - a * sget(s, j, 1);
+				out[i][2] += c * sget(s, j, 2) + d * sget(s, j, 3);
+				out[i][3] += dt * sget(s, j, 2) - c * sget(s, j, 3);
+			}
+		double *d_ = dst + (size_t)id * NSP;
+		for (int i = 0; i < DR; i++)
+			for (int sp = 0; sp < 4; sp++) {
+				cplx v = out[i][sp];
+				if (assign) v += sget(d_, i, sp);
+				sset(d_, i, sp, v);
+			}
+	}
+}
+
+/* clover_inv_kernel: library/dirac.cpp:109-225 */
+static void clover_inv_kernel(double mass, double *dst, const double *src, int part, int assign)
+{
+	int lo, hi;
+	or_compute_ldl_decomp(4.0 + mass);
+	part_range(part, &lo, &hi);
+#pragma omp parallel for
+	for (int id = lo; id < hi; id++) {
+		const double *s = src + (size_t)id * NSP;
+		cplx res[DR][4];
+		for (int half = 0; half < 2; half++) {
+			const cplx *ld = (half == 0 ? ldl_up : ldl_lo) + (size_t)id * NLDL;
+			int s0 = 2 * half, s1 = 2 * half + 1;
+			cplx x[NB];
+			for (int i = 0; i < DR; i++) { x[i] = sget(s, i, s0); x[i + DR] = sget(s, i, s1); }
+			for (int i = 1; i < NB; i++)
+				for (int k = 0; k < i; k++) x[i] -= ld[i * (i + 1) / 2 + k] * x[k];
+			for (int i = NB - 1; i >= 0; i--) {
+				x[i] /= creal(ld[i * (i + 1) / 2 + i]);
+				for (int k = i + 1; k < NB; k++) x[i] -= conj(ld[k * (k + 1) / 2 + i]) * x[k];
+			}
+			for (int i = 0; i < DR; i++) { res[i][s0] = x[i]; res[i][s1] = x[i + DR]; }
+		}
+		double *d_ = dst + (size_t)id * NSP;
+		for (int i = 0; i < DR; i++)
+			for (int sp = 0; sp < 4; sp++) {
+				cplx v = res[i][sp];
+				if (assign) v += sget(d_, i, sp);
+				sset(d_, i, sp, v);
+			}
+	}
+}
+
+/* ------------------------------------------------------------------ */
+/* dirac_kernel: library/dirac.cpp:227-263 with the per-direction spin  */
+/* projections of dirac.cpp:269-694.  For direction mu and orientation  */
+/* (forward: (1 - gamma_mu) U(x,mu) psi(x+mu); backward:                */
+/* (1 + gamma_mu) U(x-mu,mu)^+ psi(x-mu)) the projected half spinor is  */
+/*   h_k = psi_{2+k} + pc[k] * psi_{pi[k]}          (k = 0, 1)          */
+/* and after r = U h (or U^+ h) the four spin rows receive              */
+/*   sum_{2+k} += r_k,  sum_k += rc[k] * r_{ri[k]}                      */
+/* ------------------------------------------------------------------ */
+typedef struct { int pi[2]; cplx pc[2]; int ri[2]; cplx rc[2]; } proj_t;
+
+static proj_t proj_table(int mu, int fwd)
+{
+	proj_t p;
+	switch (mu * 2 + (fwd ? 0 : 1)) {
+	case 0: p = (proj_t){{0, 1}, {1, 1}, {0, 1}, {1, 1}}; break;        /* F0 dirac.cpp:269-319 */
+	case 1: p = (proj_t){{0, 1}, {-1, -1}, {0, 1}, {-1, -1}}; break;    /* B0 dirac.cpp:321-371 */
+	case 2: p = (proj_t){{1, 0}, {-I, -I}, {1, 0}, {I, I}}; break;      /* F1 dirac.cpp:373-423 */
+	case 3: p = (proj_t){{1, 0}, {I, I}, {1, 0}, {-I, -I}}; break;      /* B1 dirac.cpp:425-475 */
+	case 4: p = (proj_t){{1, 0}, {-1, 1}, {1, 0}, {1, -1}}; break;      /* F2 dirac.cpp:477-527 */
+	case 5: p = (proj_t){{1, 0}, {1, -1}, {1, 0}, {-1, 1}}; break;      /* B2 dirac.cpp:529-579 */
+	case 6: p = (proj_t){{0, 1}, {-I, I}, {0, 1}, {I, -I}}; break;      /* F3 dirac.cpp:581-631 */
+	default: p = (proj_t){{0, 1}, {I, -I}, {0, 1}, {-I, I}}; break;     /* B3 dirac.cpp:633-683 */
+	}
+	return p;
+}
+
+static void dirac_kernel(double *dst, const double *src, int part, double sign, int assign)
+{
+	int lo, hi;
+	mvm_counter++;
+	sign = -0.5 * sign;
+	part_range(part, &lo, &hi);
+	proj_t P[8];
+	for (int mu = 0; mu < 4; mu++) { P[2 * mu] = proj_table(mu, 1); P[2 * mu + 1] = proj_table(mu, 0); }
+	double *out = dst;
+	/* in-place use (dst == src, dirac.cpp:743) reads only the other parity */
+#pragma omp parallel for
+	for (int id = lo; id < hi; id++) {
+		cplx sum[DR][4];
+		for (int i = 0; i < DR; i++)
+			for (int s = 0; s < 4; s++) sum[i][s] = 0;
+		for (int mu = 0; mu < 4; mu++)
+			for (int f = 0; f < 2; f++) {
+				const proj_t *p = &P[2 * mu + f];
+				int nid = f == 0 ? nb_fw[4 * id + mu] : nb_bk[4 * id + mu];
+				int lid = f == 0 ? id : nid;
+				const double *ps = src + (size_t)nid * NSP;
+				const double *u = grepr + (4 * (size_t)lid + mu) * NLF;
+				cplx h[2][DR], r[2][DR];
+				for (int i = 0; i < DR; i++)
+					for (int k = 0; k < 2; k++) h[k][i] = sget(ps, i, 2 + k) + p->pc[k] * sget(ps, i, p->pi[k]);
+				for (int i = 0; i < DR; i++)
+					for (int k = 0; k < 2; k++) {
+						cplx acc = 0;
+						for (int j = 0; j < DR; j++) acc += (f == 0 ? fget(u, i, j) : conj(fget(u, j, i))) * h[k][j];
+						r[k][i] = acc;
+					}
+				for (int i = 0; i < DR; i++)
+					for (int k = 0; k < 2; k++) {
+						sum[i][2 + k] += r[k][i];
+						sum[i][k] += p->rc[k] * r[p->ri[k]][i];
+					}
+			}
+		double *d_ = out + (size_t)id * NSP;
+		for (int i = 0; i < DR; i++)
+			for (int s = 0; s < 4; s++) {
+				cplx v = sign * sum[i][s];
+				if (assign) v += sget(d_, i, s);
+				sset(d_, i, s, v);
+			}
+	}
+	apply_bcs_on_spinor(dst); /* dirac.cpp:700 */
+}
+
+/* ------------------------------------------------------------------ */
+/* operators: library/dirac.cpp:706-781                                 */
+/* ------------------------------------------------------------------ */
+void or_mvm_reset(void) { mvm_counter = 0; }
+int or_mvm_get(void) { return mvm_counter >> 1; }
+
+void or_Dphi_oe(double *d, double *s) { dirac_kernel(d, s, PART_ODD, 1, 0); }
+void or_Dphi_eo(double *d, double *s) { dirac_kernel(d, s, PART_EVEN, 1, 0); }
+void or_Dphi_ee(double m, double *d, double *s) { clover_kernel(m, d, s, PART_EVEN, 0); }
+void or_Dphi_oo(double m, double *d, double *s) { clover_kernel(m, d, s, PART_ODD, 0); }
+void or_Dphi_oo_inv(double m, double *d, double *s) { clover_inv_kernel(m, d, s, PART_ODD, 0); }
+
+void or_Dphi(double m, double *d, double *s)
+{
+	if (g_csw != 0.0) {
+		dirac_kernel(s, s, PART_ODD, -1, 0);
+		clover_inv_kernel(m, tmp_c, s, PART_ODD, 0);
+		dirac_kernel(d, tmp_c, PART_EVEN, 1, 0);
+		clover_kernel(m, d, s, PART_EVEN, 1);
+	} else {
+		/* dirac.cpp:755-765 */
+		dirac_kernel(s, s, PART_ODD, -1, 0);
+		dirac_kernel(d, s, PART_EVEN, 1, 0);
+		double f = (4.0 + m) * (4.0 + m);
+		for (size_t k = 0; k < (size_t)VH * NSP; k++) d[k] += f * s[k];
+	}
+}
+
+static void g5_range(double *s, int lo, int hi)
+{
+	for (int id = lo; id < hi; id++)
+		for (int i = 0; i < DR; i++)
+			for (int sp = 2; sp < 4; sp++) { SRE(s + (size_t)id * NSP, i, sp) *= -1; SIM(s + (size_t)id * NSP, i, sp) *= -1; }
+}
+
+void or_Hphi(double m, double *d, double *s)
+{
+	or_Dphi(m, d, s);
+	g5_range(d, 0, VH);
+}
+
+void or_Hphi_sq(double m, double *d, double *s)
+{
+	or_Hphi(m, tmp_b, s);
+	or_Hphi(m, d, tmp_b);
+}
+
+static void apply_mvm(int id, double m, double *d, double *s)
+{
+	if (id == MVM_DPHI) or_Dphi(m, d, s);
+	else if (id == MVM_HPHI) or_Hphi(m, d, s);
+	else or_Hphi_sq(m, d, s);
+}
+
+void or_mvm(int id, double m, double *d, double *s) { apply_mvm(id, m, d, s); }
+
+/* ------------------------------------------------------------------ */
+/* linalg: library/linalg.cpp (default selection = even half)          */
+/* ------------------------------------------------------------------ */
+#define NE ((size_t)VH * NSP)
+double or_sqnorm(const double *a)
+{
+	double r = 0;
+#pragma omp parallel for reduction(+ : r)
+	for (size_t k = 0; k < NE; k++) r += a[k] * a[k];
+	return r;
+}
+
+/* spinor_product: sum a * conj(b) (library/spinor.cpp:553-570) */
+static cplx product(const double *a, const double *b)
+{
+	double re = 0, im = 0;
+#pragma omp parallel for reduction(+ : re, im)
+	for (int id = 0; id < VH; id++) {
+		const double *x = a + (size_t)id * NSP, *y = b + (size_t)id * NSP;
+		for (int k = 0; k < 4 * DR; k++) {
+			re += x[k] * y[k] + x[4 * DR + k] * y[4 * DR + k];
+			im += x[4 * DR + k] * y[k] - x[k] * y[4 * DR + k];
+		}
+	}
+	return re + I * im;
+}
+
+void or_product(const double *a, const double *b, double *out)
+{
+	cplx c = product(a, b);
+	out[0] = creal(c);
+	out[1] = cimag(c);
+}
+
+double or_product_re(const double *a, const double *b)
+{
+	double r = 0;
+#pragma omp parallel for reduction(+ : r)
+	for (size_t k = 0; k < NE; k++) r += a[k] * b[k];
+	return r;
+}
+
+static void mulr_add(double *a, double c, const double *b)
+{
+#pragma omp parallel for
+	for (size_t k = 0; k < NE; k++) a[k] += c * b[k];
+}
+static void mulr(double *a, double c)
+{
+#pragma omp parallel for
+	for (size_t k = 0; k < NE; k++) a[k] *= c;
+}
+/* a += c * b with complex c on spinor components */
+static void mulc_add(double *a, cplx c, const double *b)
+{
+	double cr = creal(c), ci = cimag(c);
+#pragma omp parallel for
+	for (int id = 0; id < VH; id++) {
+		double *x = a + (size_t)id * NSP;
+		const double *y = b + (size_t)id * NSP;
+		for (int k = 0; k < 4 * DR; k++) {
+			double yr = y[k], yi = y[4 * DR + k];
+			x[k] += cr * yr - ci * yi;
+			x[4 * DR + k] += cr * yi + ci * yr;
+		}
+	}
+}
+static void mulc(double *a, cplx c)
+{
+	double cr = creal(c), ci = cimag(c);
+#pragma omp parallel for
+	for (int id = 0; id < VH; id++) {
+		double *x = a + (size_t)id * NSP;
+		for (int k = 0; k < 4 * DR; k++) {
+			double yr = x[k], yi = x[4 * DR + k];
+			x[k] = cr * yr - ci * yi;
+			x[4 * DR + k] = cr * yi + ci * yr;
+		}
+	}
+}
+static void copy_full(double *a, const double *b) { memcpy(a, b, sizeof(double) * (size_t)V * NSP); }
+
+void or_g5(double *s) { g5_range(s, 0, VH); }
+
+/* ------------------------------------------------------------------ */
+/* cg_inverter: library/cg.cpp:132-202                                 */
+/* ------------------------------------------------------------------ */
+int or_cg(int mvm, double mass, double prec, double *dptr, const double *sptr, double *resid_out)
+{
+	size_t n = (size_t)V * NSP;
+	double *r = calloc(n, sizeof(double)), *p = calloc(n, sizeof(double)), *q = calloc(n, sizeof(double));
+	int it = 0;
+	double alpha, beta, norm, atmp, btmp;
+	copy_full(p, dptr);
+	apply_mvm(mvm, mass, q, p);
+	copy_full(r, sptr);
+	mulr_add(r, -1.0, q);
+	copy_full(p, r);
+	norm = or_sqnorm(sptr);
+	btmp = or_sqnorm(r);
+	if (btmp < norm * prec) it = -1;
+	while (++it) {
+		apply_mvm(mvm, mass, q, p);
+		atmp = or_sqnorm(r);
+		btmp = or_product_re(p, q);
+		alpha = atmp / btmp;
+		mulr_add(dptr, alpha, p);
+		mulr_add(r, -alpha, q);
+		btmp = or_sqnorm(r);
+		beta = btmp / atmp;
+		if (btmp < norm * prec) break;
+		mulr(p, beta);
+		mulr_add(p, 1.0, r);
+	}
+	if (resid_out) *resid_out = btmp / norm;
+	free(r); free(p); free(q);
+	return it;
+}
+
+/* ------------------------------------------------------------------ */
+/* cg_mshift: library/cg.cpp:11-130 (dptr = num contiguous full fields) */
+/* ------------------------------------------------------------------ */
+int or_cg_mshift(int mvm, double mass, double prec, int num, const double *shift, double *dptr, const double *sptr)
+{
+	size_t n = (size_t)V * NSP;
+	double *r = calloc(n, sizeof(double)), *p = calloc(n, sizeof(double)), *q = calloc(n, sizeof(double));
+	double *k = calloc(n * num, sizeof(double));
+	double c1[64], c2[64], c3[64];
+	int cflag[64];
+	int it = 0, notconverged;
+	double alpha, beta, beta_old, atmp, btmp = 0, norm;
+	copy_full(r, sptr);
+	copy_full(p, sptr);
+	for (int s = 0; s < num; s++) {
+		memset(dptr + s * n, 0, sizeof(double) * n);
+		copy_full(k + s * n, sptr);
+		c1[s] = 1; c2[s] = 1; cflag[s] = 0;
+	}
+	beta = 1;
+	alpha = 0;
+	norm = or_sqnorm(sptr);
+	while (++it) {
+		apply_mvm(mvm, mass, q, p);
+		atmp = or_sqnorm(r);
+		btmp = or_product_re(p, q);
+		beta_old = beta;
+		beta = -(atmp / btmp);
+		for (int s = 0; s < num; s++) {
+			if (cflag[s]) continue;
+			c3[s] = c2[s] * c1[s] * beta_old;
+			c3[s] /= (beta * alpha * (c1[s] - c2[s])) + (c1[s] * beta_old * (1 - shift[s] * beta));
+			double bt = beta * (c3[s] / c2[s]);
+			mulr_add(dptr + s * n, -bt, k + s * n);
+		}
+		mulr_add(r, beta, q);
+		btmp = or_sqnorm(r);
+		alpha = btmp / atmp;
+		notconverged = 0;
+		for (int s = 0; s < num; s++) {
+			if (cflag[s]) continue;
+			double at = alpha * (c3[s] * c3[s]) / (c2[s] * c2[s]);
+			mulr(k + s * n, at);
+			mulr_add(k + s * n, c3[s], r);
+			if (btmp * c3[s] * c3[s] < norm * prec) cflag[s] = 1;
+			else notconverged++;
+			c1[s] = c2[s];
+			c2[s] = c3[s];
+		}
+		if (notconverged == 0) break;
+		mulr(p, alpha);
+		mulr_add(p, 1.0, r);
+	}
+	free(r); free(p); free(q); free(k);
+	return it;
+}
+
+/* ------------------------------------------------------------------ */
+/* bicgstab_inverter: library/bicgstab.cpp:9-97                         */
+/* ------------------------------------------------------------------ */
+int or_bicgstab(int mvm, double mass, double prec, double *dptr, const double *sptr, double *resid_out)
+{
+	size_t n = (size_t)V * NSP;
+	double *r = calloc(n, sizeof(double)), *r0 = calloc(n, sizeof(double)), *v = calloc(n, sizeof(double));
+	double *p = calloc(n, sizeof(double)), *s = calloc(n, sizeof(double)), *t = calloc(n, sizeof(double));
+	int it = 0;
+	double norm, residue;
+	cplx alpha, beta, rho0, rho1, omega;
+	(void)s;
+	apply_mvm(mvm, mass, p, dptr);
+	copy_full(r, sptr);
+	mulr_add(r, -1.0, p);
+	copy_full(r0, r);
+	copy_full(p, r);
+	norm = or_sqnorm(sptr);
+	rho0 = product(r, r0);
+	residue = creal(rho0);
+	if (residue < norm * prec) it = -1;
+	while (++it) {
+		apply_mvm(mvm, mass, v, p);
+		alpha = product(v, r0);
+		alpha = rho0 / alpha;
+		mulc_add(r, -alpha, v);
+		apply_mvm(mvm, mass, t, r);
+		omega = product(r, t);
+		omega /= or_sqnorm(t);
+		mulc_add(dptr, alpha, p);
+		mulc_add(dptr, omega, r);
+		mulc_add(r, -omega, t);
+		rho1 = product(r, r0);
+		beta = (rho1 / rho0) * (alpha / omega);
+		mulc(p, beta);
+		mulr_add(p, 1.0, r);
+		mulc_add(p, -(beta * omega), v);
+		residue = or_sqnorm(r);
+		if (residue < norm * prec) break;
+		rho0 = rho1;
+	}
+	if (resid_out) *resid_out = residue / norm;
+	free(r); free(r0); free(v); free(p); free(s); free(t);
+	return 2 * it;
+}
+
+/* auto_inverter: library/inverter.cpp:9-55 (mvm chosen by identity) */
+int or_auto_inverter(int mvm, double mass, double prec, double *dptr, double *sptr)
+{
+	int iter = 0;
+	if (mvm == MVM_HPHI_SQ) {
+		or_g5(sptr);
+		iter += or_bicgstab(MVM_DPHI, mass, prec, tmp_a, sptr, NULL);
+		or_g5(tmp_a);
+		iter += or_bicgstab(MVM_DPHI, mass, prec, dptr, tmp_a, NULL);
+		or_g5(sptr);
+	} else if (mvm == MVM_HPHI) {
+		or_g5(sptr);
+		iter += or_bicgstab(MVM_DPHI, mass, prec, dptr, sptr, NULL);
+		or_g5(sptr);
+	} else {
+		iter += or_bicgstab(MVM_DPHI, mass, prec, dptr, sptr, NULL);
+	}
+	return iter;
+}
+
+void or_zero_auto_scratch(void) { memset(tmp_a, 0, sizeof(double) * (size_t)V * NSP); }
