@@ -1,0 +1,186 @@
+"""Benchmark: EO Wilson-clover Dslash bandwidth and CG time-to-solution on
+one MI355X (32^4, SU(3) fundamental clover, c_sw = 1, m = -0.75,
+antiperiodic time).
+
+A "step" is one hop D_eo over the V/2 = 524,288 even sites of the 32^4
+lattice with all inputs resident in HBM.  `value` is the achieved HBM
+bandwidth of that kernel under the compulsory-traffic model (each input
+spinor, each link and each output spinor moves once per hop), measured with
+HIP events on the stream the kernel runs on.  The line also carries the
+other precisions, the CG time-to-solution (plain FP64 vs mixed FP32 / FP16
+reliable-update CG to an FP64 true residual of 1e-10), the roofline object
+and a CPU baseline (the oracle on the host cores).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+DIMS = (32, 32, 32, 32)
+MASS = -0.75
+CSW = 1.0
+# compulsory bytes per output site of one hop, D_R = 3:
+#   1 input spinor + 8 links + 1 output spinor
+BYTES_PER_SITE = {
+    ("f64", 0): 192 + 8 * 144 + 192,   # 1536
+    ("f64", 1): 192 + 8 * 96 + 192,    # 1152
+    ("f32", 1): 96 + 8 * 48 + 96,      # 576
+    ("f16", 1): 52 + 8 * 24 + 52,      # 296 (48 B half spinor + 4 B scale)
+}
+FLOP_PER_SITE = 1320
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(threads, budget_s=12.0):
+    """The oracle's hop (plain C + OpenMP, FP64, 18-real links) on a 16^4
+    sample of the same workload, timed on the host cores."""
+    import oracle
+    from yahmc_amd.synthetic import gaussian_source, random_gauge
+    dims = (16, 16, 16, 16)
+    o = oracle.Oracle("su3_fund").init(dims, bc=oracle.BC_ANTIPERIODIC, csw=0.0)
+    o.set_threads(threads)
+    o.set_gauge(random_gauge(dims, 3, seed=1))
+    o.represent()
+    s = gaussian_source(dims, 3, seed=2, parity="full")
+    d = np.zeros_like(s)
+    o.Dphi_eo(d, s)
+    n = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < budget_s:
+        o.Dphi_eo(d, s)
+        n += 1
+    dt = (time.perf_counter() - t0) / n
+    vh = o.VH
+    return {"value": round(BYTES_PER_SITE[("f64", 0)] * vh / dt / 1e9, 3), "unit": "GB/s", "cores": threads,
+            "kind": "port",
+            "sample": f"oracle Dphi_eo (FP64, 18-real links) on a 16^4 lattice, {n} applications in {budget_s:.0f} s, "
+                      f"{dt * 1e3:.2f} ms each; same compulsory-bytes model as the GPU value"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--prec", default="f32", choices=["f64", "f32", "f16"], help="precision of the headline hop")
+    ap.add_argument("--skip-cg", action="store_true")
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--only-hop", action="store_true", help="time the headline hop only (for profiling)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import yahmc_amd as yh
+    from yahmc_amd.synthetic import gaussian_source, random_gauge
+
+    t0 = time.perf_counter()
+    links = random_gauge(DIMS, 3, seed=32244)
+    src = gaussian_source(DIMS, 3, seed=32245, parity="full")
+    log(f"synthetic inputs {time.perf_counter() - t0:.1f}s")
+    lat = yh.Lattice(DIMS, nc=3, repr="fundamental", bc="antiperiodic", csw=CSW)
+    lat.set_gauge(links)
+    log("gauge field, representation and clover term built on device")
+
+    hop = {}
+    variants = [(args.prec, 0 if args.prec == "f64" else 1)] if args.only_hop else \
+        [("f64", 0), ("f64", 1), ("f32", 1), ("f16", 1)]
+    for prec, recon in variants:
+        fs = lat.field(prec).upload(src)
+        fd = lat.field(prec)
+        lat.bench_hop(prec, recon, max(1, args.warmup), fd, fs)
+        ms = lat.bench_hop(prec, recon, args.steps, fd, fs)
+        b = BYTES_PER_SITE[(prec, recon)] * lat.VH
+        hop[f"{prec}_r{18 if recon == 0 else 12}"] = {
+            "us_per_hop": round(ms * 1e3, 2), "GB_s": round(b / (ms * 1e-3) / 1e9, 1),
+            "frac_hbm": round(b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 3),
+            "GFLOP_s": round(FLOP_PER_SITE * lat.VH / (ms * 1e-3) / 1e9, 1), "bytes_per_site": BYTES_PER_SITE[(prec, recon)]}
+        log(f"hop {prec} recon{18 if recon == 0 else 12}: {hop[list(hop)[-1]]}")
+        fs.free()
+        fd.free()
+    key = f"{args.prec}_r{18 if args.prec == 'f64' else 12}"
+    head = hop[key]
+    copy_ms = lat.bench_copy(1 << 30, 20)
+    copy_gbs = 2 * (1 << 30) / (copy_ms * 1e-3) / 1e9
+    log(f"copy kernel: {copy_gbs:.0f} GB/s")
+
+    cg = {}
+    if not args.skip_cg and not args.only_hop:
+        b = gaussian_source(DIMS, 3, seed=32246)
+        fb = lat.field("f64").upload(b)
+        prec = 1e-20
+        for name, fn in (("fp64", lambda fx: (lat.cg_inverter(lat.inv_par(MASS, prec, "Hphi_sq"), fx, fb), 0)),
+                         ("mixed_f32", lambda fx: lat.cg_mixed(lat.inv_par(MASS, prec, "Hphi_sq", sloppy="f32"), fx, fb)),
+                         ("mixed_f16", lambda fx: lat.cg_mixed(lat.inv_par(MASS, prec, "Hphi_sq", sloppy="f16"), fx, fb))):
+            fx = lat.field("f64")
+            lat.synchronize()
+            t = time.perf_counter()
+            it, upd = fn(fx)
+            lat.synchronize()
+            dt = time.perf_counter() - t
+            # FP64 true residual with the FP64 GPU operator
+            ax = lat.field("f64")
+            lat.Hphi_sq(MASS, ax, fx)
+            lat.spinor_mulr_add_assign(ax, -1.0, fb)
+            res = lat.spinor_sqnorm(ax) / lat.spinor_sqnorm(fb)
+            cg[name] = {"seconds": round(dt, 4), "iterations": it, "reliable_updates": upd,
+                        "true_rel_residual": float(np.sqrt(res))}
+            log(f"cg {name}: {cg[name]}")
+            fx.free()
+            ax.free()
+        if "fp64" in cg:
+            for k in ("mixed_f32", "mixed_f16"):
+                if k in cg:
+                    cg[k]["speedup_vs_fp64"] = round(cg["fp64"]["seconds"] / cg[k]["seconds"], 2)
+
+    cpu = None
+    if not args.skip_cpu and not args.only_hop:
+        threads = min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(threads)
+        log(f"cpu baseline: {cpu}")
+
+    ms = head["us_per_hop"] / 1e3
+    algo_bytes = BYTES_PER_SITE[(args.prec, 0 if args.prec == "f64" else 1)] * lat.VH
+    line = {
+        "metric": "Dslash GB/s (% HBM peak); CG time to 1e-10, mixed+learned vs FP64, 32^4",
+        "value": head["GB_s"],
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"f64": "f64", "f32": "f32", "f16": "f16-storage/f32-arith"}[args.prec],
+        "data": "synthetic (seeded random SU(3) links, Gaussian source)",
+        "config": {"workload": f"hop D_eo, {args.prec} recon-{18 if args.prec == 'f64' else 12}, 32^4 SU(3) fundamental "
+                               "clover, antiperiodic time",
+                   "lattice": list(DIMS), "sites_per_step": lat.VH, "mass": MASS, "csw": CSW,
+                   "gauge": "random SU(3) links (hot); the 10-trajectory pure-gauge smoothing is not applied"},
+        "roofline": {"bound": "hbm", "achieved": head["GB_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(head["GB_s"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes_per_launch": algo_bytes},
+        "cpu_baseline": cpu,
+        "dslash": hop,
+        "copy_kernel_GB_s": round(copy_gbs, 1),
+        "cg_time_to_1e-10": cg,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,178 @@
+/*
+ * yahmc.h — C ABI of the MI355X-native YaHMC fermion hot path.
+ *
+ * Plain pointers, sizes and integer status codes; no C++ or torch types.
+ * Each entry point names the reference interface it replaces
+ * (paths relative to the reference repository root).
+ *
+ * Host-side field formats are the reference's own, so a caller converts
+ * nothing:
+ *   spinor field   V sites x Spinor{re[DR][4], im[DR][4]} doubles, in the
+ *                  reference's EO site order, both parities
+ *                  (include/spinor.h:7-9, include/spinorfield.h:6-39,
+ *                  library/geometry.cpp:756-878)
+ *   gauge field    4V suNg{re[NC*NC], im[NC*NC]} doubles, index 4*id+mu
+ *                  (include/gaugefield.h:6-18)
+ * Device fields live in HBM in the library's own layout (see DESIGN.md of
+ * the project) and are referred to by opaque handles.
+ *
+ * All calls are synchronous with respect to the host unless stated, run on
+ * the context's HIP stream, and return YH_OK (0) or a negative YH_ERR_*.
+ * A context is not reentrant: one host thread per context.
+ */
+#ifndef YAHMC_H
+#define YAHMC_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define YH_OK 0
+#define YH_ERR_ARG -1        /* invalid argument */
+#define YH_ERR_HIP -2        /* HIP runtime error (see yh_last_error) */
+#define YH_ERR_STATE -3      /* call out of order (e.g. no gauge field) */
+#define YH_ERR_NOCONV -4     /* solver hit max_iter without converging */
+#define YH_ERR_NONFINITE -5  /* NaN or Inf in a solver scalar */
+#define YH_ERR_UNSUPPORTED -6
+
+/* representations: the reference's REPR_ID values (include/repr.h:9-39) */
+#define YH_REPR_FUNDAMENTAL 0
+#define YH_REPR_ADJOINT 1
+#define YH_REPR_SYMMETRIC 2
+#define YH_REPR_ANTISYMMETRIC 3
+
+/* fermion boundary conditions in time (include/bcs.h:1-31) */
+#define YH_BC_PERIODIC 0
+#define YH_BC_ANTIPERIODIC 1
+#define YH_BC_OPEN 2
+#define YH_BC_SF 3
+
+/* storage precisions of device spinor fields */
+#define YH_PREC_F64 0
+#define YH_PREC_F32 1
+#define YH_PREC_F16 2
+
+/* parity masks (include/global.h:10-13) */
+#define YH_EVEN 1
+#define YH_ODD 2
+#define YH_BOTH 3
+
+/* operator selectors (the reference passes function pointers of type
+ * mvm_operator, include/inverter.h:6) */
+#define YH_MVM_DPHI 0
+#define YH_MVM_HPHI 1
+#define YH_MVM_HPHI_SQ 2
+
+typedef struct yh_ctx yh_ctx;
+typedef struct yh_field yh_field;
+
+typedef struct {
+  int dims[4];     /* T, X, Y, Z; all even (EO order needs Z even) */
+  int nc;          /* colours: 2, 3 or 4 */
+  int repr;        /* YH_REPR_* */
+  int bc;          /* YH_BC_* */
+  double csw;      /* clover coefficient; 0 disables the clover term */
+  double cf;       /* boundary improvement c_f for open / SF */
+  int device;      /* HIP device ordinal */
+} yh_lattice;
+
+/* solver parameters: mirrors inv_par (include/inverter.h:8-16) plus the
+ * controls the reference lacks (iteration cap, precision ladder) */
+typedef struct {
+  double mass;
+  double prec;        /* squared relative residual |r|^2 < prec |b|^2 */
+  int max_iter;       /* 0 = no cap (the reference has none) */
+  int mvm;            /* YH_MVM_* */
+  int sloppy_prec;    /* mixed solvers: YH_PREC_F32 or YH_PREC_F16 */
+  int reliable_every; /* mixed CG: FP64 reliable update every N iterations */
+  double delta;       /* mixed CG: reliable update when |r| drops by delta */
+  int log;
+} yh_inv_par;
+
+/* --- context / lattice ---------------------------------------------------- */
+/* geometry_init + repr_init + clover_init + bcs_init
+ * (library/geometry.cpp:1032-1191, library/repr.cpp:17-105,
+ *  library/clover.cpp:349-361, library/bcs.cpp:11-137) */
+int yh_create(const yh_lattice* lat, yh_ctx** out);
+int yh_destroy(yh_ctx* ctx);
+const char* yh_last_error(void);
+int yh_version(void);
+int yh_repr_dim(const yh_ctx* ctx);
+long yh_volume(const yh_ctx* ctx);
+
+/* --- gauge field ----------------------------------------------------------- */
+/* upload gfield_fund then represent_gauge_field: represent the links,
+ * build the clover term, apply the represented-field BCs
+ * (library/repr.cpp:265-298, library/clover.cpp:330-347,
+ *  library/bcs.cpp:264-272).  `links` is 4V suNg in the reference layout,
+ * with the gauge-field BCs (bcs.cpp:201-262) already applied. */
+int yh_set_gauge(yh_ctx* ctx, const double* links);
+/* read back gfield_repr (with the antiperiodic sign) and cl_term in the
+ * reference layouts (include/gaugefield.h:9-10, include/clover.h:12-22) */
+int yh_get_gauge_repr(yh_ctx* ctx, double* out);
+int yh_get_clover_term(yh_ctx* ctx, double* out);
+/* clover_logdet (library/clover.cpp:281-305) */
+int yh_clover_logdet(yh_ctx* ctx, double mass, double* out);
+
+/* --- spinor fields (library/linalg.cpp:10-27, include/spinorfield.h) ------- */
+int yh_field_create(yh_ctx* ctx, int prec, yh_field** out);
+int yh_field_destroy(yh_field* f);
+/* host <-> device in the reference Spinor layout; only the parities in
+ * `parity_mask` are transferred (host array always holds all V sites) */
+int yh_field_upload(yh_ctx* ctx, yh_field* f, const double* host, int parity_mask);
+int yh_field_download(yh_ctx* ctx, const yh_field* f, double* host, int parity_mask);
+/* spinor_zero (library/linalg.cpp:29-40) */
+int yh_field_zero(yh_ctx* ctx, yh_field* f, int parity_mask);
+/* spinor_copy with precision conversion (library/linalg.cpp:71-76) */
+int yh_field_copy(yh_ctx* ctx, yh_field* dst, const yh_field* src, int parity_mask);
+
+/* --- operators (include/dirac.h:6-16, library/dirac.cpp) ------------------- */
+void yh_mvm_reset(yh_ctx* ctx);
+int yh_mvm_get(yh_ctx* ctx);
+int yh_Dphi_oe(yh_ctx* ctx, yh_field* dst, const yh_field* src);
+int yh_Dphi_eo(yh_ctx* ctx, yh_field* dst, const yh_field* src);
+int yh_Dphi_ee(yh_ctx* ctx, double mass, yh_field* dst, const yh_field* src);
+int yh_Dphi_oo(yh_ctx* ctx, double mass, yh_field* dst, const yh_field* src);
+int yh_Dphi_oo_inv(yh_ctx* ctx, double mass, yh_field* dst, const yh_field* src);
+/* Dphi writes -D_oe src_e into the odd half of src, as the reference does
+ * (library/dirac.cpp:738-751); Hphi = gamma5 Dphi; Hphi_sq = Hphi Hphi */
+int yh_Dphi(yh_ctx* ctx, double mass, yh_field* dst, yh_field* src);
+int yh_Hphi(yh_ctx* ctx, double mass, yh_field* dst, yh_field* src);
+int yh_Hphi_sq(yh_ctx* ctx, double mass, yh_field* dst, yh_field* src);
+
+/* --- linear algebra on the even half (include/linalg.h:7-34) --------------- */
+int yh_spinor_sqnorm(yh_ctx* ctx, const yh_field* a, double* out);
+int yh_spinor_product(yh_ctx* ctx, const yh_field* a, const yh_field* b, double* out_re_im);
+int yh_spinor_product_re(yh_ctx* ctx, const yh_field* a, const yh_field* b, double* out);
+int yh_spinor_g5(yh_ctx* ctx, yh_field* a);
+int yh_spinor_mulr_add_assign(yh_ctx* ctx, yh_field* a, double r, const yh_field* b);
+int yh_spinor_mulc_add_assign(yh_ctx* ctx, yh_field* a, double re, double im, const yh_field* b);
+
+/* --- solvers (include/inverter.h:18-22) ------------------------------------ */
+/* cg_inverter (library/cg.cpp:132-202): dst holds the initial guess */
+int yh_cg(yh_ctx* ctx, const yh_inv_par* par, yh_field* dst, const yh_field* src, int* iters);
+/* bicgstab_inverter (library/bicgstab.cpp:9-97); returns 2 x iterations */
+int yh_bicgstab(yh_ctx* ctx, const yh_inv_par* par, yh_field* dst, const yh_field* src, int* iters);
+/* cg_mshift (library/cg.cpp:11-130): (mvm + shift[n]) dst[n] = src */
+int yh_cg_mshift(yh_ctx* ctx, const yh_inv_par* par, int num, const double* shift, yh_field** dst,
+                 const yh_field* src, int* iters);
+/* project-new: FP64 reliable-update CG with an FP32 / FP16 inner operator;
+ * reaches the same FP64 true residual as yh_cg.  iters = sloppy iterations,
+ * updates = FP64 reliable updates */
+int yh_cg_mixed(yh_ctx* ctx, const yh_inv_par* par, yh_field* dst, const yh_field* src, int* iters, int* updates);
+
+/* --- measurement support ---------------------------------------------------- */
+/* time `reps` back-to-back hop launches (Dphi_eo shape) of the given
+ * precision / recon on the context stream with HIP events; returns the
+ * average milliseconds per launch */
+int yh_bench_hop(yh_ctx* ctx, int prec, int recon, int reps, yh_field* dst, const yh_field* src, double* ms);
+/* time `reps` plain copies of `bytes` bytes with a vector copy kernel */
+int yh_bench_copy(yh_ctx* ctx, long bytes, int reps, double* ms);
+int yh_synchronize(yh_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

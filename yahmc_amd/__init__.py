@@ -1,0 +1,334 @@
+"""yahmc_amd — MI355X-native fermion hot path of YaHMC.
+
+Python host mirror of the reference's call surface (include/dirac.h,
+include/inverter.h, include/linalg.h, include/clover.h) over the C ABI in
+include/yahmc.h.  Every compute call runs in libyahmc.so's HIP kernels; if
+the library is missing, import of the native layer raises immediately —
+there is no CPU fallback.
+
+Host arrays use the reference's layouts: spinor fields (V, 8*DR) float64
+(Spinor re[DR][4] then im[DR][4], EO site order), fundamental links
+(4V, 2*NC*NC) float64 (suNg re-plane then im-plane, index 4*id+mu).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "lib", "libyahmc.so")
+
+YH_OK = 0
+ERRORS = {-1: "invalid argument", -2: "HIP error", -3: "bad state", -4: "not converged", -5: "non-finite",
+          -6: "unsupported"}
+
+REPR = {"fundamental": 0, "adjoint": 1, "symmetric": 2, "antisymmetric": 3}
+BC = {"periodic": 0, "antiperiodic": 1, "open": 2, "sf": 3}
+PREC = {"f64": 0, "f32": 1, "f16": 2}
+MVM = {"Dphi": 0, "Hphi": 1, "Hphi_sq": 2}
+EVEN, ODD, BOTH = 1, 2, 3
+RECON_18, RECON_12 = 0, 1
+
+
+class YahmcError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class _Lattice(ctypes.Structure):
+    _fields_ = [("dims", ctypes.c_int * 4), ("nc", ctypes.c_int), ("repr", ctypes.c_int), ("bc", ctypes.c_int),
+                ("csw", ctypes.c_double), ("cf", ctypes.c_double), ("device", ctypes.c_int)]
+
+
+class InvPar(ctypes.Structure):
+    """yh_inv_par: mirrors inv_par (include/inverter.h:8-16)."""
+    _fields_ = [("mass", ctypes.c_double), ("prec", ctypes.c_double), ("max_iter", ctypes.c_int),
+                ("mvm", ctypes.c_int), ("sloppy_prec", ctypes.c_int), ("reliable_every", ctypes.c_int),
+                ("delta", ctypes.c_double), ("log", ctypes.c_int)]
+
+
+_lib = None
+
+_VP = ctypes.c_void_p
+_D = ctypes.POINTER(ctypes.c_double)
+_PI = ctypes.POINTER(ctypes.c_int)
+
+_SIGS = {
+    "yh_create": ([ctypes.POINTER(_Lattice), ctypes.POINTER(_VP)], ctypes.c_int),
+    "yh_destroy": ([_VP], ctypes.c_int),
+    "yh_last_error": ([], ctypes.c_char_p),
+    "yh_version": ([], ctypes.c_int),
+    "yh_repr_dim": ([_VP], ctypes.c_int),
+    "yh_volume": ([_VP], ctypes.c_long),
+    "yh_set_gauge": ([_VP, _D], ctypes.c_int),
+    "yh_get_gauge_repr": ([_VP, _D], ctypes.c_int),
+    "yh_get_clover_term": ([_VP, _D], ctypes.c_int),
+    "yh_clover_logdet": ([_VP, ctypes.c_double, _D], ctypes.c_int),
+    "yh_field_create": ([_VP, ctypes.c_int, ctypes.POINTER(_VP)], ctypes.c_int),
+    "yh_field_destroy": ([_VP], ctypes.c_int),
+    "yh_field_upload": ([_VP, _VP, _D, ctypes.c_int], ctypes.c_int),
+    "yh_field_download": ([_VP, _VP, _D, ctypes.c_int], ctypes.c_int),
+    "yh_field_zero": ([_VP, _VP, ctypes.c_int], ctypes.c_int),
+    "yh_field_copy": ([_VP, _VP, _VP, ctypes.c_int], ctypes.c_int),
+    "yh_mvm_reset": ([_VP], None),
+    "yh_mvm_get": ([_VP], ctypes.c_int),
+    "yh_Dphi_oe": ([_VP, _VP, _VP], ctypes.c_int),
+    "yh_Dphi_eo": ([_VP, _VP, _VP], ctypes.c_int),
+    "yh_Dphi_ee": ([_VP, ctypes.c_double, _VP, _VP], ctypes.c_int),
+    "yh_Dphi_oo": ([_VP, ctypes.c_double, _VP, _VP], ctypes.c_int),
+    "yh_Dphi_oo_inv": ([_VP, ctypes.c_double, _VP, _VP], ctypes.c_int),
+    "yh_Dphi": ([_VP, ctypes.c_double, _VP, _VP], ctypes.c_int),
+    "yh_Hphi": ([_VP, ctypes.c_double, _VP, _VP], ctypes.c_int),
+    "yh_Hphi_sq": ([_VP, ctypes.c_double, _VP, _VP], ctypes.c_int),
+    "yh_spinor_sqnorm": ([_VP, _VP, _D], ctypes.c_int),
+    "yh_spinor_product": ([_VP, _VP, _VP, _D], ctypes.c_int),
+    "yh_spinor_product_re": ([_VP, _VP, _VP, _D], ctypes.c_int),
+    "yh_spinor_g5": ([_VP, _VP], ctypes.c_int),
+    "yh_spinor_mulr_add_assign": ([_VP, _VP, ctypes.c_double, _VP], ctypes.c_int),
+    "yh_spinor_mulc_add_assign": ([_VP, _VP, ctypes.c_double, ctypes.c_double, _VP], ctypes.c_int),
+    "yh_cg": ([_VP, ctypes.POINTER(InvPar), _VP, _VP, _PI], ctypes.c_int),
+    "yh_bicgstab": ([_VP, ctypes.POINTER(InvPar), _VP, _VP, _PI], ctypes.c_int),
+    "yh_cg_mshift": ([_VP, ctypes.POINTER(InvPar), ctypes.c_int, _D, ctypes.POINTER(_VP), _VP, _PI], ctypes.c_int),
+    "yh_cg_mixed": ([_VP, ctypes.POINTER(InvPar), _VP, _VP, _PI, _PI], ctypes.c_int),
+    "yh_bench_hop": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP, _VP, _D], ctypes.c_int),
+    "yh_bench_copy": ([_VP, ctypes.c_long, ctypes.c_int, _D], ctypes.c_int),
+    "yh_synchronize": ([_VP], ctypes.c_int),
+}
+
+
+def exported_symbols():
+    return sorted(_SIGS)
+
+
+def lib():
+    """Load libyahmc.so (built in-tree by yahmc_amd/build.py).  Raises if it
+    is missing: the HIP path is the only implementation."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise ImportError(f"{LIBPATH} is missing: build it with `python -m yahmc_amd.build` "
+                              "(there is no CPU fallback)")
+        L = ctypes.CDLL(LIBPATH)
+        for name, (args, res) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != YH_OK:
+        raise YahmcError(rc, lib().yh_last_error().decode())
+
+
+def _dp(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_D)
+
+
+class Field:
+    """A device spinor field (both parities) in HBM."""
+
+    def __init__(self, lat, prec="f64"):
+        self.lat = lat
+        self.prec = PREC[prec] if isinstance(prec, str) else prec
+        h = _VP()
+        _check(lib().yh_field_create(lat.ctx, self.prec, ctypes.byref(h)))
+        self.h = h
+
+    def upload(self, host, mask=BOTH):
+        host = np.ascontiguousarray(host, dtype=np.float64)
+        assert host.shape == (self.lat.V, 8 * self.lat.dr)
+        _check(lib().yh_field_upload(self.lat.ctx, self.h, _dp(host), mask))
+        return self
+
+    def download(self, mask=BOTH):
+        out = np.zeros((self.lat.V, 8 * self.lat.dr))
+        _check(lib().yh_field_download(self.lat.ctx, self.h, _dp(out), mask))
+        return out
+
+    def zero(self, mask=BOTH):
+        _check(lib().yh_field_zero(self.lat.ctx, self.h, mask))
+
+    def copy_from(self, other, mask=BOTH):
+        _check(lib().yh_field_copy(self.lat.ctx, self.h, other.h, mask))
+
+    def free(self):
+        if self.h:
+            lib().yh_field_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Lattice:
+    """One lattice context on one GPU (geometry_init + repr_init +
+    clover_init + bcs_init of the reference)."""
+
+    def __init__(self, dims, nc=3, repr="fundamental", bc="antiperiodic", csw=1.0, cf=1.0, device=0):
+        lt = _Lattice()
+        for i in range(4):
+            lt.dims[i] = int(dims[i])
+        lt.nc = nc
+        lt.repr = REPR[repr] if isinstance(repr, str) else repr
+        lt.bc = BC[bc] if isinstance(bc, str) else bc
+        lt.csw = csw
+        lt.cf = cf
+        lt.device = device
+        h = _VP()
+        _check(lib().yh_create(ctypes.byref(lt), ctypes.byref(h)))
+        self.ctx = h
+        self.dims = tuple(int(d) for d in dims)
+        self.nc = nc
+        self.dr = lib().yh_repr_dim(h)
+        self.V = int(np.prod(self.dims))
+        self.VH = self.V // 2
+
+    def close(self):
+        if self.ctx:
+            lib().yh_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- gauge ---------------------------------------------------------
+    def set_gauge(self, links):
+        links = np.ascontiguousarray(links, dtype=np.float64)
+        assert links.size == 4 * self.V * 2 * self.nc ** 2
+        _check(lib().yh_set_gauge(self.ctx, _dp(links)))
+
+    def gauge_repr(self):
+        out = np.zeros((4 * self.V, 2 * self.dr ** 2))
+        _check(lib().yh_get_gauge_repr(self.ctx, _dp(out)))
+        return out
+
+    def clover_term(self):
+        out = np.zeros((4 * self.V, 2 * self.dr ** 2))
+        _check(lib().yh_get_clover_term(self.ctx, _dp(out)))
+        return out
+
+    def clover_logdet(self, mass):
+        out = np.zeros(1)
+        _check(lib().yh_clover_logdet(self.ctx, mass, _dp(out)))
+        return out[0]
+
+    def field(self, prec="f64"):
+        return Field(self, prec)
+
+    # ---- operators (include/dirac.h) ----------------------------------
+    def mvm_reset(self):
+        lib().yh_mvm_reset(self.ctx)
+
+    def mvm_get(self):
+        return lib().yh_mvm_get(self.ctx)
+
+    def Dphi_oe(self, d, s):
+        _check(lib().yh_Dphi_oe(self.ctx, d.h, s.h))
+
+    def Dphi_eo(self, d, s):
+        _check(lib().yh_Dphi_eo(self.ctx, d.h, s.h))
+
+    def Dphi_ee(self, m, d, s):
+        _check(lib().yh_Dphi_ee(self.ctx, m, d.h, s.h))
+
+    def Dphi_oo(self, m, d, s):
+        _check(lib().yh_Dphi_oo(self.ctx, m, d.h, s.h))
+
+    def Dphi_oo_inv(self, m, d, s):
+        _check(lib().yh_Dphi_oo_inv(self.ctx, m, d.h, s.h))
+
+    def Dphi(self, m, d, s):
+        _check(lib().yh_Dphi(self.ctx, m, d.h, s.h))
+
+    def Hphi(self, m, d, s):
+        _check(lib().yh_Hphi(self.ctx, m, d.h, s.h))
+
+    def Hphi_sq(self, m, d, s):
+        _check(lib().yh_Hphi_sq(self.ctx, m, d.h, s.h))
+
+    # ---- linalg (include/linalg.h) -----------------------------------
+    def spinor_sqnorm(self, a):
+        out = np.zeros(1)
+        _check(lib().yh_spinor_sqnorm(self.ctx, a.h, _dp(out)))
+        return out[0]
+
+    def spinor_product(self, a, b):
+        out = np.zeros(2)
+        _check(lib().yh_spinor_product(self.ctx, a.h, b.h, _dp(out)))
+        return complex(out[0], out[1])
+
+    def spinor_product_re(self, a, b):
+        out = np.zeros(1)
+        _check(lib().yh_spinor_product_re(self.ctx, a.h, b.h, _dp(out)))
+        return out[0]
+
+    def spinor_g5(self, a):
+        _check(lib().yh_spinor_g5(self.ctx, a.h))
+
+    def spinor_mulr_add_assign(self, a, r, b):
+        _check(lib().yh_spinor_mulr_add_assign(self.ctx, a.h, r, b.h))
+
+    def spinor_mulc_add_assign(self, a, c, b):
+        _check(lib().yh_spinor_mulc_add_assign(self.ctx, a.h, c.real, c.imag, b.h))
+
+    # ---- solvers (include/inverter.h) --------------------------------
+    @staticmethod
+    def inv_par(mass, prec, mvm="Hphi_sq", max_iter=0, sloppy="f32", reliable_every=50, delta=0.1, log=0):
+        p = InvPar()
+        p.mass = mass
+        p.prec = prec
+        p.max_iter = max_iter
+        p.mvm = MVM[mvm] if isinstance(mvm, str) else mvm
+        p.sloppy_prec = PREC[sloppy] if isinstance(sloppy, str) else sloppy
+        p.reliable_every = reliable_every
+        p.delta = delta
+        p.log = log
+        return p
+
+    def cg_inverter(self, par, d, s):
+        it = ctypes.c_int(0)
+        _check(lib().yh_cg(self.ctx, ctypes.byref(par), d.h, s.h, ctypes.byref(it)))
+        return it.value
+
+    def bicgstab_inverter(self, par, d, s):
+        it = ctypes.c_int(0)
+        _check(lib().yh_bicgstab(self.ctx, ctypes.byref(par), d.h, s.h, ctypes.byref(it)))
+        return it.value
+
+    def cg_mshift(self, par, shifts, dsts, s):
+        shifts = np.ascontiguousarray(shifts, dtype=np.float64)
+        arr = (_VP * len(dsts))(*[d.h for d in dsts])
+        it = ctypes.c_int(0)
+        _check(lib().yh_cg_mshift(self.ctx, ctypes.byref(par), len(dsts), _dp(shifts), arr, s.h, ctypes.byref(it)))
+        return it.value
+
+    def cg_mixed(self, par, d, s):
+        it = ctypes.c_int(0)
+        up = ctypes.c_int(0)
+        _check(lib().yh_cg_mixed(self.ctx, ctypes.byref(par), d.h, s.h, ctypes.byref(it), ctypes.byref(up)))
+        return it.value, up.value
+
+    # ---- measurement ---------------------------------------------------
+    def bench_hop(self, prec, recon, reps, d, s):
+        out = np.zeros(1)
+        _check(lib().yh_bench_hop(self.ctx, PREC[prec] if isinstance(prec, str) else prec, recon, reps, d.h, s.h,
+                                  _dp(out)))
+        return out[0]
+
+    def bench_copy(self, nbytes, reps):
+        out = np.zeros(1)
+        _check(lib().yh_bench_copy(self.ctx, int(nbytes), reps, _dp(out)))
+        return out[0]
+
+    def synchronize(self):
+        _check(lib().yh_synchronize(self.ctx))

@@ -1,0 +1,304 @@
+// BLAS-1 on one parity half (library/linalg.cpp:29-271) and the fused CG
+// update kernels.  Every kernel reads each operand once and writes once;
+// reductions are FP64 and deterministic (yh_reduce.hpp) whatever the
+// storage precision.
+#include "yh_internal.hpp"
+#include "yh_reduce.hpp"
+
+namespace yh {
+
+#define YH_SITE_LOOP_BEGIN                      \
+  const int k = blockIdx.x * 256 + threadIdx.x; \
+  const bool act = k < g.VH;
+
+template <int P, int D>
+using RT = typename SpinorStore<P, D>::real;
+
+template <int P, int D>
+__global__ void __launch_bounds__(256) k_axpby(void* y, float* yn, double a, const void* x, const float* xn, double b,
+                                               Geom g) {
+  typedef RT<P, D> R;
+  YH_SITE_LOOP_BEGIN
+  if (!act) return;
+  Cx<R> X[4][D], Y[4][D];
+  load_spinor<P, D, R>(X, x, xn, k, g.VHpad);
+  load_spinor<P, D, R>(Y, y, yn, k, g.VHpad);
+#pragma unroll
+  for (int s = 0; s < 4; s++)
+#pragma unroll
+    for (int i = 0; i < D; i++) Y[s][i] = Cx<R>((R)a * X[s][i].re + (R)b * Y[s][i].re, (R)a * X[s][i].im + (R)b * Y[s][i].im);
+  store_spinor<P, D, R>(y, yn, k, g.VHpad, Y);
+}
+
+template <int P, int D>
+__global__ void __launch_bounds__(256) k_caxpy(void* y, float* yn, double ar, double ai, const void* x, const float* xn,
+                                               Geom g) {
+  typedef RT<P, D> R;
+  YH_SITE_LOOP_BEGIN
+  if (!act) return;
+  Cx<R> X[4][D], Y[4][D];
+  load_spinor<P, D, R>(X, x, xn, k, g.VHpad);
+  load_spinor<P, D, R>(Y, y, yn, k, g.VHpad);
+  Cx<R> c((R)ar, (R)ai);
+#pragma unroll
+  for (int s = 0; s < 4; s++)
+#pragma unroll
+    for (int i = 0; i < D; i++) cmac(Y[s][i], c, X[s][i]);
+  store_spinor<P, D, R>(y, yn, k, g.VHpad, Y);
+}
+
+template <int P, int D>
+__global__ void __launch_bounds__(256) k_cscale(void* y, float* yn, double ar, double ai, Geom g) {
+  typedef RT<P, D> R;
+  YH_SITE_LOOP_BEGIN
+  if (!act) return;
+  Cx<R> Y[4][D];
+  load_spinor<P, D, R>(Y, y, yn, k, g.VHpad);
+  Cx<R> c((R)ar, (R)ai);
+#pragma unroll
+  for (int s = 0; s < 4; s++)
+#pragma unroll
+    for (int i = 0; i < D; i++) Y[s][i] = c * Y[s][i];
+  store_spinor<P, D, R>(y, yn, k, g.VHpad, Y);
+}
+
+template <int P, int D>
+__global__ void __launch_bounds__(256) k_g5(void* y, float* yn, Geom g) {
+  typedef RT<P, D> R;
+  YH_SITE_LOOP_BEGIN
+  if (!act) return;
+  Cx<R> Y[4][D];
+  load_spinor<P, D, R>(Y, y, yn, k, g.VHpad);
+#pragma unroll
+  for (int s = 2; s < 4; s++)
+#pragma unroll
+    for (int i = 0; i < D; i++) Y[s][i] = Cx<R>(-Y[s][i].re, -Y[s][i].im);
+  store_spinor<P, D, R>(y, yn, k, g.VHpad, Y);
+}
+
+template <int PD, int PS, int D>
+__global__ void __launch_bounds__(256) k_convert(void* y, float* yn, const void* x, const float* xn, Geom g) {
+  YH_SITE_LOOP_BEGIN
+  if (!act) return;
+  Cx<double> X[4][D];
+  load_spinor<PS, D, double>(X, x, xn, k, g.VHpad);
+  store_spinor<PD, D, double>(y, yn, k, g.VHpad, X);
+}
+
+// OP 0: |a|^2 ; 1: <a, b> = sum a conj(b) (2 values); 2: Re <a, b>
+template <int P, int D, int OP>
+__global__ void __launch_bounds__(256) k_reduce(const void* a, const float* an, const void* b, const float* bn,
+                                                double* partials, unsigned* ticket, double* out, Geom g) {
+  typedef RT<P, D> R;
+  YH_SITE_LOOP_BEGIN
+  double v[2] = {0.0, 0.0};
+  if (act) {
+    Cx<R> A[4][D];
+    load_spinor<P, D, R>(A, a, an, k, g.VHpad);
+    if constexpr (OP == 0) {
+#pragma unroll
+      for (int s = 0; s < 4; s++)
+#pragma unroll
+        for (int i = 0; i < D; i++) v[0] += (double)A[s][i].re * A[s][i].re + (double)A[s][i].im * A[s][i].im;
+    } else {
+      Cx<R> B[4][D];
+      load_spinor<P, D, R>(B, b, bn, k, g.VHpad);
+#pragma unroll
+      for (int s = 0; s < 4; s++)
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+          v[0] += (double)A[s][i].re * B[s][i].re + (double)A[s][i].im * B[s][i].im;
+          v[1] += (double)A[s][i].im * B[s][i].re - (double)A[s][i].re * B[s][i].im;
+        }
+    }
+  }
+  block_reduce_finish<2, 256>(v, partials, gridDim.x, blockIdx.x, ticket, out);
+}
+
+// CG: alpha = rr / pap; x += alpha p; r -= alpha q; rr_new = |r|^2.
+// The last block updates the scalars (library/cg.cpp:174-189).
+template <int P, int PP, int D>
+__global__ void __launch_bounds__(256) k_cg_xr(void* x, float* xn, void* r, float* rn, const void* p, const float* pn,
+                                               const void* q, const float* qn, SolverScalars* sc, double* partials,
+                                               unsigned* ticket, Geom g) {
+  typedef RT<P, D> R;
+  if (sc->converged) return;
+  const double alpha = sc->rr / sc->red[0];
+  YH_SITE_LOOP_BEGIN
+  double v[1] = {0.0};
+  if (act) {
+    Cx<R> X[4][D], Pv[4][D];
+    load_spinor<PP, D, R>(Pv, p, pn, k, g.VHpad);
+    load_spinor<P, D, R>(X, x, xn, k, g.VHpad);
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+#pragma unroll
+      for (int i = 0; i < D; i++) X[s][i] = Cx<R>(X[s][i].re + (R)alpha * Pv[s][i].re, X[s][i].im + (R)alpha * Pv[s][i].im);
+    store_spinor<P, D, R>(x, xn, k, g.VHpad, X);
+    Cx<R> Q[4][D];
+    load_spinor<PP, D, R>(Q, q, qn, k, g.VHpad);
+    load_spinor<P, D, R>(X, r, rn, k, g.VHpad);
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+#pragma unroll
+      for (int i = 0; i < D; i++) X[s][i] = Cx<R>(X[s][i].re - (R)alpha * Q[s][i].re, X[s][i].im - (R)alpha * Q[s][i].im);
+    store_spinor<P, D, R>(r, rn, k, g.VHpad, X);
+    if constexpr (P == PREC_F16) load_spinor<P, D, R>(X, r, rn, k, g.VHpad);
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+#pragma unroll
+      for (int i = 0; i < D; i++) v[0] += (double)X[s][i].re * X[s][i].re + (double)X[s][i].im * X[s][i].im;
+  }
+  double res[1];
+  if (block_reduce_finish<1, 256>(v, partials, gridDim.x, blockIdx.x, ticket, res)) {
+    if (threadIdx.x == 0) {
+      double rr_new = res[0];
+      sc->alpha = alpha;
+      sc->beta = rr_new / sc->rr;
+      sc->rr_new = rr_new;
+      sc->rr = rr_new;
+      sc->iters += 1;
+      if (rr_new < sc->target || !(rr_new == rr_new)) sc->converged = 1;
+      if (rr_new > sc->maxrr) sc->maxrr = rr_new;
+    }
+  }
+}
+
+// CG: p = r + beta p   (p in precision PP, r in P)
+template <int P, int PP, int D>
+__global__ void __launch_bounds__(256) k_cg_p(void* p, float* pn, const void* r, const float* rn, const SolverScalars* sc,
+                                              Geom g) {
+  typedef RT<P, D> R;
+  if (sc->converged) return;
+  const R beta = (R)sc->beta;
+  YH_SITE_LOOP_BEGIN
+  if (!act) return;
+  Cx<R> Pv[4][D], Rv[4][D];
+  load_spinor<PP, D, R>(Pv, p, pn, k, g.VHpad);
+  load_spinor<P, D, R>(Rv, r, rn, k, g.VHpad);
+#pragma unroll
+  for (int s = 0; s < 4; s++)
+#pragma unroll
+    for (int i = 0; i < D; i++) Pv[s][i] = Cx<R>(Rv[s][i].re + beta * Pv[s][i].re, Rv[s][i].im + beta * Pv[s][i].im);
+  store_spinor<PP, D, R>(p, pn, k, g.VHpad, Pv);
+}
+
+// ---------------------------------------------------------------------------
+#define YH_D_SWITCH(CALL)                                   \
+  switch (c->dr) {                                          \
+    case 2: { constexpr int D_ = 2; CALL; } break;          \
+    case 3: { constexpr int D_ = 3; CALL; } break;          \
+    case 4: { constexpr int D_ = 4; CALL; } break;          \
+    case 6: { constexpr int D_ = 6; CALL; } break;          \
+    default: set_error("unsupported representation dimension"); return YH_ERR_UNSUPPORTED; \
+  }
+
+#define YH_P_SWITCH(PREC, CALL)                             \
+  switch (PREC) {                                           \
+    case PREC_F64: { constexpr int P_ = PREC_F64; CALL; } break; \
+    case PREC_F32: { constexpr int P_ = PREC_F32; CALL; } break; \
+    default: { constexpr int P_ = PREC_F16; CALL; } break;  \
+  }
+
+static dim3 gsz(Ctx* c) { return dim3((c->g.VH + 255) / 256); }
+
+int blas_axpby(Ctx* c, Field* y, double a, const Field* x, double b, int par) {
+  if (x->prec != y->prec) { set_error("precision mismatch"); return YH_ERR_ARG; }
+  YH_D_SWITCH(YH_P_SWITCH(y->prec, hipLaunchKernelGGL((k_axpby<P_, D_>), gsz(c), dim3(256), 0, c->stream, y->half[par],
+                                                      y->nrm[par], a, x->half[par], x->nrm[par], b, c->g)));
+  YH_HIP(hipGetLastError());
+  return YH_OK;
+}
+
+int blas_caxpy(Ctx* c, Field* y, double ar, double ai, const Field* x, int par) {
+  if (x->prec != y->prec) { set_error("precision mismatch"); return YH_ERR_ARG; }
+  YH_D_SWITCH(YH_P_SWITCH(y->prec, hipLaunchKernelGGL((k_caxpy<P_, D_>), gsz(c), dim3(256), 0, c->stream, y->half[par],
+                                                      y->nrm[par], ar, ai, x->half[par], x->nrm[par], c->g)));
+  YH_HIP(hipGetLastError());
+  return YH_OK;
+}
+
+int blas_cscale(Ctx* c, Field* y, double ar, double ai, int par) {
+  YH_D_SWITCH(YH_P_SWITCH(y->prec, hipLaunchKernelGGL((k_cscale<P_, D_>), gsz(c), dim3(256), 0, c->stream, y->half[par],
+                                                      y->nrm[par], ar, ai, c->g)));
+  YH_HIP(hipGetLastError());
+  return YH_OK;
+}
+
+int blas_g5(Ctx* c, Field* y, int par) {
+  YH_D_SWITCH(YH_P_SWITCH(y->prec, hipLaunchKernelGGL((k_g5<P_, D_>), gsz(c), dim3(256), 0, c->stream, y->half[par],
+                                                      y->nrm[par], c->g)));
+  YH_HIP(hipGetLastError());
+  return YH_OK;
+}
+
+int field_convert(Ctx* c, Field* dst, int dpar, const Field* src, int spar) {
+#define YH_CV(PD, PS) \
+  YH_D_SWITCH(hipLaunchKernelGGL((k_convert<PD, PS, D_>), gsz(c), dim3(256), 0, c->stream, dst->half[dpar], dst->nrm[dpar], src->half[spar], src->nrm[spar], c->g))
+  int key = dst->prec * 3 + src->prec;
+  switch (key) {
+    case 0: YH_CV(PREC_F64, PREC_F64); break;
+    case 1: YH_CV(PREC_F64, PREC_F32); break;
+    case 2: YH_CV(PREC_F64, PREC_F16); break;
+    case 3: YH_CV(PREC_F32, PREC_F64); break;
+    case 4: YH_CV(PREC_F32, PREC_F32); break;
+    case 5: YH_CV(PREC_F32, PREC_F16); break;
+    case 6: YH_CV(PREC_F16, PREC_F64); break;
+    case 7: YH_CV(PREC_F16, PREC_F32); break;
+    default: YH_CV(PREC_F16, PREC_F16); break;
+  }
+#undef YH_CV
+  YH_HIP(hipGetLastError());
+  return YH_OK;
+}
+
+int blas_copy(Ctx* c, Field* dst, const Field* src, int par) { return field_convert(c, dst, par, src, par); }
+
+static int reduce_launch(Ctx* c, int op, const Field* a, const Field* b, int par, int slot) {
+  if (b && a->prec != b->prec) { set_error("precision mismatch"); return YH_ERR_ARG; }
+  double* out = &c->sc->red[slot];
+  const void* bp = b ? b->half[par] : nullptr;
+  const float* bn = b ? b->nrm[par] : nullptr;
+#define YH_RL(OPV) \
+  YH_D_SWITCH(YH_P_SWITCH(a->prec, hipLaunchKernelGGL((k_reduce<P_, D_, OPV>), gsz(c), dim3(256), 0, c->stream, a->half[par], a->nrm[par], bp, bn, c->partials, c->ticket, out, c->g)))
+  if (op == 0) { YH_RL(0); }
+  else if (op == 1) { YH_RL(1); }
+  else { YH_RL(2); }
+#undef YH_RL
+  YH_HIP(hipGetLastError());
+  return YH_OK;
+}
+
+int blas_sqnorm(Ctx* c, const Field* a, int par, int slot) { return reduce_launch(c, 0, a, nullptr, par, slot); }
+int blas_dot(Ctx* c, const Field* a, const Field* b, int par, int slot) { return reduce_launch(c, 1, a, b, par, slot); }
+int blas_dot_re(Ctx* c, const Field* a, const Field* b, int par, int slot) { return reduce_launch(c, 2, a, b, par, slot); }
+
+// supported (x/r precision, p/q precision) pairs: (64,64), (32,32), (32,16)
+int cg_update_xr(Ctx* c, Field* x, Field* r, const Field* p, const Field* q, int par, int prec) {
+  (void)prec;
+  if (x->prec != r->prec || p->prec != q->prec) { set_error("precision mismatch"); return YH_ERR_ARG; }
+#define YH_XR(PX, PPV) \
+  YH_D_SWITCH(hipLaunchKernelGGL((k_cg_xr<PX, PPV, D_>), gsz(c), dim3(256), 0, c->stream, x->half[par], x->nrm[par], r->half[par], r->nrm[par], p->half[par], p->nrm[par], q->half[par], q->nrm[par], c->sc, c->partials, c->ticket, c->g))
+  if (x->prec == PREC_F64 && p->prec == PREC_F64) { YH_XR(PREC_F64, PREC_F64); }
+  else if (x->prec == PREC_F32 && p->prec == PREC_F32) { YH_XR(PREC_F32, PREC_F32); }
+  else if (x->prec == PREC_F32 && p->prec == PREC_F16) { YH_XR(PREC_F32, PREC_F16); }
+  else { set_error("unsupported CG precision pair"); return YH_ERR_UNSUPPORTED; }
+#undef YH_XR
+  YH_HIP(hipGetLastError());
+  return YH_OK;
+}
+
+int cg_update_p(Ctx* c, Field* p, const Field* r, int par, int prec) {
+  (void)prec;
+#define YH_PU(PX, PPV) \
+  YH_D_SWITCH(hipLaunchKernelGGL((k_cg_p<PX, PPV, D_>), gsz(c), dim3(256), 0, c->stream, p->half[par], p->nrm[par], r->half[par], r->nrm[par], c->sc, c->g))
+  if (r->prec == PREC_F64 && p->prec == PREC_F64) { YH_PU(PREC_F64, PREC_F64); }
+  else if (r->prec == PREC_F32 && p->prec == PREC_F32) { YH_PU(PREC_F32, PREC_F32); }
+  else if (r->prec == PREC_F32 && p->prec == PREC_F16) { YH_PU(PREC_F32, PREC_F16); }
+  else { set_error("unsupported CG precision pair"); return YH_ERR_UNSUPPORTED; }
+#undef YH_PU
+  YH_HIP(hipGetLastError());
+  return YH_OK;
+}
+
+}  // namespace yh

@@ -1,0 +1,354 @@
+// Shared device/host definitions for the MI355X Wilson-clover hot path.
+//
+// HBM layouts (all per parity half, "site-minor" structure-of-arrays so that
+// a wave of 64 consecutive output sites issues fully coalesced 8/16-byte
+// loads for every component):
+//
+//   spinor half   vec[v * VHpad + site], v < SPV
+//                 complex component k = spin * D + colour
+//                 FP64: double2 (1 complex / vec), SPV = 4D
+//                 FP32: float4  (2 complex / vec), SPV = 2D
+//                 FP16: uint4   (4 complex / vec, IEEE half), SPV = D,
+//                       plus one float scale per site (max |component|)
+//   links         for output parity p and direction mu, the forward link
+//                 U(x, mu) of every parity-p site x:
+//                 vec[((p * 4 + mu) * LV + v) * VHpad + site]
+//                 stored WITHOUT the antiperiodic sign (applied in-kernel)
+//   clover        per site two Hermitian 2D x 2D chiral blocks (spins 0,1
+//                 and spins 2,3; row a = spin_in_pair * D + colour), packed
+//                 as n real diagonal entries then the strictly lower
+//                 triangle row-major as complex: n^2 reals per block
+//
+// Site order inside a half is the reference's EO order restricted to one
+// parity: half index = lex / 2 with lex = ((t*X + x)*Y + y)*Z + z
+// (library/geometry.cpp:756-878, single process), so host fields convert
+// site by site.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <stdint.h>
+
+namespace yh {
+
+enum { PREC_F64 = 0, PREC_F32 = 1, PREC_F16 = 2 };
+enum { BC_PERIODIC = 0, BC_ANTIPERIODIC = 1, BC_OPEN = 2, BC_SF = 3 };
+enum { RECON_18 = 0, RECON_12 = 1 };
+
+struct Geom {
+  int L[4];       // T, X, Y, Z
+  int V, VH;      // volume, half volume
+  int VHpad;      // padded half-field stride (elements)
+  int bc;         // boundary condition
+  int Ta, Tb;     // time slices carrying the c_f improvement (open / SF)
+  double cf;      // c_f (open / SF); the diagonal gets c_f - 1 extra there
+};
+
+template <typename R>
+struct Cx {
+  R re, im;
+  __host__ __device__ Cx() {}
+  __host__ __device__ Cx(R a, R b) : re(a), im(b) {}
+};
+
+template <typename R>
+__host__ __device__ inline Cx<R> operator+(Cx<R> a, Cx<R> b) { return Cx<R>(a.re + b.re, a.im + b.im); }
+template <typename R>
+__host__ __device__ inline Cx<R> operator-(Cx<R> a, Cx<R> b) { return Cx<R>(a.re - b.re, a.im - b.im); }
+template <typename R>
+__host__ __device__ inline Cx<R> operator*(Cx<R> a, Cx<R> b) {
+  return Cx<R>(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+template <typename R>
+__host__ __device__ inline Cx<R> operator*(R s, Cx<R> a) { return Cx<R>(s * a.re, s * a.im); }
+template <typename R>
+__host__ __device__ inline Cx<R> conj(Cx<R> a) { return Cx<R>(a.re, -a.im); }
+// a += b * c
+template <typename R>
+__host__ __device__ inline void cmac(Cx<R>& a, Cx<R> b, Cx<R> c) {
+  a.re += b.re * c.re - b.im * c.im;
+  a.im += b.re * c.im + b.im * c.re;
+}
+// a += conj(b) * c
+template <typename R>
+__host__ __device__ inline void cmac_conj(Cx<R>& a, Cx<R> b, Cx<R> c) {
+  a.re += b.re * c.re + b.im * c.im;
+  a.im += b.re * c.im - b.im * c.re;
+}
+
+// multiplication by one of {+1, -1, +i, -i} (codes 0..3)
+template <int C, typename R>
+__device__ __forceinline__ Cx<R> unit_mul(Cx<R> z) {
+  if constexpr (C == 0) return z;
+  else if constexpr (C == 1) return Cx<R>(-z.re, -z.im);
+  else if constexpr (C == 2) return Cx<R>(-z.im, z.re);
+  else return Cx<R>(z.im, -z.re);
+}
+
+// ---------------------------------------------------------------------------
+// geometry
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void site_coords(int k, int parity, const Geom& g, int c[4]) {
+  int lex = 2 * k;
+  int z = lex % g.L[3];
+  int r = lex / g.L[3];
+  int y = r % g.L[2];
+  r /= g.L[2];
+  int x = r % g.L[1];
+  int t = r / g.L[1];
+  if (((t + x + y + z) & 1) != parity) z += 1;
+  c[0] = t; c[1] = x; c[2] = y; c[3] = z;
+}
+
+__device__ __forceinline__ int half_index(int t, int x, int y, int z, const Geom& g) {
+  return (((t * g.L[1] + x) * g.L[2] + y) * g.L[3] + z) >> 1;
+}
+
+// true where the boundary conditions force spinors to zero
+// (library/bcs.cpp:139-164: open t in {0, T-1}; SF t in {0, T-2, T-1})
+__device__ __forceinline__ bool bc_zero_slice(int t, const Geom& g) {
+  if (g.bc == BC_OPEN) return t == 0 || t == g.L[0] - 1;
+  if (g.bc == BC_SF) return t == 0 || t == g.L[0] - 2 || t == g.L[0] - 1;
+  return false;
+}
+
+__device__ __forceinline__ double bc_cf_shift(int t, const Geom& g) {
+  if ((g.bc == BC_OPEN || g.bc == BC_SF) && (t == g.Ta || t == g.Tb)) return g.cf - 1.0;
+  return 0.0;
+}
+
+// XCD-aware block order: the hardware deals consecutive workgroups round
+// robin to the 8 XCDs; remap so that each XCD sweeps one contiguous slab
+// of the lattice and neighbouring t-slices share that XCD's L2.
+__device__ __forceinline__ int xcd_block(int nblocks) {
+  int b = blockIdx.x;
+  const int nx = 8;
+  int per = nblocks / nx;
+  if (per == 0 || b >= per * nx) return b;
+  return (b % nx) * per + b / nx;
+}
+
+// ---------------------------------------------------------------------------
+// spinor storage traits
+// ---------------------------------------------------------------------------
+template <int PREC, int D>
+struct SpinorStore;
+
+template <int D>
+struct SpinorStore<PREC_F64, D> {
+  typedef double2 vec;
+  typedef double real;
+  static constexpr int NV = 4 * D;
+  static constexpr int BYTES = 64 * D;  // per site
+};
+template <int D>
+struct SpinorStore<PREC_F32, D> {
+  typedef float4 vec;
+  typedef float real;
+  static constexpr int NV = 2 * D;
+  static constexpr int BYTES = 32 * D;
+};
+template <int D>
+struct SpinorStore<PREC_F16, D> {
+  typedef uint4 vec;
+  typedef float real;
+  static constexpr int NV = D;
+  static constexpr int BYTES = 16 * D + 4;
+};
+
+__device__ __forceinline__ float h2f(unsigned short h) { return __half2float(__ushort_as_half(h)); }
+__device__ __forceinline__ unsigned short f2h(float f) { return __half_as_ushort(__float2half_rn(f)); }
+
+// load the full spinor of one site into registers s[spin][colour]
+template <int PREC, int D, typename R>
+__device__ __forceinline__ void load_spinor(Cx<R> (&s)[4][D], const void* base, const float* nrm, int site,
+                                            int stride) {
+  if constexpr (PREC == PREC_F64) {
+    const double2* p = (const double2*)base;
+#pragma unroll
+    for (int k = 0; k < 4 * D; k++) {
+      double2 v = p[(size_t)k * stride + site];
+      s[k / D][k % D] = Cx<R>((R)v.x, (R)v.y);
+    }
+  } else if constexpr (PREC == PREC_F32) {
+    const float4* p = (const float4*)base;
+#pragma unroll
+    for (int k = 0; k < 2 * D; k++) {
+      float4 v = p[(size_t)k * stride + site];
+      s[(2 * k) / D][(2 * k) % D] = Cx<R>((R)v.x, (R)v.y);
+      s[(2 * k + 1) / D][(2 * k + 1) % D] = Cx<R>((R)v.z, (R)v.w);
+    }
+  } else {
+    const uint4* p = (const uint4*)base;
+    float sc = nrm[site];
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+      uint4 v = p[(size_t)k * stride + site];
+      unsigned w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        int c = 4 * k + q;
+        s[c / D][c % D] = Cx<R>((R)(h2f(w[q] & 0xffff) * sc), (R)(h2f(w[q] >> 16) * sc));
+      }
+    }
+  }
+}
+
+template <int PREC, int D, typename R>
+__device__ __forceinline__ void store_spinor(void* base, float* nrm, int site, int stride, const Cx<R> (&s)[4][D]) {
+  if constexpr (PREC == PREC_F64) {
+    double2* p = (double2*)base;
+#pragma unroll
+    for (int k = 0; k < 4 * D; k++) p[(size_t)k * stride + site] = make_double2(s[k / D][k % D].re, s[k / D][k % D].im);
+  } else if constexpr (PREC == PREC_F32) {
+    float4* p = (float4*)base;
+#pragma unroll
+    for (int k = 0; k < 2 * D; k++) {
+      Cx<R> a = s[(2 * k) / D][(2 * k) % D], b = s[(2 * k + 1) / D][(2 * k + 1) % D];
+      p[(size_t)k * stride + site] = make_float4((float)a.re, (float)a.im, (float)b.re, (float)b.im);
+    }
+  } else {
+    uint4* p = (uint4*)base;
+    float mx = 0.f;
+#pragma unroll
+    for (int c = 0; c < 4 * D; c++) {
+      mx = fmaxf(mx, fabsf((float)s[c / D][c % D].re));
+      mx = fmaxf(mx, fabsf((float)s[c / D][c % D].im));
+    }
+    float inv = mx > 0.f ? 1.0f / mx : 0.f;
+    nrm[site] = mx;
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+      unsigned w[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        int c = 4 * k + q;
+        w[q] = (unsigned)f2h((float)s[c / D][c % D].re * inv) | ((unsigned)f2h((float)s[c / D][c % D].im * inv) << 16);
+      }
+      p[(size_t)k * stride + site] = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// link storage traits
+// ---------------------------------------------------------------------------
+// number of storage vectors per link
+template <int PREC, int RECON, int D>
+struct LinkStore;
+template <int D>
+struct LinkStore<PREC_F64, RECON_18, D> { typedef double2 vec; static constexpr int NV = D * D; static constexpr int BYTES = 16 * D * D; };
+template <int D>
+struct LinkStore<PREC_F64, RECON_12, D> { typedef double2 vec; static constexpr int NV = 6; static constexpr int BYTES = 96; };
+template <int D>
+struct LinkStore<PREC_F32, RECON_18, D> { typedef float2 vec; static constexpr int NV = D * D; static constexpr int BYTES = 8 * D * D; };
+template <int D>
+struct LinkStore<PREC_F32, RECON_12, D> { typedef float4 vec; static constexpr int NV = 3; static constexpr int BYTES = 48; };
+template <int D>
+struct LinkStore<PREC_F16, RECON_18, D> { typedef unsigned vec; static constexpr int NV = D * D; static constexpr int BYTES = 4 * D * D; };
+template <int D>
+struct LinkStore<PREC_F16, RECON_12, D> { typedef uint2 vec; static constexpr int NV = 3; static constexpr int BYTES = 24; };
+
+// third row of an SU(3) matrix from the first two: conj(r0 x r1)
+template <typename R>
+__device__ __forceinline__ void su3_third_row(Cx<R> (&u)[3][3]) {
+  u[2][0] = conj(u[0][1] * u[1][2] - u[0][2] * u[1][1]);
+  u[2][1] = conj(u[0][2] * u[1][0] - u[0][0] * u[1][2]);
+  u[2][2] = conj(u[0][0] * u[1][1] - u[0][1] * u[1][0]);
+}
+
+template <int PREC, int RECON, int D, typename R>
+__device__ __forceinline__ void load_link(Cx<R> (&u)[D][D], const void* base, int site, int stride) {
+  typedef typename LinkStore<PREC, RECON, D>::vec V;
+  const V* p = (const V*)base;
+  if constexpr (RECON == RECON_18) {
+#pragma unroll
+    for (int k = 0; k < D * D; k++) {
+      V v = p[(size_t)k * stride + site];
+      if constexpr (PREC == PREC_F64 || PREC == PREC_F32) u[k / D][k % D] = Cx<R>((R)v.x, (R)v.y);
+      else u[k / D][k % D] = Cx<R>((R)h2f(v & 0xffff), (R)h2f(v >> 16));
+    }
+  } else {
+    static_assert(D == 3, "12-real reconstruction is SU(3) only");
+    Cx<R>(&w)[3][3] = *reinterpret_cast<Cx<R>(*)[3][3]>(&u);
+    if constexpr (PREC == PREC_F64) {
+#pragma unroll
+      for (int k = 0; k < 6; k++) {
+        double2 v = p[(size_t)k * stride + site];
+        w[k / 3][k % 3] = Cx<R>((R)v.x, (R)v.y);
+      }
+    } else if constexpr (PREC == PREC_F32) {
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        float4 v = p[(size_t)k * stride + site];
+        w[(2 * k) / 3][(2 * k) % 3] = Cx<R>((R)v.x, (R)v.y);
+        w[(2 * k + 1) / 3][(2 * k + 1) % 3] = Cx<R>((R)v.z, (R)v.w);
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        uint2 v = p[(size_t)k * stride + site];
+        w[(2 * k) / 3][(2 * k) % 3] = Cx<R>((R)h2f(v.x & 0xffff), (R)h2f(v.x >> 16));
+        w[(2 * k + 1) / 3][(2 * k + 1) % 3] = Cx<R>((R)h2f(v.y & 0xffff), (R)h2f(v.y >> 16));
+      }
+    }
+    su3_third_row(w);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// clover storage traits (n = 2D, n*n reals per block, two blocks per site)
+// ---------------------------------------------------------------------------
+template <int PREC, int D>
+struct CloverStore;
+template <int D>
+struct CloverStore<PREC_F64, D> { typedef double2 vec; static constexpr int NV = (2 * D) * (2 * D); static constexpr int BYTES = 8 * 2 * (2 * D) * (2 * D); };
+template <int D>
+struct CloverStore<PREC_F32, D> { typedef float4 vec; static constexpr int NV = (2 * D) * (2 * D) / 2; static constexpr int BYTES = 4 * 2 * (2 * D) * (2 * D); };
+template <int D>
+struct CloverStore<PREC_F16, D> { typedef float4 vec; static constexpr int NV = (2 * D) * (2 * D) / 2; static constexpr int BYTES = 4 * 2 * (2 * D) * (2 * D); };
+
+// load block b's packed n*n reals into a register array
+template <int PREC, int D, typename R>
+__device__ __forceinline__ void load_clover_block(R (&c)[(2 * D) * (2 * D)], const void* base, int blk, int site,
+                                                  int stride) {
+  constexpr int n2 = (2 * D) * (2 * D);
+  if constexpr (PREC == PREC_F64) {
+    const double2* p = (const double2*)base + (size_t)blk * (n2 / 2) * stride;
+#pragma unroll
+    for (int k = 0; k < n2 / 2; k++) {
+      double2 v = p[(size_t)k * stride + site];
+      c[2 * k] = (R)v.x;
+      c[2 * k + 1] = (R)v.y;
+    }
+  } else {
+    const float4* p = (const float4*)base + (size_t)blk * (n2 / 4) * stride;
+#pragma unroll
+    for (int k = 0; k < n2 / 4; k++) {
+      float4 v = p[(size_t)k * stride + site];
+      c[4 * k] = (R)v.x;
+      c[4 * k + 1] = (R)v.y;
+      c[4 * k + 2] = (R)v.z;
+      c[4 * k + 3] = (R)v.w;
+    }
+  }
+}
+
+// y = H x for the packed Hermitian block
+template <int N, typename R>
+__device__ __forceinline__ void herm_mv(Cx<R> (&y)[N], const R (&c)[N * N], const Cx<R> (&x)[N]) {
+#pragma unroll
+  for (int i = 0; i < N; i++) y[i] = c[i] * x[i];
+  int o = N;
+#pragma unroll
+  for (int i = 1; i < N; i++) {
+#pragma unroll
+    for (int j = 0; j < i; j++) {
+      Cx<R> h(c[o], c[o + 1]);
+      o += 2;
+      cmac(y[i], h, x[j]);
+      cmac_conj(y[j], h, x[i]);
+    }
+  }
+}
+
+}  // namespace yh

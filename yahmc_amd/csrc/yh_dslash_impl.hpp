@@ -1,0 +1,337 @@
+// Hopping term with fused epilogues (library/dirac.cpp:227-704, 706-781).
+//
+//   out(x) = scale * sum_mu [ (1 - g_mu) U(x,mu) psi(x+mu)
+//                           + (1 + g_mu) U(x-mu,mu)^+ psi(x-mu) ]
+//
+// One thread per output site of one parity; a wave covers 64 consecutive
+// EO sites so every neighbour / link / clover component load is a
+// contiguous 512 B - 1 KB wave access.  The spin projection keeps two of
+// the four spin rows (2 * D complex) per direction, the link multiplies
+// the projected half spinor, and the other two rows are rebuilt with a
+// multiplication by one of {+-1, +-i}.  Projection codes per direction
+// (codes: 0 = +1, 1 = -1, 2 = +i, 3 = -i) follow from the chiral gamma
+// basis of library/spinor.cpp:191-264:
+//   h_k = psi_{2+k} + PC_k * psi_{PI_k};   r = U h (or U^+ h)
+//   acc_{2+k} += r_k;  acc_k += RC_k * r_{RI_k}
+// Boundary conditions are applied in-kernel: antiperiodic flips the sign
+// of U_0 on links leaving t = 0 (library/bcs.cpp:264-272), open / SF zero
+// the output on the boundary slices (library/bcs.cpp:139-164).
+//
+// Epilogues:
+//   PLAIN    out = scale * sum                           (Dphi_eo / Dphi_oe)
+//   CLOVINV  h = scale * sum; out = D_oo^-1 h; aux = h   (first half of Dphi)
+//   CLOV     out = scale * sum + D_ee self [; gamma5] [; |out|^2]
+//                                                        (second half of Dphi)
+#pragma once
+#include "yh_internal.hpp"
+#include "yh_reduce.hpp"
+
+namespace yh {
+
+template <int MU, bool FWD>
+struct Proj;
+template <> struct Proj<0, true> { static constexpr int PI0 = 0, PI1 = 1, PC0 = 0, PC1 = 0, RI0 = 0, RI1 = 1, RC0 = 0, RC1 = 0; };
+template <> struct Proj<0, false> { static constexpr int PI0 = 0, PI1 = 1, PC0 = 1, PC1 = 1, RI0 = 0, RI1 = 1, RC0 = 1, RC1 = 1; };
+template <> struct Proj<1, true> { static constexpr int PI0 = 1, PI1 = 0, PC0 = 3, PC1 = 3, RI0 = 1, RI1 = 0, RC0 = 2, RC1 = 2; };
+template <> struct Proj<1, false> { static constexpr int PI0 = 1, PI1 = 0, PC0 = 2, PC1 = 2, RI0 = 1, RI1 = 0, RC0 = 3, RC1 = 3; };
+template <> struct Proj<2, true> { static constexpr int PI0 = 1, PI1 = 0, PC0 = 1, PC1 = 0, RI0 = 1, RI1 = 0, RC0 = 0, RC1 = 1; };
+template <> struct Proj<2, false> { static constexpr int PI0 = 1, PI1 = 0, PC0 = 0, PC1 = 1, RI0 = 1, RI1 = 0, RC0 = 1, RC1 = 0; };
+template <> struct Proj<3, true> { static constexpr int PI0 = 0, PI1 = 1, PC0 = 3, PC1 = 2, RI0 = 0, RI1 = 1, RC0 = 2, RC1 = 3; };
+template <> struct Proj<3, false> { static constexpr int PI0 = 0, PI1 = 1, PC0 = 2, PC1 = 3, RI0 = 0, RI1 = 1, RC0 = 3, RC1 = 2; };
+
+struct HopArgs {
+  const void* in;
+  const float* in_nrm;
+  void* out;
+  float* out_nrm;
+  void* aux;
+  float* aux_nrm;
+  const void* self;
+  const float* self_nrm;
+  const void* lf;   // links of the output parity, [mu][vec][site]
+  const void* lb;   // links of the input parity
+  const void* clov; // packed clover (CLOV: term of the output parity; CLOVINV: inverse)
+  double scale, diag;
+  int parity, gamma5, zero_self_bc, reduce;
+  double* partials;
+  unsigned* ticket;
+  double* red_out;
+  const int* skip;
+  Geom g;
+};
+
+template <int MU, bool FWD, int D, int PREC, int RECON, typename R>
+__device__ __forceinline__ void hop_dir(Cx<R> (&acc)[4][D], const HopArgs& a, int nidx, int lidx, bool flip) {
+  typedef Proj<MU, FWD> P;
+  constexpr int LNV = LinkStore<PREC, RECON, D>::NV;
+  typedef typename LinkStore<PREC, RECON, D>::vec LV;
+  Cx<R> s[4][D];
+  load_spinor<PREC, D, R>(s, a.in, a.in_nrm, nidx, a.g.VHpad);
+  Cx<R> h[2][D];
+#pragma unroll
+  for (int i = 0; i < D; i++) {
+    h[0][i] = s[2][i] + unit_mul<P::PC0>(s[P::PI0][i]);
+    h[1][i] = s[3][i] + unit_mul<P::PC1>(s[P::PI1][i]);
+  }
+  const LV* lbase = (const LV*)(FWD ? a.lf : a.lb) + (size_t)MU * LNV * a.g.VHpad;
+  Cx<R> u[D][D];
+  load_link<PREC, RECON, D, R>(u, lbase, lidx, a.g.VHpad);
+  Cx<R> r[2][D];
+#pragma unroll
+  for (int i = 0; i < D; i++) {
+    Cx<R> r0((R)0, (R)0), r1((R)0, (R)0);
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+      if constexpr (FWD) {
+        cmac(r0, u[i][j], h[0][j]);
+        cmac(r1, u[i][j], h[1][j]);
+      } else {
+        cmac_conj(r0, u[j][i], h[0][j]);
+        cmac_conj(r1, u[j][i], h[1][j]);
+      }
+    }
+    if (flip) {
+      r0 = Cx<R>(-r0.re, -r0.im);
+      r1 = Cx<R>(-r1.re, -r1.im);
+    }
+    r[0][i] = r0;
+    r[1][i] = r1;
+  }
+#pragma unroll
+  for (int i = 0; i < D; i++) {
+    acc[2][i] = acc[2][i] + r[0][i];
+    acc[3][i] = acc[3][i] + r[1][i];
+    acc[0][i] = acc[0][i] + unit_mul<P::RC0>(r[P::RI0][i]);
+    acc[1][i] = acc[1][i] + unit_mul<P::RC1>(r[P::RI1][i]);
+  }
+}
+
+template <int D, int PREC, int RECON, int EPI>
+__global__ void __launch_bounds__(256) k_hop(HopArgs a) {
+  typedef typename SpinorStore<PREC, D>::real R;
+  if (a.skip && *a.skip) return;
+  const int nblk = gridDim.x;
+  const int blk = xcd_block(nblk);
+  const int k = blk * 256 + threadIdx.x;
+  const Geom& g = a.g;
+  double nrm2 = 0.0;
+  if (k < g.VH) {
+    int c[4];
+    site_coords(k, a.parity, g, c);
+    const int t = c[0], x = c[1], y = c[2], z = c[3];
+    const int T = g.L[0];
+    Cx<R> acc[4][D];
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+#pragma unroll
+      for (int i = 0; i < D; i++) acc[s][i] = Cx<R>((R)0, (R)0);
+    const bool ap = g.bc == BC_ANTIPERIODIC;
+    // t
+    {
+      int tf = (t + 1 == T) ? 0 : t + 1;
+      int tb = (t == 0) ? T - 1 : t - 1;
+      int nf = half_index(tf, x, y, z, g), nb = half_index(tb, x, y, z, g);
+      hop_dir<0, true, D, PREC, RECON, R>(acc, a, nf, k, ap && t == 0);
+      hop_dir<0, false, D, PREC, RECON, R>(acc, a, nb, nb, ap && tb == 0);
+    }
+    {
+      int xf = (x + 1 == g.L[1]) ? 0 : x + 1;
+      int xb = (x == 0) ? g.L[1] - 1 : x - 1;
+      int nf = half_index(t, xf, y, z, g), nb = half_index(t, xb, y, z, g);
+      hop_dir<1, true, D, PREC, RECON, R>(acc, a, nf, k, false);
+      hop_dir<1, false, D, PREC, RECON, R>(acc, a, nb, nb, false);
+    }
+    {
+      int yf = (y + 1 == g.L[2]) ? 0 : y + 1;
+      int yb = (y == 0) ? g.L[2] - 1 : y - 1;
+      int nf = half_index(t, x, yf, z, g), nb = half_index(t, x, yb, z, g);
+      hop_dir<2, true, D, PREC, RECON, R>(acc, a, nf, k, false);
+      hop_dir<2, false, D, PREC, RECON, R>(acc, a, nb, nb, false);
+    }
+    {
+      int zf = (z + 1 == g.L[3]) ? 0 : z + 1;
+      int zb = (z == 0) ? g.L[3] - 1 : z - 1;
+      int nf = half_index(t, x, y, zf, g), nb = half_index(t, x, y, zb, g);
+      hop_dir<3, true, D, PREC, RECON, R>(acc, a, nf, k, false);
+      hop_dir<3, false, D, PREC, RECON, R>(acc, a, nb, nb, false);
+    }
+    const bool bz = bc_zero_slice(t, g);
+    const R sc = bz ? (R)0 : (R)a.scale;
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+#pragma unroll
+      for (int i = 0; i < D; i++) acc[s][i] = sc * acc[s][i];
+
+    if constexpr (EPI == EPI_PLAIN) {
+      store_spinor<PREC, D, R>(a.out, a.out_nrm, k, g.VHpad, acc);
+    } else if constexpr (EPI == EPI_CLOVINV) {
+      if (a.aux) store_spinor<PREC, D, R>(a.aux, a.aux_nrm, k, g.VHpad, acc);
+      constexpr int N = 2 * D;
+      Cx<R> o[4][D];
+#pragma unroll
+      for (int blk2 = 0; blk2 < 2; blk2++) {
+        R cl[N * N];
+        load_clover_block<PREC, D, R>(cl, a.clov, blk2, k, g.VHpad);
+        Cx<R> xin[N], yout[N];
+#pragma unroll
+        for (int i = 0; i < D; i++) { xin[i] = acc[2 * blk2][i]; xin[D + i] = acc[2 * blk2 + 1][i]; }
+        herm_mv<N, R>(yout, cl, xin);
+#pragma unroll
+        for (int i = 0; i < D; i++) { o[2 * blk2][i] = yout[i]; o[2 * blk2 + 1][i] = yout[D + i]; }
+      }
+      store_spinor<PREC, D, R>(a.out, a.out_nrm, k, g.VHpad, o);
+    } else {
+      constexpr int N = 2 * D;
+      Cx<R> xs[4][D];
+      load_spinor<PREC, D, R>(xs, a.self, a.self_nrm, k, g.VHpad);
+      const R dg = (R)(a.diag + bc_cf_shift(t, g));
+      const bool zs = a.zero_self_bc && bz;
+#pragma unroll
+      for (int blk2 = 0; blk2 < 2; blk2++) {
+        R cl[N * N];
+        load_clover_block<PREC, D, R>(cl, a.clov, blk2, k, g.VHpad);
+        Cx<R> xin[N], yout[N];
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+          xin[i] = zs ? Cx<R>((R)0, (R)0) : xs[2 * blk2][i];
+          xin[D + i] = zs ? Cx<R>((R)0, (R)0) : xs[2 * blk2 + 1][i];
+        }
+        herm_mv<N, R>(yout, cl, xin);
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+          acc[2 * blk2][i] = acc[2 * blk2][i] + yout[i] + dg * xin[i];
+          acc[2 * blk2 + 1][i] = acc[2 * blk2 + 1][i] + yout[D + i] + dg * xin[D + i];
+        }
+      }
+      if (a.gamma5) {
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+          acc[2][i] = Cx<R>(-acc[2][i].re, -acc[2][i].im);
+          acc[3][i] = Cx<R>(-acc[3][i].re, -acc[3][i].im);
+        }
+      }
+      store_spinor<PREC, D, R>(a.out, a.out_nrm, k, g.VHpad, acc);
+      if (a.reduce) {
+        if constexpr (PREC == PREC_F16) {
+          // reduce what was stored (the rounded field), as a consumer sees it
+          Cx<R> back[4][D];
+          load_spinor<PREC, D, R>(back, a.out, a.out_nrm, k, g.VHpad);
+#pragma unroll
+          for (int s = 0; s < 4; s++)
+#pragma unroll
+            for (int i = 0; i < D; i++) nrm2 += (double)back[s][i].re * back[s][i].re + (double)back[s][i].im * back[s][i].im;
+        } else {
+#pragma unroll
+          for (int s = 0; s < 4; s++)
+#pragma unroll
+            for (int i = 0; i < D; i++) nrm2 += (double)acc[s][i].re * acc[s][i].re + (double)acc[s][i].im * acc[s][i].im;
+        }
+      }
+    }
+  }
+  if constexpr (EPI == EPI_CLOV) {
+    if (a.reduce) {
+      double v[1] = {nrm2};
+      block_reduce_finish<1, 256>(v, a.partials, nblk, blockIdx.x, a.ticket, a.red_out);
+    }
+  }
+}
+
+// standalone clover apply (Dphi_ee / Dphi_oo) and inverse (Dphi_oo_inv)
+// (library/dirac.cpp:25-225)
+template <int D, int PREC, bool INV>
+__global__ void __launch_bounds__(256) k_clover(void* out, float* out_nrm, const void* in, const float* in_nrm,
+                                                const void* clov, double diag, int parity, Geom g) {
+  typedef typename SpinorStore<PREC, D>::real R;
+  int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= g.VH) return;
+  constexpr int N = 2 * D;
+  int c[4];
+  site_coords(k, parity, g, c);
+  Cx<R> xs[4][D], o[4][D];
+  load_spinor<PREC, D, R>(xs, in, in_nrm, k, g.VHpad);
+  const R dg = (R)(diag + bc_cf_shift(c[0], g));
+#pragma unroll
+  for (int blk2 = 0; blk2 < 2; blk2++) {
+    R cl[N * N];
+    load_clover_block<PREC, D, R>(cl, clov, blk2, k, g.VHpad);
+    Cx<R> xin[N], yout[N];
+#pragma unroll
+    for (int i = 0; i < D; i++) { xin[i] = xs[2 * blk2][i]; xin[D + i] = xs[2 * blk2 + 1][i]; }
+    herm_mv<N, R>(yout, cl, xin);
+#pragma unroll
+    for (int i = 0; i < D; i++) {
+      if (INV) {
+        o[2 * blk2][i] = yout[i];
+        o[2 * blk2 + 1][i] = yout[D + i];
+      } else {
+        o[2 * blk2][i] = yout[i] + dg * xin[i];
+        o[2 * blk2 + 1][i] = yout[D + i] + dg * xin[D + i];
+      }
+    }
+  }
+  store_spinor<PREC, D, R>(out, out_nrm, k, g.VHpad, o);
+}
+
+// ---------------------------------------------------------------------------
+template <int D, int PREC, int RECON>
+static int hop_dispatch_epi(Ctx* c, const HopArgs& a, int epi, hipStream_t st) {
+  dim3 grid(c->nblk), block(256);
+  switch (epi) {
+    case EPI_PLAIN: hipLaunchKernelGGL((k_hop<D, PREC, RECON, EPI_PLAIN>), grid, block, 0, st, a); break;
+    case EPI_CLOVINV: hipLaunchKernelGGL((k_hop<D, PREC, RECON, EPI_CLOVINV>), grid, block, 0, st, a); break;
+    default: hipLaunchKernelGGL((k_hop<D, PREC, RECON, EPI_CLOV>), grid, block, 0, st, a); break;
+  }
+  YH_HIP(hipGetLastError());
+  return YH_OK;
+}
+
+template <int D>
+int hop_dispatch_d(Ctx* c, const HopArgs& a, int prec, int recon, int epi) {
+  hipStream_t st = c->stream;
+  if (recon == RECON_12) {
+    if constexpr (D == 3) {
+      if (prec == PREC_F64) return hop_dispatch_epi<3, PREC_F64, RECON_12>(c, a, epi, st);
+      if (prec == PREC_F32) return hop_dispatch_epi<3, PREC_F32, RECON_12>(c, a, epi, st);
+      return hop_dispatch_epi<3, PREC_F16, RECON_12>(c, a, epi, st);
+    }
+    set_error("12-real links need SU(3) fundamental");
+    return YH_ERR_UNSUPPORTED;
+  }
+  if (prec == PREC_F64) return hop_dispatch_epi<D, PREC_F64, RECON_18>(c, a, epi, st);
+  if (prec == PREC_F32) return hop_dispatch_epi<D, PREC_F32, RECON_18>(c, a, epi, st);
+  if constexpr (D <= 4) return hop_dispatch_epi<D, PREC_F16, RECON_18>(c, a, epi, st);
+  set_error("FP16 fields are built for representation dimension <= 4");
+  return YH_ERR_UNSUPPORTED;
+}
+
+template <int D>
+int clover_apply_d(Ctx* c, int prec, int parity, double diag, Field* out, const Field* in, int inverse) {
+  int N = 2 * c->dr;
+  int cprec = prec == PREC_F64 ? PREC_F64 : PREC_F32;
+  size_t cvb = cprec == PREC_F64 ? sizeof(double2) : sizeof(float4);
+  size_t cnv = cprec == PREC_F64 ? (size_t)N * N : (size_t)N * N / 2;
+  const void* cl;
+  if (inverse) cl = c->clinv[cprec];
+  else cl = (const char*)c->clov[cprec] + (size_t)parity * cnv * c->g.VHpad * cvb;
+  if (!cl) { set_error("clover data missing"); return YH_ERR_STATE; }
+  dim3 grid((c->g.VH + 255) / 256);
+  void* o = out->half[parity];
+  float* on = out->nrm[parity];
+  const void* i = in->half[parity];
+  const float* inn = in->nrm[parity];
+#define YH_CL(P)                                                                                               \
+  if (inverse) hipLaunchKernelGGL((k_clover<D, P, true>), grid, dim3(256), 0, c->stream, o, on, i, inn, cl, diag, parity, c->g); \
+  else hipLaunchKernelGGL((k_clover<D, P, false>), grid, dim3(256), 0, c->stream, o, on, i, inn, cl, diag, parity, c->g);
+  if (prec == PREC_F64) { YH_CL(PREC_F64) }
+  else if (prec == PREC_F32) { YH_CL(PREC_F32) }
+  else {
+    if constexpr (D <= 4) { YH_CL(PREC_F16) }
+    else { set_error("FP16 fields are built for representation dimension <= 4"); return YH_ERR_UNSUPPORTED; }
+  }
+#undef YH_CL
+  YH_HIP(hipGetLastError());
+  return YH_OK;
+}
+
+}  // namespace yh

@@ -1,0 +1,147 @@
+// Host-side context, field objects and the internal launch wrappers shared
+// by the translation units of libyahmc.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <string>
+#include "yh_common.hpp"
+#include "../../include/yahmc.h"
+
+namespace yh {
+
+#define YH_HIP(call)                                                         \
+  do {                                                                       \
+    hipError_t e_ = (call);                                                  \
+    if (e_ != hipSuccess) {                                                  \
+      set_error(std::string(#call) + ": " + hipGetErrorString(e_));          \
+      return YH_ERR_HIP;                                                     \
+    }                                                                        \
+  } while (0)
+
+void set_error(const std::string& msg);
+
+// a device spinor field: both parity halves, one precision
+struct Field {
+  int prec;
+  void* half[2];     // even, odd
+  float* nrm[2];     // FP16 per-site scale (null otherwise)
+  void* mem;         // single allocation
+  size_t bytes;
+};
+
+enum { RED_SLOTS = 64 };
+
+// scalars of the device-resident solvers (FP64, one cache line each group)
+struct SolverScalars {
+  double rr;        // |r|^2 of the current iterate
+  double rr_new;    // |r|^2 after the update
+  double pap;       // <p, A p>
+  double alpha, beta;
+  double target;    // prec * |b|^2
+  double norm_b;
+  double maxrr;     // reliable-update reference norm
+  double red[RED_SLOTS];  // general reduction results
+  int converged;
+  int iters;
+  int pad[14];
+};
+
+struct Ctx {
+  Geom g;
+  int nc, dr, repr;
+  double csw;
+  int device;
+  hipStream_t stream;
+  int nblk;            // blocks of 256 threads over one half
+  // gauge field: fundamental links in the reference layout (4V suNg)
+  double* fund = nullptr;
+  // represented links, SoA, per (prec, recon); null if not built
+  void* links[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
+  // packed clover term (no mass) for both parities, per precision
+  void* clov[3] = {nullptr, nullptr, nullptr};
+  // packed inverse of the odd-site diagonal at sigma = cl_sigma
+  void* clinv[3] = {nullptr, nullptr, nullptr};
+  double cl_sigma = 0xF00F;
+  double* logdet_site = nullptr;  // per odd site log det (for the sum)
+  bool gauge_set = false;
+  // reductions
+  double* partials = nullptr;      // nblk * RED_SLOTS doubles
+  unsigned* ticket = nullptr;      // last-block counter
+  SolverScalars* sc = nullptr;     // device scalars
+  SolverScalars* sc_host = nullptr;  // pinned mirror
+  long mvm_counter = 0;
+  // operator scratch (odd halves / even halves)
+  Field* tmp[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+};
+
+// ---- fields ---------------------------------------------------------------
+int field_alloc(Ctx* c, int prec, Field** out);
+void field_free(Field* f);
+size_t half_bytes(const Ctx* c, int prec);
+int field_zero(Ctx* c, Field* f, int parity_mask);
+int field_upload(Ctx* c, Field* f, const double* host, int parity_mask);
+int field_download(Ctx* c, const Field* f, double* host, int parity_mask);
+int field_convert(Ctx* c, Field* dst, int dpar, const Field* src, int spar);
+
+// ---- gauge ------------------------------------------------------------------
+int gauge_init_basis(Ctx* c);
+int gauge_represent(Ctx* c);                 // fund -> F64 R18 links
+int gauge_derive_links(Ctx* c);              // F64 R18 -> other precisions
+int clover_build(Ctx* c);                    // packed FP64 clover
+int clover_invert(Ctx* c, double sigma);     // odd-site inverses (+ logdet)
+int clover_logdet_sum(Ctx* c, double* out);
+int gauge_download_repr(Ctx* c, double* host);    // reference gfield_repr layout, BC sign applied
+int clover_download(Ctx* c, double* host);         // reference cl_term layout
+
+// ---- dslash -----------------------------------------------------------------
+enum { EPI_PLAIN = 0, EPI_CLOVINV = 1, EPI_CLOV = 2 };
+struct HopCall {
+  int prec, recon;
+  int out_parity;
+  const Field* in;        // reads half[1 - out_parity]
+  Field* out;             // writes half[out_parity]
+  Field* aux;             // EPI_CLOVINV: optional copy of the plain hop result (same half)
+  const Field* self;      // EPI_CLOV: same-parity input for the clover term
+  double scale;           // multiplies the hop sum
+  double diag;            // 4 + m  (EPI_CLOV)
+  int epi;
+  int gamma5;             // EPI_CLOV: apply gamma5 to the result
+  int zero_self_bc;       // EPI_CLOV: the reference zeroes boundary slices of the input
+  int reduce_norm;        // EPI_CLOV: |out|^2 into sc->red[reduce_slot]
+  int reduce_slot;
+  const int* skip_flag;   // if non-null and *skip_flag, every thread returns
+};
+int hop_launch(Ctx* c, const HopCall& h);
+int clover_apply(Ctx* c, int prec, int parity, double diag, Field* out, const Field* in, int inverse);
+
+// ---- blas -------------------------------------------------------------------
+int blas_launch_reduce_finish(Ctx* c);
+int blas_sqnorm(Ctx* c, const Field* a, int parity, int slot);
+int blas_dot(Ctx* c, const Field* a, const Field* b, int parity, int slot);       // complex -> slot, slot+1
+int blas_dot_re(Ctx* c, const Field* a, const Field* b, int parity, int slot);
+int blas_axpby(Ctx* c, Field* y, double a, const Field* x, double b, int parity);  // y = a x + b y
+int blas_caxpy(Ctx* c, Field* y, double ar, double ai, const Field* x, int parity); // y += (ar + i ai) x
+int blas_cscale(Ctx* c, Field* y, double ar, double ai, int parity);
+int blas_g5(Ctx* c, Field* f, int parity);
+int blas_copy(Ctx* c, Field* dst, const Field* src, int parity);
+
+// ---- host helpers shared by the solver translation units -------------------
+Field* tmp_field(Ctx* c, int prec, int slot, int* rc);
+int default_recon(Ctx* c, int prec);
+int apply_op(Ctx* c, int prec, int recon, int mvm, double mass, Field* out, Field* in, Field* side, int reduce_slot,
+             const int* skip);
+int read_scalars(Ctx* c);
+int write_scalars(Ctx* c);
+int reduce_read(Ctx* c, int which, const Field* a, const Field* b, int par, double* out2);
+int cg_device(Ctx* c, int prec, int recon, const yh_inv_par* par, Field* x, const Field* b, int* iters,
+              double* final_rr, double* norm_b_out);
+
+// CG step kernels operating on device scalars (no host round trip)
+int cg_update_xr(Ctx* c, Field* x, Field* r, const Field* p, const Field* q, int parity, int prec);
+int cg_update_p(Ctx* c, Field* p, const Field* r, int parity, int prec);
+
+}  // namespace yh
+
+// the opaque C handles are the internal objects
+struct yh_ctx : public yh::Ctx {};
+struct yh_field : public yh::Field {};

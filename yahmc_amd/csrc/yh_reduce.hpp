@@ -1,0 +1,68 @@
+// Deterministic grid reduction: wave shuffle -> LDS -> one FP64 partial per
+// workgroup -> the last workgroup to arrive (ticket counter) sums the
+// partials in a fixed order and writes the result.  The result therefore
+// does not depend on scheduling, and no host round trip is needed.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace yh {
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// NR values per thread; partials holds nblk * NR doubles.
+// Returns true in the (whole) last block after out[] has been written.
+template <int NR, int NT>
+__device__ bool block_reduce_finish(double (&v)[NR], double* partials, int nblk, int blk, unsigned* ticket,
+                                    double* out) {
+  __shared__ double sm[NR][NT / 64];
+  __shared__ int is_last;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    double s = wave_sum(v[r]);
+    if (lane == 0) sm[r][w] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+      double s = 0.0;
+#pragma unroll
+      for (int q = 0; q < NT / 64; q++) s += sm[r][q];
+      partials[(size_t)r * nblk + blk] = s;
+    }
+    __threadfence();
+    unsigned t = atomicAdd(ticket, 1u);
+    is_last = (t == (unsigned)(nblk - 1));
+  }
+  __syncthreads();
+  if (!is_last) return false;
+  __threadfence();
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nblk; i += NT) s += __hip_atomic_load(&partials[(size_t)r * nblk + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s = wave_sum(s);
+    __syncthreads();
+    if (lane == 0) sm[r][w] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double tot = 0.0;
+#pragma unroll
+      for (int q = 0; q < NT / 64; q++) tot += sm[r][q];
+      out[r] = tot;
+    }
+  }
+  if (threadIdx.x == 0) {
+    atomicExch(ticket, 0u);
+    __threadfence();
+  }
+  __syncthreads();
+  return true;
+}
+
+}  // namespace yh

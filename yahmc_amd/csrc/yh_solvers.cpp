@@ -1,0 +1,277 @@
+// BiCGstab, multishift CG and the mixed-precision reliable-update CG.
+//
+// bicgstab_inverter : library/bicgstab.cpp:9-97
+// cg_mshift         : library/cg.cpp:11-130
+// cg (mixed)        : project-new; FP64 true residual b - A x recomputed with
+//                     the FP64 operator at every reliable update, sloppy
+//                     FP32 / FP16 operator and vectors in between
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+#include <complex>
+#include <vector>
+#include "yh_internal.hpp"
+
+using namespace yh;
+typedef std::complex<double> zd;
+
+namespace {
+
+int need(Ctx* c, int prec, int slot, Field** f) {
+  int rc;
+  *f = tmp_field(c, prec, slot, &rc);
+  return rc;
+}
+
+int sync(Ctx* c) {
+  YH_HIP(hipStreamSynchronize(c->stream));
+  return YH_OK;
+}
+
+double sqnorm(Ctx* c, const Field* a, int* rc) {
+  double v[2] = {0, 0};
+  *rc = reduce_read(c, 0, a, nullptr, 0, v);
+  return v[0];
+}
+
+zd dot(Ctx* c, const Field* a, const Field* b, int* rc) {
+  double v[2] = {0, 0};
+  *rc = reduce_read(c, 1, a, b, 0, v);
+  return zd(v[0], v[1]);
+}
+
+double dot_re(Ctx* c, const Field* a, const Field* b, int* rc) {
+  double v[2] = {0, 0};
+  *rc = reduce_read(c, 2, a, b, 0, v);
+  return v[0];
+}
+
+}  // namespace
+
+extern "C" {
+
+int yh_bicgstab(yh_ctx* cc, const yh_inv_par* par, yh_field* dptr, const yh_field* sptr, int* iters) {
+  Ctx* c = cc;
+  if (!c || !par || !dptr || !sptr || !iters) { set_error("null argument"); return YH_ERR_ARG; }
+  if (!c->gauge_set) { set_error("no gauge field"); return YH_ERR_STATE; }
+  if (dptr->prec != PREC_F64 || sptr->prec != PREC_F64) { set_error("solver fields must be FP64"); return YH_ERR_ARG; }
+  if (!(par->prec > 0.0)) { set_error("prec must be positive"); return YH_ERR_ARG; }
+  int rc;
+  Field *r, *r0, *v, *p, *t;
+  if ((rc = need(c, PREC_F64, 2, &r)) || (rc = need(c, PREC_F64, 3, &p)) || (rc = need(c, PREC_F64, 4, &v)) ||
+      (rc = need(c, PREC_F64, 5, &r0)) || (rc = need(c, PREC_F64, 6, &t)))
+    return rc;
+  const int mvm = par->mvm;
+  const int recon = default_recon(c, PREC_F64);
+  Field* x = dptr;
+  // r = b - A x ; r0 = r ; p = r
+  if ((rc = apply_op(c, PREC_F64, recon, mvm, par->mass, p, x, nullptr, -1, nullptr))) return rc;
+  if ((rc = blas_copy(c, r, sptr, 0))) return rc;
+  if ((rc = blas_axpby(c, r, -1.0, p, 1.0, 0))) return rc;
+  if ((rc = blas_copy(c, r0, r, 0))) return rc;
+  if ((rc = blas_copy(c, p, r, 0))) return rc;
+  double norm = sqnorm(c, sptr, &rc);
+  if (rc) return rc;
+  zd rho0 = dot(c, r, r0, &rc);
+  if (rc) return rc;
+  double residue = rho0.real();
+  int it = 0;
+  if (residue < norm * par->prec) it = -1;
+  while (++it) {
+    if ((rc = apply_op(c, PREC_F64, recon, mvm, par->mass, v, p, nullptr, -1, nullptr))) return rc;
+    zd alpha = dot(c, v, r0, &rc);
+    if (rc) return rc;
+    alpha = rho0 / alpha;
+    if ((rc = blas_caxpy(c, r, -alpha.real(), -alpha.imag(), v, 0))) return rc;
+    if ((rc = apply_op(c, PREC_F64, recon, mvm, par->mass, t, r, nullptr, -1, nullptr))) return rc;
+    zd omega = dot(c, r, t, &rc);
+    if (rc) return rc;
+    double tt = sqnorm(c, t, &rc);
+    if (rc) return rc;
+    omega /= tt;
+    if ((rc = blas_caxpy(c, x, alpha.real(), alpha.imag(), p, 0))) return rc;
+    if ((rc = blas_caxpy(c, x, omega.real(), omega.imag(), r, 0))) return rc;
+    if ((rc = blas_caxpy(c, r, -omega.real(), -omega.imag(), t, 0))) return rc;
+    zd rho1 = dot(c, r, r0, &rc);
+    if (rc) return rc;
+    zd beta = (rho1 / rho0) * (alpha / omega);
+    if ((rc = blas_cscale(c, p, beta.real(), beta.imag(), 0))) return rc;
+    if ((rc = blas_axpby(c, p, 1.0, r, 1.0, 0))) return rc;
+    zd bo = beta * omega;
+    if ((rc = blas_caxpy(c, p, -bo.real(), -bo.imag(), v, 0))) return rc;
+    residue = sqnorm(c, r, &rc);
+    if (rc) return rc;
+    if (!(residue == residue)) { set_error("bicgstab: non-finite residual"); return YH_ERR_NONFINITE; }
+    if (residue < norm * par->prec) break;
+    if (par->max_iter > 0 && it >= par->max_iter) {
+      *iters = 2 * it;
+      set_error("bicgstab: max_iter reached");
+      return YH_ERR_NOCONV;
+    }
+    rho0 = rho1;
+  }
+  *iters = 2 * it;
+  if (par->log) fprintf(stderr, "[INVERTER][NOTICE] bicgstab: iterations = %d, precision = %1.4e\n", 2 * it, residue / norm);
+  return sync(c);
+}
+
+int yh_cg_mshift(yh_ctx* cc, const yh_inv_par* par, int num, const double* shift, yh_field** dptr,
+                 const yh_field* sptr, int* iters) {
+  Ctx* c = cc;
+  if (!c || !par || !shift || !dptr || !sptr || !iters || num < 1 || num > 64) { set_error("bad argument"); return YH_ERR_ARG; }
+  if (!c->gauge_set) { set_error("no gauge field"); return YH_ERR_STATE; }
+  if (!(par->prec > 0.0)) { set_error("prec must be positive"); return YH_ERR_ARG; }
+  for (int n = 0; n < num; n++)
+    if (!dptr[n] || dptr[n]->prec != PREC_F64) { set_error("solution fields must be FP64"); return YH_ERR_ARG; }
+  int rc;
+  Field *r, *p, *q;
+  if ((rc = need(c, PREC_F64, 2, &r)) || (rc = need(c, PREC_F64, 3, &p)) || (rc = need(c, PREC_F64, 4, &q))) return rc;
+  std::vector<Field*> k(num);
+  for (int n = 0; n < num; n++)
+    if ((rc = field_alloc(c, PREC_F64, &k[n]))) {
+      for (int m = 0; m < n; m++) field_free(k[m]);
+      return rc;
+    }
+  auto cleanup = [&]() { for (auto f : k) field_free(f); };
+  const int recon = default_recon(c, PREC_F64);
+  std::vector<double> c1(num, 1.0), c2(num, 1.0), c3(num, 0.0);
+  std::vector<int> cflag(num, 0);
+  if ((rc = blas_copy(c, r, sptr, 0)) || (rc = blas_copy(c, p, sptr, 0))) { cleanup(); return rc; }
+  for (int n = 0; n < num; n++) {
+    if ((rc = field_zero(c, dptr[n], YH_BOTH)) || (rc = blas_copy(c, k[n], sptr, 0))) { cleanup(); return rc; }
+  }
+  double beta = 1.0, alpha = 0.0, beta_old, atmp, btmp = 0.0;
+  double norm = sqnorm(c, sptr, &rc);
+  if (rc) { cleanup(); return rc; }
+  int it = 0;
+  while (++it) {
+    if ((rc = apply_op(c, PREC_F64, recon, par->mvm, par->mass, q, p, nullptr, -1, nullptr))) { cleanup(); return rc; }
+    atmp = sqnorm(c, r, &rc);
+    if (!rc) btmp = dot_re(c, p, q, &rc);
+    if (rc) { cleanup(); return rc; }
+    beta_old = beta;
+    beta = -(atmp / btmp);
+    for (int n = 0; n < num; n++) {
+      if (cflag[n]) continue;
+      c3[n] = c2[n] * c1[n] * beta_old;
+      c3[n] /= (beta * alpha * (c1[n] - c2[n])) + (c1[n] * beta_old * (1 - shift[n] * beta));
+      double bt = beta * (c3[n] / c2[n]);
+      if ((rc = blas_axpby(c, dptr[n], -bt, k[n], 1.0, 0))) { cleanup(); return rc; }
+    }
+    if ((rc = blas_axpby(c, r, beta, q, 1.0, 0))) { cleanup(); return rc; }
+    btmp = sqnorm(c, r, &rc);
+    if (rc) { cleanup(); return rc; }
+    if (!(btmp == btmp)) { cleanup(); set_error("cg_mshift: non-finite residual"); return YH_ERR_NONFINITE; }
+    alpha = btmp / atmp;
+    int notconverged = 0;
+    for (int n = 0; n < num; n++) {
+      if (cflag[n]) continue;
+      double at = alpha * (c3[n] * c3[n]) / (c2[n] * c2[n]);
+      if ((rc = blas_axpby(c, k[n], c3[n], r, at, 0))) { cleanup(); return rc; }
+      if (btmp * c3[n] * c3[n] < norm * par->prec) cflag[n] = 1;
+      else notconverged++;
+      c1[n] = c2[n];
+      c2[n] = c3[n];
+    }
+    if (notconverged == 0) break;
+    if (par->max_iter > 0 && it >= par->max_iter) {
+      cleanup();
+      *iters = it;
+      set_error("cg_mshift: max_iter reached");
+      return YH_ERR_NOCONV;
+    }
+    if ((rc = blas_axpby(c, p, 1.0, r, alpha, 0))) { cleanup(); return rc; }
+  }
+  rc = sync(c);
+  cleanup();
+  *iters = it;
+  if (par->log) fprintf(stderr, "[INVERTER][NOTICE] cg_mshift: iterations = %d, precision = %1.4e\n", it, btmp / norm);
+  return rc;
+}
+
+int yh_cg_mixed(yh_ctx* cc, const yh_inv_par* par, yh_field* dptr, const yh_field* sptr, int* iters, int* updates) {
+  Ctx* c = cc;
+  if (!c || !par || !dptr || !sptr || !iters) { set_error("null argument"); return YH_ERR_ARG; }
+  if (!c->gauge_set) { set_error("no gauge field"); return YH_ERR_STATE; }
+  if (dptr->prec != PREC_F64 || sptr->prec != PREC_F64) { set_error("solver fields must be FP64"); return YH_ERR_ARG; }
+  if (par->mvm != YH_MVM_HPHI_SQ) { set_error("mixed CG needs the Hermitian operator Hphi_sq"); return YH_ERR_ARG; }
+  if (par->sloppy_prec != PREC_F32 && par->sloppy_prec != PREC_F16) { set_error("sloppy precision must be F32 or F16"); return YH_ERR_ARG; }
+  if (!(par->prec > 0.0)) { set_error("prec must be positive"); return YH_ERR_ARG; }
+  const int S = par->sloppy_prec;
+  const int K = par->reliable_every > 0 ? par->reliable_every : 50;
+  int rc;
+  Field *r64, *t64, *xs, *rs, *p, *q;
+  if ((rc = need(c, PREC_F64, 2, &r64)) || (rc = need(c, PREC_F64, 3, &t64))) return rc;
+  if ((rc = need(c, PREC_F32, 2, &xs)) || (rc = need(c, PREC_F32, 3, &rs))) return rc;
+  if (S == PREC_F32) {
+    if ((rc = need(c, PREC_F32, 4, &p)) || (rc = need(c, PREC_F32, 5, &q))) return rc;
+  } else {
+    if ((rc = need(c, PREC_F16, 2, &p)) || (rc = need(c, PREC_F16, 3, &q))) return rc;
+  }
+  const int recon64 = default_recon(c, PREC_F64);
+  const int recons = default_recon(c, S);
+  Field* x = dptr;
+  double norm_b = sqnorm(c, sptr, &rc);
+  if (rc) return rc;
+  const double target = norm_b * par->prec;
+  auto true_residual = [&](double* rr) -> int {
+    int e;
+    if ((e = apply_op(c, PREC_F64, recon64, YH_MVM_HPHI_SQ, par->mass, t64, x, nullptr, -1, nullptr))) return e;
+    if ((e = blas_copy(c, r64, sptr, 0))) return e;
+    if ((e = blas_axpby(c, r64, -1.0, t64, 1.0, 0))) return e;
+    *rr = sqnorm(c, r64, &e);
+    return e;
+  };
+  double rr;
+  if ((rc = true_residual(&rr))) return rc;
+  int nupd = 0;
+  *iters = 0;
+  if (rr < target) {
+    if (updates) *updates = 0;
+    return YH_OK;
+  }
+  if ((rc = field_convert(c, rs, 0, r64, 0)) || (rc = field_convert(c, p, 0, r64, 0)) || (rc = field_zero(c, xs, YH_EVEN)))
+    return rc;
+  SolverScalars* h = c->sc_host;
+  memset(h, 0, sizeof(SolverScalars));
+  h->rr = rr;
+  h->target = target;
+  h->norm_b = norm_b;
+  if ((rc = write_scalars(c))) return rc;
+  int total = 0;
+  while (true) {
+    for (int i = 0; i < K; i++) {
+      if ((rc = apply_op(c, S, recons, YH_MVM_HPHI_SQ, par->mass, q, p, nullptr, 0, &c->sc->converged))) return rc;
+      if ((rc = cg_update_xr(c, xs, rs, p, q, 0, PREC_F32))) return rc;
+      if ((rc = cg_update_p(c, p, rs, 0, PREC_F32))) return rc;
+    }
+    if ((rc = read_scalars(c))) return rc;
+    total = h->iters;
+    if (!(h->rr == h->rr)) { set_error("cg_mixed: non-finite residual"); return YH_ERR_NONFINITE; }
+    // reliable update: x += xs ; r = b - A x in FP64 ; restart the sloppy residual
+    if ((rc = field_convert(c, t64, 0, xs, 0)) || (rc = blas_axpby(c, x, 1.0, t64, 1.0, 0))) return rc;
+    if ((rc = true_residual(&rr))) return rc;
+    nupd++;
+    if (!(rr == rr)) { set_error("cg_mixed: non-finite true residual"); return YH_ERR_NONFINITE; }
+    if (rr < target) break;
+    if (par->max_iter > 0 && total >= par->max_iter) {
+      *iters = total;
+      if (updates) *updates = nupd;
+      set_error("cg_mixed: max_iter reached");
+      return YH_ERR_NOCONV;
+    }
+    if ((rc = field_convert(c, rs, 0, r64, 0)) || (rc = field_zero(c, xs, YH_EVEN))) return rc;
+    h->rr = rr;
+    h->converged = 0;
+    h->iters = total;
+    if ((rc = write_scalars(c))) return rc;
+  }
+  *iters = total;
+  if (updates) *updates = nupd;
+  if (par->log)
+    fprintf(stderr, "[INVERTER][NOTICE] cg_mixed: iterations = %d, updates = %d, precision = %1.4e\n", total, nupd,
+            rr / norm_b);
+  return sync(c);
+}
+
+}  // extern "C"
