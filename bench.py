@@ -122,9 +122,9 @@ def main():
         b = gaussian_source(DIMS, 3, seed=32246)
         fb = lat.field("f64").upload(b)
         prec = 1e-20
-        for name, fn in (("fp64", lambda fx: (lat.cg_inverter(lat.inv_par(MASS, prec, "Hphi_sq"), fx, fb), 0)),
-                         ("mixed_f32", lambda fx: lat.cg_mixed(lat.inv_par(MASS, prec, "Hphi_sq", sloppy="f32"), fx, fb)),
-                         ("mixed_f16", lambda fx: lat.cg_mixed(lat.inv_par(MASS, prec, "Hphi_sq", sloppy="f16"), fx, fb))):
+        for name, fn in (("fp64", lambda fx: (lat.cg_inverter(lat.inv_par(MASS, prec, "Hphi_sq", max_iter=20000), fx, fb), 0)),
+                         ("mixed_f32", lambda fx: lat.cg_mixed(lat.inv_par(MASS, prec, "Hphi_sq", sloppy="f32", max_iter=20000), fx, fb)),
+                         ("mixed_f16", lambda fx: lat.cg_mixed(lat.inv_par(MASS, prec, "Hphi_sq", sloppy="f16", max_iter=20000), fx, fb))):
             fx = lat.field("f64")
             lat.synchronize()
             t = time.perf_counter()

@@ -227,7 +227,7 @@ def test_cg_mixed_true_residual(sloppy):
     b = src_field(o, rng, even_only=True)
     fb = lat.field("f64").upload(b)
     fx = lat.field("f64")
-    it, upd = lat.cg_mixed(lat.inv_par(mass, prec, "Hphi_sq", sloppy=sloppy, reliable_every=50), fx, fb)
+    it, upd = lat.cg_mixed(lat.inv_par(mass, prec, "Hphi_sq", sloppy=sloppy, reliable_every=50, max_iter=5000, log=1), fx, fb)
     x = fx.download()
     r = true_res(o, MVM_HPHI_SQ, mass, x, b)
     print(f"mixed {sloppy}: iterations={it} updates={upd} true residual^2={r:.3e}")

@@ -158,7 +158,13 @@ __global__ void __launch_bounds__(256) k_cg_xr(void* x, float* xn, void* r, floa
       sc->rr_new = rr_new;
       sc->rr = rr_new;
       sc->iters += 1;
-      if (rr_new < sc->target || !(rr_new == rr_new)) sc->converged = 1;
+      if (rr_new < sc->target || !(rr_new == rr_new)) {
+        sc->converged = 1;
+        sc->reason = 1;
+      } else if (sc->delta2 > 0.0 && rr_new < sc->delta2 * sc->maxrr) {
+        sc->converged = 1;
+        sc->reason = 2;
+      }
       if (rr_new > sc->maxrr) sc->maxrr = rr_new;
     }
   }

@@ -517,6 +517,7 @@ __global__ void k_clover_to_f32(const double2* __restrict__ src, float4* __restr
   YH_DISPATCH_MODEL(4, YH_REPR_ANTISYMMETRIC, 6, __VA_ARGS__)
 
 static int grid1(long n) { return (int)((n + 255) / 256); }
+static int grid_bs(long n, int bs) { return (int)((n + bs - 1) / bs); }
 
 int gauge_represent(Ctx* c) {
   if (!c->links[PREC_F64][RECON_18]) {
@@ -586,7 +587,7 @@ int clover_build(Ctx* c) {
   c->cl_sigma = 0xF00F;  // invalidate the inverse (clover.cpp:337)
   int rc = YH_OK;
   auto launch = [&]() -> int {
-    YH_ALL_MODELS(hipLaunchKernelGGL((k_clover_build<NC_, REPR_, D_>), dim3(grid1(c->g.V)), dim3(128), 0, c->stream,
+    YH_ALL_MODELS(hipLaunchKernelGGL((k_clover_build<NC_, REPR_, D_>), dim3(grid_bs(c->g.V, 128)), dim3(128), 0, c->stream,
                                      c->fund, out, c->g, c->csw);
                   YH_HIP(hipGetLastError()));
     set_error("unsupported (nc, repr) model");
@@ -615,10 +616,10 @@ int clover_invert(Ctx* c, double sigma) {
   const double2* src = (const double2*)c->clov[PREC_F64];
   double2* inv = (double2*)c->clinv[PREC_F64];
   switch (c->dr) {
-    case 2: hipLaunchKernelGGL(k_clover_invert<2>, dim3(grid1(c->g.VH)), dim3(64), 0, c->stream, src, inv, c->logdet_site, c->g, sigma); break;
-    case 3: hipLaunchKernelGGL(k_clover_invert<3>, dim3(grid1(c->g.VH)), dim3(64), 0, c->stream, src, inv, c->logdet_site, c->g, sigma); break;
-    case 4: hipLaunchKernelGGL(k_clover_invert<4>, dim3(grid1(c->g.VH)), dim3(64), 0, c->stream, src, inv, c->logdet_site, c->g, sigma); break;
-    case 6: hipLaunchKernelGGL(k_clover_invert<6>, dim3(grid1(c->g.VH)), dim3(64), 0, c->stream, src, inv, c->logdet_site, c->g, sigma); break;
+    case 2: hipLaunchKernelGGL(k_clover_invert<2>, dim3(grid_bs(c->g.VH, 64)), dim3(64), 0, c->stream, src, inv, c->logdet_site, c->g, sigma); break;
+    case 3: hipLaunchKernelGGL(k_clover_invert<3>, dim3(grid_bs(c->g.VH, 64)), dim3(64), 0, c->stream, src, inv, c->logdet_site, c->g, sigma); break;
+    case 4: hipLaunchKernelGGL(k_clover_invert<4>, dim3(grid_bs(c->g.VH, 64)), dim3(64), 0, c->stream, src, inv, c->logdet_site, c->g, sigma); break;
+    case 6: hipLaunchKernelGGL(k_clover_invert<6>, dim3(grid_bs(c->g.VH, 64)), dim3(64), 0, c->stream, src, inv, c->logdet_site, c->g, sigma); break;
     default: set_error("unsupported representation dimension"); return YH_ERR_UNSUPPORTED;
   }
   YH_HIP(hipGetLastError());
@@ -698,7 +699,7 @@ int clover_download(Ctx* c, double* host) {
   size_t n = (size_t)4 * c->g.V * 2 * c->dr * c->dr;
   double* d;
   YH_HIP(hipMalloc(&d, n * sizeof(double)));
-  hipLaunchKernelGGL(k_clover_to_aos, dim3(grid1(c->g.V)), dim3(64), 0, c->stream, (const double2*)c->clov[PREC_F64],
+  hipLaunchKernelGGL(k_clover_to_aos, dim3(grid_bs(c->g.V, 64)), dim3(64), 0, c->stream, (const double2*)c->clov[PREC_F64],
                      d, c->g, c->dr);
   hipError_t e = hipMemcpyAsync(host, d, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);

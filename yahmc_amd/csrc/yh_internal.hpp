@@ -39,11 +39,13 @@ struct SolverScalars {
   double alpha, beta;
   double target;    // prec * |b|^2
   double norm_b;
-  double maxrr;     // reliable-update reference norm
+  double maxrr;     // reliable-update reference norm: max |r|^2 since the last update
+  double delta2;    // stop for a reliable update once |r|^2 < delta2 * maxrr (0: off)
   double red[RED_SLOTS];  // general reduction results
-  int converged;
+  int converged;    // stop flag: every solver kernel returns at once while set
   int iters;
-  int pad[14];
+  int reason;       // why `converged` was set: 1 target reached / non-finite, 2 reliable update due
+  int pad[13];
 };
 
 struct Ctx {

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>
+#include <algorithm>
 #include <complex>
 #include <vector>
 #include "yh_internal.hpp"
@@ -232,22 +233,43 @@ int yh_cg_mixed(yh_ctx* cc, const yh_inv_par* par, yh_field* dptr, const yh_fiel
   }
   if ((rc = field_convert(c, rs, 0, r64, 0)) || (rc = field_convert(c, p, 0, r64, 0)) || (rc = field_zero(c, xs, YH_EVEN)))
     return rc;
+  // Reliable updates (Sleijpen & van der Vorst; Clark et al. for lattice
+  // QCD): the sloppy recurrence runs on device until |r| has dropped by
+  // `delta` since the last update (flag raised by the CG update kernel) or
+  // `reliable_every` iterations have passed; then x += xs and the residual
+  // is recomputed in FP64.  The search direction p is kept.
+  const double delta = (par->delta > 0.0 && par->delta < 1.0) ? par->delta : 0.1;
   SolverScalars* h = c->sc_host;
   memset(h, 0, sizeof(SolverScalars));
   h->rr = rr;
+  h->maxrr = rr;
+  h->delta2 = delta * delta;
   h->target = target;
   h->norm_b = norm_b;
   if ((rc = write_scalars(c))) return rc;
-  int total = 0;
+  int total = 0, at_update = 0;
+  const int chunk = 16;
   while (true) {
-    for (int i = 0; i < K; i++) {
+    int n = std::min(chunk, std::max(1, K - (total - at_update)));
+    for (int i = 0; i < n; i++) {
       if ((rc = apply_op(c, S, recons, YH_MVM_HPHI_SQ, par->mass, q, p, nullptr, 0, &c->sc->converged))) return rc;
       if ((rc = cg_update_xr(c, xs, rs, p, q, 0, PREC_F32))) return rc;
       if ((rc = cg_update_p(c, p, rs, 0, PREC_F32))) return rc;
     }
     if ((rc = read_scalars(c))) return rc;
     total = h->iters;
+    if (par->log)
+      fprintf(stderr, "[INVERTER][DEBUG] cg_mixed: iters=%d rr=%.6e pap=%.6e alpha=%.6e beta=%.6e stop=%d reason=%d\n",
+              h->iters, h->rr, h->red[0], h->alpha, h->beta, h->converged, h->reason);
     if (!(h->rr == h->rr)) { set_error("cg_mixed: non-finite residual"); return YH_ERR_NONFINITE; }
+    if (!h->converged && total - at_update < K) continue;
+    if (h->converged) {
+      // the iteration that raised the stop flag skipped its p = r + beta p
+      // step; complete it with the iterated residual before r is replaced
+      h->converged = 0;
+      if ((rc = write_scalars(c))) return rc;
+      if ((rc = cg_update_p(c, p, rs, 0, PREC_F32))) return rc;
+    }
     // reliable update: x += xs ; r = b - A x in FP64 ; restart the sloppy residual
     if ((rc = field_convert(c, t64, 0, xs, 0)) || (rc = blas_axpby(c, x, 1.0, t64, 1.0, 0))) return rc;
     if ((rc = true_residual(&rr))) return rc;
@@ -262,8 +284,11 @@ int yh_cg_mixed(yh_ctx* cc, const yh_inv_par* par, yh_field* dptr, const yh_fiel
     }
     if ((rc = field_convert(c, rs, 0, r64, 0)) || (rc = field_zero(c, xs, YH_EVEN))) return rc;
     h->rr = rr;
+    h->maxrr = rr;
     h->converged = 0;
+    h->reason = 0;
     h->iters = total;
+    at_update = total;
     if ((rc = write_scalars(c))) return rc;
   }
   *iters = total;
