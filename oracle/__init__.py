@@ -91,6 +91,14 @@ class Oracle:
                      "or_apply_bcs_on_spinor", "or_g5"):
             getattr(lib, name).argtypes = [_D]
         lib.or_coords.argtypes = [ctypes.c_int, _I]
+        lib.or_clover_fermion_force.argtypes = [ctypes.c_double, _D, _D]
+        lib.or_wilson_fermion_force.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, _D, _D]
+        lib.or_wilson_fermion_force_q.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, _D]
+        lib.or_clover_logdet_force.argtypes = [ctypes.c_double, ctypes.c_double]
+        lib.or_end_fermion_force.argtypes = [ctypes.c_double]
+        lib.or_hmc_action.argtypes = [ctypes.c_double, ctypes.c_double, _D]
+        lib.or_hmc_action.restype = ctypes.c_double
+        lib.or_spin_matrix.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _D]
 
     # ---- lattice -------------------------------------------------------
     def init(self, dims, bc=BC_ANTIPERIODIC, csw=1.0, cf=1.0):
@@ -254,3 +262,44 @@ class Oracle:
 
     def set_threads(self, n):
         self.lib.or_set_threads(n)
+
+    # ---- forces (forces.h) ---------------------------------------------
+    def get_momenta(self):
+        out = np.zeros((4 * self.V, self.nc ** 2 - 1))
+        self.lib.or_get_momenta(_p(out))
+        return out
+
+    def set_momenta(self, m):
+        m = np.ascontiguousarray(m, dtype=np.float64)
+        self.lib.or_set_momenta(_p(m))
+
+    def get_clover_force(self):
+        out = np.zeros((6 * self.V, 2 * self.dr ** 2))
+        self.lib.or_get_clover_force(_p(out))
+        return out
+
+    def begin_fermion_force(self):
+        self.lib.or_begin_fermion_force()
+
+    def end_fermion_force(self, dt):
+        self.lib.or_end_fermion_force(dt)
+
+    def clover_fermion_force(self, residue, lhs, rhs):
+        self.lib.or_clover_fermion_force(residue, _p(lhs), _p(rhs))
+
+    def wilson_fermion_force(self, dt, mass, residue, lhs, rhs):
+        self.lib.or_wilson_fermion_force(dt, mass, residue, _p(lhs), _p(rhs))
+
+    def wilson_fermion_force_q(self, dt, mass, residue, rhs):
+        self.lib.or_wilson_fermion_force_q(dt, mass, residue, _p(rhs))
+
+    def clover_logdet_force(self, mass, coeff):
+        self.lib.or_clover_logdet_force(mass, coeff)
+
+    def hmc_action(self, mass, prec, pf):
+        return self.lib.or_hmc_action(mass, prec, _p(np.ascontiguousarray(pf)))
+
+    def spin_matrix(self, kind, mu, nu=0):
+        out = np.zeros(32)
+        self.lib.or_spin_matrix(kind, mu, nu, _p(out))
+        return (out[0::2] + 1j * out[1::2]).reshape(4, 4)

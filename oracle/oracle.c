@@ -329,6 +329,8 @@ void or_get_generators(double *fund_out, double *repr_out)
 	}
 }
 
+static void gamma_init_serial(void);
+
 /* ------------------------------------------------------------------ */
 /* init / free                                                          */
 /* ------------------------------------------------------------------ */
@@ -389,6 +391,7 @@ int or_init(int T, int X, int Y, int Z, int bc, double csw, double cf)
 	cl_sigma = 0xF00F;
 	mvm_counter = 0;
 	repr_init();
+	gamma_init_serial();
 	return 0;
 }
 
@@ -1085,3 +1088,315 @@ int or_auto_inverter(int mvm, double mass, double prec, double *dptr, double *sp
 }
 
 void or_zero_auto_scratch(void) { memset(tmp_a, 0, sizeof(double) * (size_t)V * NSP); }
+
+/* ------------------------------------------------------------------ */
+/* forces: library/forces.cpp, library/clover.cpp:170-279,              */
+/* library/suNa.cpp:91-111 (algebra_project)                             */
+/* Spin matrices: gamma5 (1 - gamma_mu) (spinor.cpp:440-500) and         */
+/* gamma5 (1/2)[gamma_mu, gamma_nu] (spinor.cpp:266-438), built from the */
+/* chiral basis of spinor.cpp:191-264.                                   */
+/* ------------------------------------------------------------------ */
+static cplx GM[4][4][4];
+static int gm_init = 0;
+
+/* filled once, serially, by or_init (the force loops only read it) */
+static void gamma_init_serial(void)
+{
+	memset(GM, 0, sizeof(GM));
+	GM[0][0][2] = GM[0][1][3] = GM[0][2][0] = GM[0][3][1] = -1;
+	GM[1][0][3] = GM[1][1][2] = -I;
+	GM[1][2][1] = GM[1][3][0] = I;
+	GM[2][0][3] = -1; GM[2][1][2] = 1; GM[2][2][1] = 1; GM[2][3][0] = -1;
+	GM[3][0][2] = -I; GM[3][1][3] = I; GM[3][2][0] = I; GM[3][3][1] = -I;
+	gm_init = 1;
+}
+
+static void gamma_init(void) { (void)gm_init; }
+
+static const double G5D[4] = {1, 1, -1, -1};
+
+/* out = M psi on the spin index, M a 4x4 spin matrix */
+static void spin_apply(cplx M[4][4], const double *in, cplx out[DR][4])
+{
+	for (int c = 0; c < DR; c++)
+		for (int a = 0; a < 4; a++) {
+			cplx s = 0;
+			for (int b = 0; b < 4; b++) s += M[a][b] * sget(in, c, b);
+			out[c][a] = s;
+		}
+}
+
+static void g5_one_minus_gamma_mat(int mu, cplx M[4][4])
+{
+	gamma_init();
+	for (int a = 0; a < 4; a++)
+		for (int b = 0; b < 4; b++) M[a][b] = G5D[a] * ((a == b ? 1.0 : 0.0) - GM[mu][a][b]);
+}
+
+static void g5_sigma_mat(int mu, int nu, cplx M[4][4])
+{
+	gamma_init();
+	for (int a = 0; a < 4; a++)
+		for (int b = 0; b < 4; b++) {
+			cplx s = 0;
+			for (int c = 0; c < 4; c++) s += GM[mu][a][c] * GM[nu][c][b] - GM[nu][a][c] * GM[mu][c][b];
+			M[a][b] = G5D[a] * 0.5 * s;
+		}
+}
+
+void or_spin_matrix(int kind, int mu, int nu, double *out)
+{
+	cplx M[4][4];
+	if (kind == 0) g5_one_minus_gamma_mat(mu, M);
+	else g5_sigma_mat(mu, nu, M);
+	for (int a = 0; a < 16; a++) { out[2 * a] = creal(M[a / 4][a % 4]); out[2 * a + 1] = cimag(M[a / 4][a % 4]); }
+}
+
+/* fmat = a_lhs # a_rhs + b_lhs # b_rhs, (x # y)_ij = sum_spin x_i conj(y_j)
+ * (forces.cpp:11-30); the left operands are already spin-transformed */
+static void fmat_create(cplx al[DR][4], const double *ar, cplx bl[DR][4], const double *br, cplx F[DR][DR])
+{
+	for (int i = 0; i < DR; i++)
+		for (int j = 0; j < DR; j++) {
+			cplx s = 0;
+			for (int k = 0; k < 4; k++) s += al[i][k] * conj(sget(ar, j, k)) + bl[i][k] * conj(sget(br, j, k));
+			F[i][j] = s;
+		}
+}
+
+/* algebra_project(suNf, coeff): av_a = -(coeff/TF) Re Tr(iT^R_a M)
+ * (suNa.cpp:102-111, suNf.cpp:160-171) */
+static void algebra_project_f(cplx M[DR][DR], double coeff, double *av)
+{
+	for (int a = 0; a < NG; a++) {
+		double re = 0;
+		for (int i = 0; i < DR; i++)
+			for (int k = 0; k < DR; k++) re += creal(iTrepr[a][i][k] * M[k][i]);
+		av[a] += -(coeff / TF) * re;
+	}
+}
+
+static void load_repr(int id, int mu, cplx U[DR][DR])
+{
+	const double *u = grepr + (4 * (size_t)id + mu) * NLF;
+	for (int i = 0; i < DR; i++)
+		for (int j = 0; j < DR; j++) U[i][j] = fget(u, i, j);
+}
+
+static void mmf(cplx A[DR][DR], cplx B[DR][DR], cplx C[DR][DR])
+{
+	cplx T[DR][DR];
+	for (int i = 0; i < DR; i++)
+		for (int j = 0; j < DR; j++) {
+			cplx s = 0;
+			for (int k = 0; k < DR; k++) s += A[i][k] * B[k][j];
+			T[i][j] = s;
+		}
+	memcpy(C, T, sizeof(T));
+}
+
+/* begin_fermion_force: forces.cpp:239-251 */
+void or_begin_fermion_force(void) { memset(clforce, 0, sizeof(double) * 6 * (size_t)V * NLF); }
+
+/* clover_fermion_force: forces.cpp:122-172 (planes 01, 02, 12, 03, 13, 23) */
+void or_clover_fermion_force(double residue, double *lhs, double *rhs)
+{
+	static const int planes[6][2] = {{0, 1}, {0, 2}, {1, 2}, {0, 3}, {1, 3}, {2, 3}};
+	if (g_csw == 0.0) return;
+#pragma omp parallel for
+	for (int id = 0; id < V; id++) {
+		const double *l = lhs + (size_t)id * NSP, *r = rhs + (size_t)id * NSP;
+		for (int k = 0; k < 6; k++) {
+			cplx M[4][4], tl[DR][4], tr[DR][4], F[DR][DR];
+			g5_sigma_mat(planes[k][0], planes[k][1], M);
+			spin_apply(M, r, tr);
+			spin_apply(M, l, tl);
+			fmat_create(tl, r, tr, l, F);
+			double *cf = clforce + (6 * (size_t)id + k) * NLF;
+			for (int i = 0; i < DR; i++)
+				for (int j = 0; j < DR; j++) fset(cf, i, j, fget(cf, i, j) + residue * F[i][j]);
+		}
+	}
+}
+
+/* wilson_fermion_force: forces.cpp:175-227 (EO + clover branch) */
+void or_wilson_fermion_force(double dt, double mass, double residue, double *lhs, double *rhs)
+{
+	double coeff = dt * residue;
+	dirac_kernel(rhs, rhs, PART_ODD, 1, 0);
+	dirac_kernel(lhs, lhs, PART_ODD, 1, 0);
+	if (g_csw != 0.0) {
+		clover_inv_kernel(mass, rhs, rhs, PART_ODD, 0);
+		clover_inv_kernel(mass, lhs, lhs, PART_ODD, 0);
+	}
+	coeff = -coeff;
+	if (g_csw != 0.0) or_clover_fermion_force(residue, lhs, rhs);
+#pragma omp parallel for
+	for (int id = 0; id < V; id++)
+		for (int mu = 0; mu < 4; mu++) {
+			int fw = nb_fw[4 * id + mu];
+			cplx M[4][4], tl[DR][4], tr[DR][4], F[DR][DR], U[DR][DR];
+			g5_one_minus_gamma_mat(mu, M);
+			spin_apply(M, rhs + (size_t)fw * NSP, tr);
+			spin_apply(M, lhs + (size_t)fw * NSP, tl);
+			fmat_create(tl, rhs + (size_t)id * NSP, tr, lhs + (size_t)id * NSP, F);
+			load_repr(id, mu, U);
+			mmf(U, F, F);
+			algebra_project_f(F, coeff, mom + (4 * (size_t)id + mu) * NG);
+		}
+}
+
+/* wilson_fermion_force_q: forces.cpp:230-236 */
+void or_wilson_fermion_force_q(double dt, double mass, double residue, double *rhs)
+{
+	double *tmp = calloc((size_t)V * NSP, sizeof(double));
+	or_Hphi(mass, tmp, rhs);
+	or_wilson_fermion_force(dt, mass, residue, tmp, rhs);
+	free(tmp);
+}
+
+/* compute_clover_force / clover_logdet_force: clover.cpp:170-279,
+ * forces.cpp:109-119.  coeff * (D_oo block)^-1 from the LDL factors,
+ * then the six sigma-plane combinations into cl_force */
+void or_clover_logdet_force(double mass, double coeff)
+{
+	if (g_csw == 0.0) return;
+	or_compute_ldl_decomp(4.0 + mass);
+#pragma omp parallel for
+	for (int id = VH; id < V; id++) {
+		cplx A[NB][NB], B[NB][NB];
+		const cplx *lu = ldl_up + (size_t)id * NLDL, *ll = ldl_lo + (size_t)id * NLDL;
+		memset(A, 0, sizeof(A));
+		memset(B, 0, sizeof(B));
+		for (int n = 0; n < NB; n++) {
+			A[n][n] = coeff;
+			B[n][n] = coeff;
+			for (int i = n; i < NB; i++)
+				for (int k = 0; k < i; k++) {
+					A[i][n] -= lu[i * (i + 1) / 2 + k] * A[k][n];
+					B[i][n] -= ll[i * (i + 1) / 2 + k] * B[k][n];
+				}
+			for (int i = NB - 1; i >= n; i--) {
+				A[i][n] /= creal(lu[i * (i + 1) / 2 + i]);
+				B[i][n] /= creal(ll[i * (i + 1) / 2 + i]);
+				for (int k = i + 1; k < NB; k++) {
+					A[i][n] -= conj(lu[k * (k + 1) / 2 + i]) * A[k][n];
+					B[i][n] -= conj(ll[k * (k + 1) / 2 + i]) * B[k][n];
+				}
+			}
+		}
+		double *cf = clforce + 6 * (size_t)id * NLF;
+		for (int i = 0; i < DR; i++)
+			for (int j = 0; j < DR; j++) {
+				cplx a21 = A[i + DR][j], a12 = conj(A[j + DR][i]);
+				cplx a43 = B[i + DR][j], a34 = conj(B[j + DR][i]);
+				cplx a11, a22, a33, a44;
+				if (i < j) {
+					a11 = conj(A[j][i]); a22 = conj(A[j + DR][i + DR]);
+					a33 = conj(B[j][i]); a44 = conj(B[j + DR][i + DR]);
+				} else {
+					a11 = A[i][j]; a22 = A[i + DR][j + DR];
+					a33 = B[i][j]; a44 = B[i + DR][j + DR];
+				}
+				/* clover.cpp:249-260 */
+				cplx x01 = (cimag(a12) + cimag(a21) - cimag(a34) - cimag(a43)) + I * (-creal(a12) - creal(a21) + creal(a34) + creal(a43));
+				cplx x02 = (creal(a12) - creal(a21) + creal(a43) - creal(a34)) + I * (cimag(a12) - cimag(a21) + cimag(a43) - cimag(a34));
+				cplx x12 = (cimag(a22) - cimag(a11) + cimag(a44) - cimag(a33)) + I * (creal(a11) - creal(a22) + creal(a33) - creal(a44));
+				cplx x03 = (cimag(a11) - cimag(a22) + cimag(a44) - cimag(a33)) + I * (creal(a22) - creal(a11) + creal(a33) - creal(a44));
+				cplx x13 = (creal(a12) - creal(a21) + creal(a34) - creal(a43)) + I * (cimag(a12) - cimag(a21) + cimag(a34) - cimag(a43));
+				cplx x23 = (-cimag(a12) - cimag(a21) - cimag(a34) - cimag(a43)) + I * (creal(a12) + creal(a21) + creal(a34) + creal(a43));
+				cplx xs[6] = {x01, x02, x12, x03, x13, x23};
+				for (int k = 0; k < 6; k++) fset(cf + k * NLF, i, j, fget(cf + k * NLF, i, j) + xs[k]);
+			}
+	}
+}
+
+static void load_clf(int id, int k, cplx Z[DR][DR])
+{
+	const double *p = clforce + (6 * (size_t)id + k) * NLF;
+	for (int i = 0; i < DR; i++)
+		for (int j = 0; j < DR; j++) Z[i][j] = fget(p, i, j);
+}
+
+static void dagf(cplx A[DR][DR], cplx B[DR][DR])
+{
+	cplx T[DR][DR];
+	for (int i = 0; i < DR; i++)
+		for (int j = 0; j < DR; j++) T[i][j] = conj(A[j][i]);
+	memcpy(B, T, sizeof(T));
+}
+
+static void addf(cplx A[DR][DR], cplx B[DR][DR], double s)
+{
+	for (int i = 0; i < DR; i++)
+		for (int j = 0; j < DR; j++) A[i][j] += s * B[i][j];
+}
+
+/* clover_force_core via end_fermion_force: forces.cpp:33-106, 254-259 */
+void or_end_fermion_force(double dt)
+{
+	if (g_csw == 0.0) return;
+	double coeff = (g_csw * dt) / 8.0;
+#pragma omp parallel for
+	for (int id = 0; id < V; id++)
+		for (int mu = 0; mu < 4; mu++)
+			for (int nu = 0; nu < 4; nu++) {
+				if (mu == nu) continue;
+				int o1 = nb_fw[4 * id + mu], o2 = nb_fw[4 * id + nu], o3 = nb_bk[4 * id + nu];
+				int o4 = nb_fw[4 * o3 + mu], o5 = nb_fw[4 * o2 + mu];
+				int num, sign;
+				if (mu < nu) { num = nu * (nu - 1) / 2 + mu; sign = 1; }
+				else { num = mu * (mu - 1) / 2 + nu; sign = -1; }
+				cplx Z0[DR][DR], Z1[DR][DR], Z2[DR][DR], Z3[DR][DR], Z4[DR][DR], Z5[DR][DR];
+				load_clf(id, num, Z0); load_clf(o1, num, Z1); load_clf(o3, num, Z2);
+				load_clf(o4, num, Z3); load_clf(o5, num, Z4); load_clf(o2, num, Z5);
+				cplx W[9][DR][DR], T1[DR][DR], T2[DR][DR], F[DR][DR];
+				load_repr(o3, mu, W[0]); dagf(W[0], W[0]);
+				load_repr(o3, nu, W[1]);
+				load_repr(o1, nu, W[2]);
+				load_repr(o2, mu, W[3]); dagf(W[3], W[3]);
+				load_repr(id, nu, W[4]); dagf(W[4], W[4]);
+				load_repr(o4, nu, W[5]); dagf(W[5], W[5]);
+				mmf(W[0], W[1], W[6]);
+				mmf(W[2], W[3], W[7]);
+				mmf(W[7], W[4], T1);
+				mmf(W[5], W[6], T2);
+				memcpy(W[8], T1, sizeof(T1));
+				addf(W[8], T2, -1.0);
+				/* fmat = W8 Z0 + Z1 W8 */
+				mmf(W[8], Z0, F);
+				mmf(Z1, W[8], T1);
+				addf(F, T1, 1.0);
+				/* fmat -= W5 (W0 Z2 W1 + Z3 W6) */
+				mmf(W[0], Z2, T1); mmf(T1, W[1], T1);
+				mmf(Z3, W[6], T2);
+				addf(T1, T2, 1.0);
+				mmf(W[5], T1, T2);
+				addf(F, T2, -1.0);
+				/* fmat += (W2 Z4 W3 + W7 Z5) W4 */
+				mmf(W[2], Z4, T1); mmf(T1, W[3], T1);
+				mmf(W[7], Z5, T2);
+				addf(T1, T2, 1.0);
+				mmf(T1, W[4], T2);
+				addf(F, T2, 1.0);
+				load_repr(id, mu, T1);
+				mmf(T1, F, F);
+				algebra_project_f(F, sign * coeff, mom + (4 * (size_t)id + mu) * NG);
+			}
+}
+
+/* S_f = |Dphi^-1 pf|^2 - 2 log det D_oo (monomials/hmc.cpp:40-65) with the
+ * inverse taken by CG on H^2: Dphi^-1 pf = H^-2 Dphi^+ pf... here computed
+ * as x = (H^2)^-1 (gamma5 Dphi gamma5 ...) is avoided: the action is
+ * |y|^2 with Dphi y = pf solved by BiCGstab, as the reference does. */
+double or_hmc_action(double mass, double prec, const double *pf)
+{
+	size_t n = (size_t)V * NSP;
+	double *y = calloc(n, sizeof(double)), *b = malloc(n * sizeof(double));
+	memcpy(b, pf, n * sizeof(double));
+	or_bicgstab(MVM_DPHI, mass, prec, y, b, NULL);
+	double v = or_sqnorm(y);
+	if (g_csw != 0.0) v -= 2 * or_clover_logdet(mass);
+	free(y); free(b);
+	return v;
+}
