@@ -37,6 +37,10 @@ BYTES_PER_SITE = {
     ("f16", 1): 52 + 8 * 24 + 52,      # 296 (48 B half spinor + 4 B scale)
 }
 FLOP_PER_SITE = 1320
+# HBM-side bytes per launch of each hop variant, from two rocprofv3 --pmc
+# passes (FETCH_SIZE, WRITE_SIZE) scaled by the same counters on the copy
+# kernel; see profiles/hop_f32r12_32x4_traffic.md.  None = not measured.
+TRAFFIC_PER_LAUNCH = {"f32_r12": 406281886}
 
 
 def log(msg):
@@ -172,7 +176,7 @@ def main():
                    "lattice": list(DIMS), "sites_per_step": lat.VH, "mass": MASS, "csw": CSW,
                    "gauge": "random SU(3) links (hot); the 10-trajectory pure-gauge smoothing is not applied"},
         "roofline": {"bound": "hbm", "achieved": head["GB_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(head["GB_s"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(head["GB_s"] / HBM_PEAK_GBS, 4), "traffic": TRAFFIC_PER_LAUNCH.get(key),
                      "algorithmic_bytes_per_launch": algo_bytes},
         "cpu_baseline": cpu,
         "dslash": hop,

@@ -162,6 +162,27 @@ int yh_cg_mshift(yh_ctx* ctx, const yh_inv_par* par, int num, const double* shif
  * updates = FP64 reliable updates */
 int yh_cg_mixed(yh_ctx* ctx, const yh_inv_par* par, yh_field* dst, const yh_field* src, int* iters, int* updates);
 
+/* --- forces (include/forces.h:6-15, library/forces.cpp) -------------------- */
+/* momenta afield_momenta: 4V suNa{av[NC*NC-1]} doubles, index 4*id+mu
+ * (include/gaugefield.h:12-18) */
+int yh_set_momenta(yh_ctx* ctx, const double* host);
+int yh_get_momenta(yh_ctx* ctx, double* host);
+/* cl_force: 6V suNf, index 6*id+k, planes 01,02,12,03,13,23
+ * (include/clover.h:15-16) */
+int yh_get_clover_force(yh_ctx* ctx, double* host);
+/* begin_fermion_force / end_fermion_force (forces.cpp:239-259) */
+int yh_begin_fermion_force(yh_ctx* ctx);
+int yh_end_fermion_force(yh_ctx* ctx, double dt);
+/* clover_fermion_force (forces.cpp:122-172) */
+int yh_clover_fermion_force(yh_ctx* ctx, double residue, const yh_field* lhs, const yh_field* rhs);
+/* wilson_fermion_force (forces.cpp:175-227): overwrites the odd halves of
+ * lhs and rhs with D_oo^-1 D_oe X_e as the reference does */
+int yh_wilson_fermion_force(yh_ctx* ctx, double dt, double mass, double residue, yh_field* lhs, yh_field* rhs);
+/* wilson_fermion_force_q (forces.cpp:230-236) */
+int yh_wilson_fermion_force_q(yh_ctx* ctx, double dt, double mass, double residue, yh_field* rhs);
+/* clover_logdet_force (forces.cpp:109-119) */
+int yh_clover_logdet_force(yh_ctx* ctx, double mass, double residue);
+
 /* --- measurement support ---------------------------------------------------- */
 /* time `reps` back-to-back hop launches (Dphi_eo shape) of the given
  * precision / recon on the context stream with HIP events; returns the

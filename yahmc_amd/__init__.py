@@ -91,6 +91,15 @@ _SIGS = {
     "yh_bicgstab": ([_VP, ctypes.POINTER(InvPar), _VP, _VP, _PI], ctypes.c_int),
     "yh_cg_mshift": ([_VP, ctypes.POINTER(InvPar), ctypes.c_int, _D, ctypes.POINTER(_VP), _VP, _PI], ctypes.c_int),
     "yh_cg_mixed": ([_VP, ctypes.POINTER(InvPar), _VP, _VP, _PI, _PI], ctypes.c_int),
+    "yh_set_momenta": ([_VP, _D], ctypes.c_int),
+    "yh_get_momenta": ([_VP, _D], ctypes.c_int),
+    "yh_get_clover_force": ([_VP, _D], ctypes.c_int),
+    "yh_begin_fermion_force": ([_VP], ctypes.c_int),
+    "yh_end_fermion_force": ([_VP, ctypes.c_double], ctypes.c_int),
+    "yh_clover_fermion_force": ([_VP, ctypes.c_double, _VP, _VP], ctypes.c_int),
+    "yh_wilson_fermion_force": ([_VP, ctypes.c_double, ctypes.c_double, ctypes.c_double, _VP, _VP], ctypes.c_int),
+    "yh_wilson_fermion_force_q": ([_VP, ctypes.c_double, ctypes.c_double, ctypes.c_double, _VP], ctypes.c_int),
+    "yh_clover_logdet_force": ([_VP, ctypes.c_double, ctypes.c_double], ctypes.c_int),
     "yh_bench_hop": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP, _VP, _D], ctypes.c_int),
     "yh_bench_copy": ([_VP, ctypes.c_long, ctypes.c_int, _D], ctypes.c_int),
     "yh_synchronize": ([_VP], ctypes.c_int),
@@ -317,6 +326,44 @@ class Lattice:
         up = ctypes.c_int(0)
         _check(lib().yh_cg_mixed(self.ctx, ctypes.byref(par), d.h, s.h, ctypes.byref(it), ctypes.byref(up)))
         return it.value, up.value
+
+    # ---- forces (include/forces.h) ------------------------------------
+    @property
+    def ng(self):
+        return self.nc ** 2 - 1
+
+    def set_momenta(self, m):
+        m = np.ascontiguousarray(m, dtype=np.float64)
+        assert m.size == 4 * self.V * self.ng
+        _check(lib().yh_set_momenta(self.ctx, _dp(m)))
+
+    def get_momenta(self):
+        out = np.zeros((4 * self.V, self.ng))
+        _check(lib().yh_get_momenta(self.ctx, _dp(out)))
+        return out
+
+    def get_clover_force(self):
+        out = np.zeros((6 * self.V, 2 * self.dr ** 2))
+        _check(lib().yh_get_clover_force(self.ctx, _dp(out)))
+        return out
+
+    def begin_fermion_force(self):
+        _check(lib().yh_begin_fermion_force(self.ctx))
+
+    def end_fermion_force(self, dt):
+        _check(lib().yh_end_fermion_force(self.ctx, dt))
+
+    def clover_fermion_force(self, residue, lhs, rhs):
+        _check(lib().yh_clover_fermion_force(self.ctx, residue, lhs.h, rhs.h))
+
+    def wilson_fermion_force(self, dt, mass, residue, lhs, rhs):
+        _check(lib().yh_wilson_fermion_force(self.ctx, dt, mass, residue, lhs.h, rhs.h))
+
+    def wilson_fermion_force_q(self, dt, mass, residue, rhs):
+        _check(lib().yh_wilson_fermion_force_q(self.ctx, dt, mass, residue, rhs.h))
+
+    def clover_logdet_force(self, mass, residue):
+        _check(lib().yh_clover_logdet_force(self.ctx, mass, residue))
 
     # ---- measurement ---------------------------------------------------
     def bench_hop(self, prec, recon, reps, d, s):

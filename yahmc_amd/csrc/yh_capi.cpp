@@ -362,6 +362,7 @@ int yh_create(const yh_lattice* lat, yh_ctx** out) {
     return YH_ERR_HIP;
   }
   int rc = gauge_init_basis(c);
+  if (!rc) rc = force_init_constants(c);
   if (rc) { yh_destroy(c); return rc; }
   *out = c;
   return YH_OK;
@@ -381,6 +382,8 @@ int yh_destroy(yh_ctx* c) {
   if (c->tmp[5]) { (void)hipFree(c->tmp[5]->mem); delete static_cast<yh_field*>(c->tmp[5]); }
   if (c->fund) (void)hipFree(c->fund);
   if (c->logdet_site) (void)hipFree(c->logdet_site);
+  if (c->mom) (void)hipFree(c->mom);
+  if (c->clf) (void)hipFree(c->clf);
   if (c->partials) (void)hipFree(c->partials);
   if (c->ticket) (void)hipFree(c->ticket);
   if (c->sc) (void)hipFree(c->sc);

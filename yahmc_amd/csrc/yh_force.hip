@@ -1,0 +1,641 @@
+// Fermion forces (FP64): hopping force, clover force, clover log-det force
+// and the clover-leaf derivative that turns the clover force matrices into
+// momentum updates.
+//
+// wilson_fermion_force : library/forces.cpp:175-227
+// clover_fermion_force : library/forces.cpp:122-172
+// clover_logdet_force  : library/forces.cpp:109-119 + clover.cpp:170-279
+// end_fermion_force    : library/forces.cpp:33-106, 254-259
+// algebra_project      : library/suNa.cpp:102-111 (TF = 2)
+//
+// Device layouts: momenta mom[(mu * NG + a) * V + id] (FP64, id = full EO
+// index), clover force matrices clf[(plane * D * D + e) * V + id] (double2),
+// planes in the reference order 01, 02, 12, 03, 13, 23.
+#include <vector>
+#include "yh_internal.hpp"
+
+namespace yh {
+
+__constant__ double2 c_itr[15 * 36];     // iT^R_a, a < NG, D x D row-major
+__constant__ double2 c_g5omg[4 * 16];    // gamma5 (1 - gamma_mu)
+__constant__ double2 c_g5sig[6 * 16];    // gamma5 (1/2)[gamma_mu, gamma_nu], reference plane order
+
+typedef Cx<double> cd;
+
+// ---------------------------------------------------------------------------
+// host: generators of the fermion representation and the spin matrices
+// ---------------------------------------------------------------------------
+static void gen_fund(int nc, int a, std::vector<cd>& g) {
+  g.assign(nc * nc, cd(0, 0));
+  int k = 0;
+  for (int i = 0; i < nc; i++)
+    for (int j = 0; j < nc; j++) {
+      if (i > j) {
+        if (k == a) { g[i * nc + j].im += 1; g[j * nc + i].im += 1; }
+        k++;
+      } else if (i < j) {
+        if (k == a) { g[i * nc + j].re += 1; g[j * nc + i].re -= 1; }
+        k++;
+      } else if (i > 0) {
+        if (k == a) {
+          double v = sqrt(2.0 / (i * (i + 1)));
+          for (int q = 0; q < i; q++) g[q * nc + q].im += v;
+          g[i * nc + i].im -= i * v;
+        }
+        k++;
+      }
+    }
+}
+
+// basis of the representation space (matches the device basis used by the
+// representation kernel): sym/asym real tensors, adjoint (-i/sqrt2) iT_a
+static void rep_basis(int nc, int repr, int dr, std::vector<std::vector<cd>>& b) {
+  const double s2 = 0.70710678118654752440;
+  b.assign(dr, std::vector<cd>(nc * nc, cd(0, 0)));
+  if (repr == YH_REPR_SYMMETRIC) {
+    for (int i = 0, n = 0; i < nc; i++)
+      for (int j = 0; j <= i; j++, n++) {
+        if (i == j) b[n][i * nc + i].re = 1;
+        else { b[n][i * nc + j].re = s2; b[n][j * nc + i].re = s2; }
+      }
+  } else if (repr == YH_REPR_ANTISYMMETRIC) {
+    for (int i = 0, n = 0; i < nc; i++)
+      for (int j = 0; j < i; j++, n++) {
+        b[n][j * nc + i].re = s2;
+        b[n][i * nc + j].re = -s2;
+      }
+  } else if (repr == YH_REPR_ADJOINT) {
+    std::vector<cd> g;
+    for (int n = 0; n < dr; n++) {
+      gen_fund(nc, n, g);
+      for (int k = 0; k < nc * nc; k++) b[n][k] = cd(s2 * g[k].im, -s2 * g[k].re);
+    }
+  }
+}
+
+// iT^R_a = d/dt R(exp(t iT_a)) at t = 0 (library/repr.cpp:71-103):
+//   fund: iT_a;  sym/asym: sum conj(b_n)_{xy} (G b_m + b_m G^T)_{xy};
+//   adjoint: Tr(b_n [G, b_m])
+static void rep_generators(int nc, int repr, int dr, std::vector<cd>& out) {
+  int ng = nc * nc - 1;
+  out.assign(ng * dr * dr, cd(0, 0));
+  std::vector<std::vector<cd>> b;
+  rep_basis(nc, repr, dr, b);
+  std::vector<cd> G;
+  for (int a = 0; a < ng; a++) {
+    gen_fund(nc, a, G);
+    for (int n = 0; n < dr; n++)
+      for (int m = 0; m < dr; m++) {
+        cd s(0, 0);
+        if (repr == YH_REPR_FUNDAMENTAL) {
+          s = G[n * nc + m];
+        } else if (repr == YH_REPR_ADJOINT) {
+          for (int x = 0; x < nc; x++)
+            for (int y = 0; y < nc; y++)
+              for (int z = 0; z < nc; z++)
+                s = s + b[n][x * nc + y] * (G[y * nc + z] * b[m][z * nc + x] - b[m][y * nc + z] * G[z * nc + x]);
+        } else {
+          for (int x = 0; x < nc; x++)
+            for (int y = 0; y < nc; y++) {
+              cd v(0, 0);
+              for (int z = 0; z < nc; z++) v = v + G[x * nc + z] * b[m][z * nc + y] + b[m][x * nc + z] * G[y * nc + z];
+              s = s + conj(b[n][x * nc + y]) * v;
+            }
+        }
+        out[(a * dr + n) * dr + m] = s;
+      }
+  }
+}
+
+// chiral gamma basis as applied by library/spinor.cpp:191-264
+static void gamma_mats(cd g[4][4][4]) {
+  for (int m = 0; m < 4; m++)
+    for (int a = 0; a < 4; a++)
+      for (int b = 0; b < 4; b++) g[m][a][b] = cd(0, 0);
+  g[0][0][2] = g[0][1][3] = g[0][2][0] = g[0][3][1] = cd(-1, 0);
+  g[1][0][3] = g[1][1][2] = cd(0, -1);
+  g[1][2][1] = g[1][3][0] = cd(0, 1);
+  g[2][0][3] = cd(-1, 0); g[2][1][2] = cd(1, 0); g[2][2][1] = cd(1, 0); g[2][3][0] = cd(-1, 0);
+  g[3][0][2] = cd(0, -1); g[3][1][3] = cd(0, 1); g[3][2][0] = cd(0, 1); g[3][3][1] = cd(0, -1);
+}
+
+int force_init_constants(Ctx* c) {
+  std::vector<cd> itr;
+  rep_generators(c->nc, c->repr, c->dr, itr);
+  std::vector<double2> h(15 * 36, make_double2(0, 0));
+  for (size_t i = 0; i < itr.size(); i++) h[i] = make_double2(itr[i].re, itr[i].im);
+  YH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_itr), h.data(), sizeof(double2) * h.size()));
+  cd g[4][4][4];
+  gamma_mats(g);
+  const double g5[4] = {1, 1, -1, -1};
+  std::vector<double2> om(4 * 16), sg(6 * 16);
+  for (int m = 0; m < 4; m++)
+    for (int a = 0; a < 4; a++)
+      for (int b = 0; b < 4; b++) {
+        cd v = cd(a == b ? 1.0 : 0.0, 0) - g[m][a][b];
+        om[m * 16 + a * 4 + b] = make_double2(g5[a] * v.re, g5[a] * v.im);
+      }
+  const int planes[6][2] = {{0, 1}, {0, 2}, {1, 2}, {0, 3}, {1, 3}, {2, 3}};
+  for (int k = 0; k < 6; k++) {
+    int mu = planes[k][0], nu = planes[k][1];
+    for (int a = 0; a < 4; a++)
+      for (int b = 0; b < 4; b++) {
+        cd s(0, 0);
+        for (int q = 0; q < 4; q++) s = s + g[mu][a][q] * g[nu][q][b] - g[nu][a][q] * g[mu][q][b];
+        sg[k * 16 + a * 4 + b] = make_double2(g5[a] * 0.5 * s.re, g5[a] * 0.5 * s.im);
+      }
+  }
+  YH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_g5omg), om.data(), sizeof(double2) * om.size()));
+  YH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_g5sig), sg.data(), sizeof(double2) * sg.size()));
+  return YH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------
+struct FF {
+  const void* x[2];   // lhs halves (FP64)
+  const void* y[2];   // rhs halves (FP64)
+};
+
+__device__ __forceinline__ int fidx(int t, int x, int y, int z, const Geom& g) {
+  return ((t + x + y + z) & 1) * g.VH + half_index(t, x, y, z, g);
+}
+
+__device__ __forceinline__ void fcoords(int id, const Geom& g, int c[4]) {
+  int p = id >= g.VH;
+  site_coords(id - p * g.VH, p, g, c);
+}
+
+__device__ __forceinline__ int fshift(const int c[4], int mu, int d, const Geom& g) {
+  int n[4] = {c[0], c[1], c[2], c[3]};
+  n[mu] = (n[mu] + d + g.L[mu]) % g.L[mu];
+  return fidx(n[0], n[1], n[2], n[3], g);
+}
+
+template <int D>
+__device__ __forceinline__ void load_full_spinor(cd (&s)[4][D], const void* const (&h)[2], int id, const Geom& g) {
+  int p = id >= g.VH;
+  load_spinor<PREC_F64, D, double>(s, h[p], nullptr, id - p * g.VH, g.VHpad);
+}
+
+// represented link U_R(id, mu) with the antiperiodic sign of the reference's
+// gfield_repr (library/bcs.cpp:264-272)
+template <int D>
+__device__ __forceinline__ void load_link_bc(cd (&u)[D][D], const double2* links, int id, int mu, const Geom& g) {
+  int p = id >= g.VH, k = id - p * g.VH;
+  const double2* base = links + (size_t)(p * 4 + mu) * D * D * g.VHpad;
+#pragma unroll
+  for (int e = 0; e < D * D; e++) {
+    double2 v = base[(size_t)e * g.VHpad + k];
+    u[e / D][e % D] = cd(v.x, v.y);
+  }
+  if (g.bc == BC_ANTIPERIODIC && mu == 0) {
+    int c[4];
+    site_coords(k, p, g, c);
+    if (c[0] == 0) {
+#pragma unroll
+      for (int i = 0; i < D; i++)
+#pragma unroll
+        for (int j = 0; j < D; j++) u[i][j] = cd(-u[i][j].re, -u[i][j].im);
+    }
+  }
+}
+
+template <int D>
+__device__ __forceinline__ void spin_mat(cd (&o)[4][D], const double2* M, const cd (&s)[4][D]) {
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int c = 0; c < D; c++) {
+      cd v(0, 0);
+#pragma unroll
+      for (int b = 0; b < 4; b++) {
+        double2 m = M[a * 4 + b];
+        if (m.x != 0.0 || m.y != 0.0) cmac(v, cd(m.x, m.y), s[b][c]);
+      }
+      o[a][c] = v;
+    }
+}
+
+// F_ij += sum_spin a_i conj(b_j)
+template <int D>
+__device__ __forceinline__ void outer_acc(cd (&F)[D][D], const cd (&a)[4][D], const cd (&b)[4][D]) {
+#pragma unroll
+  for (int i = 0; i < D; i++)
+#pragma unroll
+    for (int j = 0; j < D; j++)
+#pragma unroll
+      for (int s = 0; s < 4; s++) {
+        F[i][j].re += a[s][i].re * b[s][j].re + a[s][i].im * b[s][j].im;
+        F[i][j].im += a[s][i].im * b[s][j].re - a[s][i].re * b[s][j].im;
+      }
+}
+
+template <int D>
+__device__ __forceinline__ void mmul(cd (&R)[D][D], const cd (&A)[D][D], const cd (&B)[D][D]) {
+#pragma unroll
+  for (int i = 0; i < D; i++)
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+      cd s(0, 0);
+#pragma unroll
+      for (int k = 0; k < D; k++) cmac(s, A[i][k], B[k][j]);
+      R[i][j] = s;
+    }
+}
+
+// mom_a += -(coeff / TF) Re Tr(iT^R_a F)
+template <int NG, int D>
+__device__ __forceinline__ void project_add(double* mom, int mu, int id, int V, const cd (&F)[D][D], double coeff) {
+#pragma unroll
+  for (int a = 0; a < NG; a++) {
+    double re = 0.0;
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+      for (int k = 0; k < D; k++) {
+        double2 t = c_itr[(a * D + i) * D + k];
+        re += t.x * F[k][i].re - t.y * F[k][i].im;
+      }
+    mom[(size_t)(mu * NG + a) * V + id] += -(coeff * 0.5) * re;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+// hopping force, one thread per (site, mu)  (forces.cpp:204-223)
+template <int NG, int D>
+__global__ void __launch_bounds__(256) k_force_hop(FF f, const double2* __restrict__ links, double* __restrict__ mom,
+                                                   double coeff, Geom g) {
+  int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= 4 * g.V) return;
+  int id = t >> 2, mu = t & 3;
+  int c[4];
+  fcoords(id, g, c);
+  int fw = fshift(c, mu, 1, g);
+  cd F[D][D];
+#pragma unroll
+  for (int i = 0; i < D; i++)
+#pragma unroll
+    for (int j = 0; j < D; j++) F[i][j] = cd(0, 0);
+  {
+    cd s[4][D], ps[4][D], o[4][D];
+    load_full_spinor<D>(s, f.x, fw, g);          // lhs(x + mu)
+    spin_mat<D>(ps, c_g5omg + mu * 16, s);
+    load_full_spinor<D>(o, f.y, id, g);          // rhs(x)
+    outer_acc<D>(F, ps, o);
+    load_full_spinor<D>(s, f.y, fw, g);          // rhs(x + mu)
+    spin_mat<D>(ps, c_g5omg + mu * 16, s);
+    load_full_spinor<D>(o, f.x, id, g);          // lhs(x)
+    outer_acc<D>(F, ps, o);
+  }
+  cd U[D][D], R[D][D];
+  load_link_bc<D>(U, links, id, mu, g);
+  mmul<D>(R, U, F);
+  project_add<NG, D>(mom, mu, id, g.V, R, coeff);
+}
+
+// clover force matrices, one thread per site  (forces.cpp:130-168)
+template <int D>
+__global__ void __launch_bounds__(256) k_force_clover_site(FF f, double2* __restrict__ clf, double residue, Geom g) {
+  int id = blockIdx.x * 256 + threadIdx.x;
+  if (id >= g.V) return;
+  cd X[4][D], Y[4][D];
+  load_full_spinor<D>(X, f.x, id, g);
+  load_full_spinor<D>(Y, f.y, id, g);
+  for (int k = 0; k < 6; k++) {
+    cd sx[4][D], sy[4][D], F[D][D];
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+      for (int j = 0; j < D; j++) F[i][j] = cd(0, 0);
+    spin_mat<D>(sx, c_g5sig + k * 16, X);
+    spin_mat<D>(sy, c_g5sig + k * 16, Y);
+    outer_acc<D>(F, sx, Y);  // (g5 sigma lhs) # rhs
+    outer_acc<D>(F, sy, X);  // (g5 sigma rhs) # lhs
+#pragma unroll
+    for (int e = 0; e < D * D; e++) {
+      double2* p = clf + (size_t)(k * D * D + e) * g.V + id;
+      double2 v = *p;
+      *p = make_double2(v.x + residue * F[e / D][e % D].re, v.y + residue * F[e / D][e % D].im);
+    }
+  }
+}
+
+// log-det force: coeff * (sigma + clover)^-1 per odd site -> six plane
+// combinations (clover.cpp:184-262); the inverse is the stored explicit one
+template <int D>
+__global__ void __launch_bounds__(128) k_force_logdet(const double2* __restrict__ inv, double2* __restrict__ clf,
+                                                      double coeff, Geom g) {
+  int k = blockIdx.x * 128 + threadIdx.x;
+  if (k >= g.VH) return;
+  constexpr int N = 2 * D;
+  int id = g.VH + k;
+  // element (i, j) of block b from the packed Hermitian store
+  auto get = [&](int b, int i, int j) -> cd {
+    int base = b * N * N / 2;
+    double re, im;
+    if (i == j) {
+      double2 w = inv[(size_t)(base + i / 2) * g.VHpad + k];
+      return cd(coeff * ((i & 1) ? w.y : w.x), 0.0);
+    }
+    bool lower = i > j;
+    int a = lower ? i : j, c = lower ? j : i;
+    int o = N + 2 * (a * (a - 1) / 2 + c);  // real index of the lower entry (a, c)
+    double2 w0 = inv[(size_t)(base + o / 2) * g.VHpad + k];
+    double2 w1 = inv[(size_t)(base + (o + 1) / 2) * g.VHpad + k];
+    re = (o & 1) ? w0.y : w0.x;
+    im = ((o + 1) & 1) ? w1.y : w1.x;
+    return lower ? cd(coeff * re, coeff * im) : cd(coeff * re, -coeff * im);
+  };
+  for (int i = 0; i < D; i++)
+    for (int j = 0; j < D; j++) {
+      cd a21 = get(0, i + D, j), a12 = conj(get(0, j + D, i));
+      cd a43 = get(1, i + D, j), a34 = conj(get(1, j + D, i));
+      cd a11, a22, a33, a44;
+      if (i < j) {
+        a11 = conj(get(0, j, i)); a22 = conj(get(0, j + D, i + D));
+        a33 = conj(get(1, j, i)); a44 = conj(get(1, j + D, i + D));
+      } else {
+        a11 = get(0, i, j); a22 = get(0, i + D, j + D);
+        a33 = get(1, i, j); a44 = get(1, i + D, j + D);
+      }
+      cd xs[6];
+      xs[0] = cd(a12.im + a21.im - a34.im - a43.im, -a12.re - a21.re + a34.re + a43.re);  // X_01
+      xs[1] = cd(a12.re - a21.re + a43.re - a34.re, a12.im - a21.im + a43.im - a34.im);   // X_02
+      xs[2] = cd(a22.im - a11.im + a44.im - a33.im, a11.re - a22.re + a33.re - a44.re);   // X_12
+      xs[3] = cd(a11.im - a22.im + a44.im - a33.im, a22.re - a11.re + a33.re - a44.re);   // X_03
+      xs[4] = cd(a12.re - a21.re + a34.re - a43.re, a12.im - a21.im + a34.im - a43.im);   // X_13
+      xs[5] = cd(-a12.im - a21.im - a34.im - a43.im, a12.re + a21.re + a34.re + a43.re);  // X_23
+      for (int q = 0; q < 6; q++) {
+        double2* p = clf + (size_t)(q * D * D + i * D + j) * g.V + id;
+        double2 v = *p;
+        *p = make_double2(v.x + xs[q].re, v.y + xs[q].im);
+      }
+    }
+}
+
+template <int D>
+__device__ __forceinline__ void load_clf(cd (&Z)[D][D], const double2* clf, int num, int id, int V) {
+#pragma unroll
+  for (int e = 0; e < D * D; e++) {
+    double2 v = clf[(size_t)(num * D * D + e) * V + id];
+    Z[e / D][e % D] = cd(v.x, v.y);
+  }
+}
+
+template <int D>
+__device__ __forceinline__ void dag(cd (&A)[D][D]) {
+  cd T[D][D];
+#pragma unroll
+  for (int i = 0; i < D; i++)
+#pragma unroll
+    for (int j = 0; j < D; j++) T[i][j] = conj(A[j][i]);
+#pragma unroll
+  for (int i = 0; i < D; i++)
+#pragma unroll
+    for (int j = 0; j < D; j++) A[i][j] = T[i][j];
+}
+
+template <int D>
+__device__ __forceinline__ void madd(cd (&A)[D][D], const cd (&B)[D][D], double s) {
+#pragma unroll
+  for (int i = 0; i < D; i++)
+#pragma unroll
+    for (int j = 0; j < D; j++) A[i][j] = A[i][j] + s * B[i][j];
+}
+
+// clover-leaf derivative, one thread per (site, mu) looping over nu != mu
+// (forces.cpp:46-102)
+template <int NG, int D>
+__global__ void __launch_bounds__(128) k_force_clover_core(const double2* __restrict__ clf,
+                                                           const double2* __restrict__ links, double* __restrict__ mom,
+                                                           double coeff, Geom g) {
+  int t = blockIdx.x * 128 + threadIdx.x;
+  if (t >= 4 * g.V) return;
+  int id = t >> 2, mu = t & 3;
+  int c[4];
+  fcoords(id, g, c);
+  int o1 = fshift(c, mu, 1, g);
+  for (int nu = 0; nu < 4; nu++) {
+    if (nu == mu) continue;
+    int o2 = fshift(c, nu, 1, g);
+    int o3 = fshift(c, nu, -1, g);
+    int c3[4];
+    fcoords(o3, g, c3);
+    int o4 = fshift(c3, mu, 1, g);
+    int c2[4];
+    fcoords(o2, g, c2);
+    int o5 = fshift(c2, mu, 1, g);
+    int num, sign;
+    if (mu < nu) { num = nu * (nu - 1) / 2 + mu; sign = 1; }
+    else { num = mu * (mu - 1) / 2 + nu; sign = -1; }
+    cd W0[D][D], W1[D][D], W2[D][D], W3[D][D], W4[D][D], W5[D][D], W6[D][D], W7[D][D], W8[D][D];
+    cd Z[D][D], T1[D][D], T2[D][D], Fm[D][D];
+    load_link_bc<D>(W0, links, o3, mu, g); dag<D>(W0);
+    load_link_bc<D>(W1, links, o3, nu, g);
+    load_link_bc<D>(W2, links, o1, nu, g);
+    load_link_bc<D>(W3, links, o2, mu, g); dag<D>(W3);
+    load_link_bc<D>(W4, links, id, nu, g); dag<D>(W4);
+    load_link_bc<D>(W5, links, o4, nu, g); dag<D>(W5);
+    mmul<D>(W6, W0, W1);
+    mmul<D>(W7, W2, W3);
+    mmul<D>(W8, W7, W4);
+    mmul<D>(T1, W5, W6);
+    madd<D>(W8, T1, -1.0);
+    // fmat = W8 Z0 + Z1 W8
+    load_clf<D>(Z, clf, num, id, g.V);
+    mmul<D>(Fm, W8, Z);
+    load_clf<D>(Z, clf, num, o1, g.V);
+    mmul<D>(T1, Z, W8);
+    madd<D>(Fm, T1, 1.0);
+    // fmat -= W5 (W0 Z2 W1 + Z3 W6)
+    load_clf<D>(Z, clf, num, o3, g.V);
+    mmul<D>(T1, W0, Z);
+    mmul<D>(T2, T1, W1);
+    load_clf<D>(Z, clf, num, o4, g.V);
+    mmul<D>(T1, Z, W6);
+    madd<D>(T2, T1, 1.0);
+    mmul<D>(T1, W5, T2);
+    madd<D>(Fm, T1, -1.0);
+    // fmat += (W2 Z4 W3 + W7 Z5) W4
+    load_clf<D>(Z, clf, num, o5, g.V);
+    mmul<D>(T1, W2, Z);
+    mmul<D>(T2, T1, W3);
+    load_clf<D>(Z, clf, num, o2, g.V);
+    mmul<D>(T1, W7, Z);
+    madd<D>(T2, T1, 1.0);
+    mmul<D>(T1, T2, W4);
+    madd<D>(Fm, T1, 1.0);
+    load_link_bc<D>(T1, links, id, mu, g);
+    mmul<D>(T2, T1, Fm);
+    project_add<NG, D>(mom, mu, id, g.V, T2, sign * coeff);
+  }
+}
+
+// momenta: reference layout (4*id + mu)[NG] <-> device SoA
+__global__ void k_mom_from_ref(const double* __restrict__ src, double* __restrict__ dst, int V, int NG) {
+  int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= 4 * V) return;
+  int id = t >> 2, mu = t & 3;
+  for (int a = 0; a < NG; a++) dst[(size_t)(mu * NG + a) * V + id] = src[(size_t)t * NG + a];
+}
+__global__ void k_mom_to_ref(const double* __restrict__ src, double* __restrict__ dst, int V, int NG) {
+  int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= 4 * V) return;
+  int id = t >> 2, mu = t & 3;
+  for (int a = 0; a < NG; a++) dst[(size_t)t * NG + a] = src[(size_t)(mu * NG + a) * V + id];
+}
+// clover force matrices -> reference cl_force layout [6*id + k] suNf
+__global__ void k_clf_to_ref(const double2* __restrict__ src, double* __restrict__ dst, int V, int D) {
+  int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= 6 * V) return;
+  int id = t / 6, k = t % 6;
+  double* o = dst + (size_t)t * 2 * D * D;
+  for (int e = 0; e < D * D; e++) {
+    double2 v = src[(size_t)(k * D * D + e) * V + id];
+    o[e] = v.x;
+    o[D * D + e] = v.y;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host wrappers
+// ---------------------------------------------------------------------------
+static int ng_of(Ctx* c) { return c->nc * c->nc - 1; }
+
+int force_alloc(Ctx* c) {
+  if (!c->mom) {
+    YH_HIP(hipMalloc(&c->mom, sizeof(double) * 4 * (size_t)ng_of(c) * c->g.V));
+    YH_HIP(hipMemsetAsync(c->mom, 0, sizeof(double) * 4 * (size_t)ng_of(c) * c->g.V, c->stream));
+  }
+  if (!c->clf) {
+    YH_HIP(hipMalloc(&c->clf, sizeof(double2) * 6 * (size_t)c->dr * c->dr * c->g.V));
+    YH_HIP(hipMemsetAsync(c->clf, 0, sizeof(double2) * 6 * (size_t)c->dr * c->dr * c->g.V, c->stream));
+  }
+  return YH_OK;
+}
+
+#define YH_FORCE_MODELS(...)                                                             \
+  if (c->nc == 3 && c->dr == 3) { constexpr int NG_ = 8, D_ = 3; __VA_ARGS__; }          \
+  else if (c->nc == 2 && c->dr == 2) { constexpr int NG_ = 3, D_ = 2; __VA_ARGS__; }     \
+  else if (c->nc == 2 && c->dr == 3) { constexpr int NG_ = 3, D_ = 3; __VA_ARGS__; }     \
+  else if (c->nc == 3 && c->dr == 6) { constexpr int NG_ = 8, D_ = 6; __VA_ARGS__; }     \
+  else if (c->nc == 4 && c->dr == 4) { constexpr int NG_ = 15, D_ = 4; __VA_ARGS__; }    \
+  else if (c->nc == 4 && c->dr == 6) { constexpr int NG_ = 15, D_ = 6; __VA_ARGS__; }    \
+  else { set_error("forces: unsupported (nc, representation)"); return YH_ERR_UNSUPPORTED; }
+
+int force_hop(Ctx* c, double coeff, const Field* lhs, const Field* rhs) {
+  int rc;
+  if ((rc = force_alloc(c))) return rc;
+  FF f;
+  f.x[0] = lhs->half[0]; f.x[1] = lhs->half[1];
+  f.y[0] = rhs->half[0]; f.y[1] = rhs->half[1];
+  const double2* links = (const double2*)c->links[PREC_F64][RECON_18];
+  dim3 grid((4 * c->g.V + 255) / 256);
+  YH_FORCE_MODELS(hipLaunchKernelGGL((k_force_hop<NG_, D_>), grid, dim3(256), 0, c->stream, f, links, c->mom, coeff, c->g));
+  YH_HIP(hipGetLastError());
+  return YH_OK;
+}
+
+int force_clover_site(Ctx* c, double residue, const Field* lhs, const Field* rhs) {
+  int rc;
+  if ((rc = force_alloc(c))) return rc;
+  FF f;
+  f.x[0] = lhs->half[0]; f.x[1] = lhs->half[1];
+  f.y[0] = rhs->half[0]; f.y[1] = rhs->half[1];
+  dim3 grid((c->g.V + 255) / 256);
+  switch (c->dr) {
+    case 2: hipLaunchKernelGGL(k_force_clover_site<2>, grid, dim3(256), 0, c->stream, f, (double2*)c->clf, residue, c->g); break;
+    case 3: hipLaunchKernelGGL(k_force_clover_site<3>, grid, dim3(256), 0, c->stream, f, (double2*)c->clf, residue, c->g); break;
+    case 4: hipLaunchKernelGGL(k_force_clover_site<4>, grid, dim3(256), 0, c->stream, f, (double2*)c->clf, residue, c->g); break;
+    case 6: hipLaunchKernelGGL(k_force_clover_site<6>, grid, dim3(256), 0, c->stream, f, (double2*)c->clf, residue, c->g); break;
+    default: set_error("unsupported representation dimension"); return YH_ERR_UNSUPPORTED;
+  }
+  YH_HIP(hipGetLastError());
+  return YH_OK;
+}
+
+int force_logdet(Ctx* c, double mass, double coeff) {
+  int rc;
+  if ((rc = force_alloc(c))) return rc;
+  if ((rc = clover_invert(c, 4.0 + mass))) return rc;
+  dim3 grid((c->g.VH + 127) / 128);
+  const double2* inv = (const double2*)c->clinv[PREC_F64];
+  switch (c->dr) {
+    case 2: hipLaunchKernelGGL(k_force_logdet<2>, grid, dim3(128), 0, c->stream, inv, (double2*)c->clf, coeff, c->g); break;
+    case 3: hipLaunchKernelGGL(k_force_logdet<3>, grid, dim3(128), 0, c->stream, inv, (double2*)c->clf, coeff, c->g); break;
+    case 4: hipLaunchKernelGGL(k_force_logdet<4>, grid, dim3(128), 0, c->stream, inv, (double2*)c->clf, coeff, c->g); break;
+    case 6: hipLaunchKernelGGL(k_force_logdet<6>, grid, dim3(128), 0, c->stream, inv, (double2*)c->clf, coeff, c->g); break;
+    default: set_error("unsupported representation dimension"); return YH_ERR_UNSUPPORTED;
+  }
+  YH_HIP(hipGetLastError());
+  return YH_OK;
+}
+
+int force_clover_core(Ctx* c, double dt) {
+  int rc;
+  if ((rc = force_alloc(c))) return rc;
+  double coeff = c->csw * dt / 8.0;
+  const double2* links = (const double2*)c->links[PREC_F64][RECON_18];
+  dim3 grid((4 * c->g.V + 127) / 128);
+  YH_FORCE_MODELS(hipLaunchKernelGGL((k_force_clover_core<NG_, D_>), grid, dim3(128), 0, c->stream, (const double2*)c->clf,
+                                     links, c->mom, coeff, c->g));
+  YH_HIP(hipGetLastError());
+  return YH_OK;
+}
+
+int clover_force_zero(Ctx* c) {
+  int rc;
+  if ((rc = force_alloc(c))) return rc;
+  YH_HIP(hipMemsetAsync(c->clf, 0, sizeof(double2) * 6 * (size_t)c->dr * c->dr * c->g.V, c->stream));
+  return YH_OK;
+}
+
+int momenta_upload(Ctx* c, const double* host) {
+  int rc;
+  if ((rc = force_alloc(c))) return rc;
+  size_t n = (size_t)4 * c->g.V * ng_of(c);
+  double* d;
+  YH_HIP(hipMalloc(&d, n * sizeof(double)));
+  hipError_t e = hipMemcpyAsync(d, host, n * sizeof(double), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess)
+    hipLaunchKernelGGL(k_mom_from_ref, dim3((4 * c->g.V + 255) / 256), dim3(256), 0, c->stream, d, c->mom, c->g.V, ng_of(c));
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess) { set_error(hipGetErrorString(e)); return YH_ERR_HIP; }
+  return YH_OK;
+}
+
+int momenta_download(Ctx* c, double* host) {
+  int rc;
+  if ((rc = force_alloc(c))) return rc;
+  size_t n = (size_t)4 * c->g.V * ng_of(c);
+  double* d;
+  YH_HIP(hipMalloc(&d, n * sizeof(double)));
+  hipLaunchKernelGGL(k_mom_to_ref, dim3((4 * c->g.V + 255) / 256), dim3(256), 0, c->stream, c->mom, d, c->g.V, ng_of(c));
+  hipError_t e = hipMemcpyAsync(host, d, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess) { set_error(hipGetErrorString(e)); return YH_ERR_HIP; }
+  return YH_OK;
+}
+
+int clover_force_download(Ctx* c, double* host) {
+  int rc;
+  if ((rc = force_alloc(c))) return rc;
+  size_t n = (size_t)6 * c->g.V * 2 * c->dr * c->dr;
+  double* d;
+  YH_HIP(hipMalloc(&d, n * sizeof(double)));
+  hipLaunchKernelGGL(k_clf_to_ref, dim3((6 * c->g.V + 255) / 256), dim3(256), 0, c->stream, (const double2*)c->clf, d,
+                     c->g.V, c->dr);
+  hipError_t e = hipMemcpyAsync(host, d, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess) { set_error(hipGetErrorString(e)); return YH_ERR_HIP; }
+  return YH_OK;
+}
+
+}  // namespace yh

@@ -74,6 +74,9 @@ struct Ctx {
   long mvm_counter = 0;
   // operator scratch (odd halves / even halves)
   Field* tmp[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  // molecular dynamics: momenta (SoA, FP64) and clover force matrices
+  double* mom = nullptr;
+  void* clf = nullptr;
 };
 
 // ---- fields ---------------------------------------------------------------
@@ -126,6 +129,18 @@ int blas_caxpy(Ctx* c, Field* y, double ar, double ai, const Field* x, int parit
 int blas_cscale(Ctx* c, Field* y, double ar, double ai, int parity);
 int blas_g5(Ctx* c, Field* f, int parity);
 int blas_copy(Ctx* c, Field* dst, const Field* src, int parity);
+
+// ---- forces -----------------------------------------------------------------
+int force_init_constants(Ctx* c);
+int force_alloc(Ctx* c);
+int force_hop(Ctx* c, double coeff, const Field* lhs, const Field* rhs);
+int force_clover_site(Ctx* c, double residue, const Field* lhs, const Field* rhs);
+int force_logdet(Ctx* c, double mass, double coeff);
+int force_clover_core(Ctx* c, double dt);
+int clover_force_zero(Ctx* c);
+int momenta_upload(Ctx* c, const double* host);
+int momenta_download(Ctx* c, double* host);
+int clover_force_download(Ctx* c, double* host);
 
 // ---- host helpers shared by the solver translation units -------------------
 Field* tmp_field(Ctx* c, int prec, int slot, int* rc);
