@@ -162,6 +162,43 @@ int yh_cg_mshift(yh_ctx* ctx, const yh_inv_par* par, int num, const double* shif
  * updates = FP64 reliable updates */
 int yh_cg_mixed(yh_ctx* ctx, const yh_inv_par* par, yh_field* dst, const yh_field* src, int* iters, int* updates);
 
+/* batched multishift CG (project-new, generalises cg_mshift, cg.cpp:11-130):
+ * nrhs independent right-hand sides src[j], num shifts each; solutions
+ * dst[j * num + s] solve (H^2 + shift[s]) dst = src[j] (mvm must be
+ * YH_MVM_HPHI_SQ); all recurrences run on the device; iters[j] per rhs */
+int yh_cg_mshift_batched(yh_ctx* ctx, const yh_inv_par* par, int nrhs, int num, const double* shift, yh_field** dst,
+                         const yh_field** src, int* iters);
+
+/* --- rational approximations (include/rational.h:7-21) --------------------- */
+/* same fields as rational_app: r(x) = a0 prod (x + root) / (x + pole)
+ * = a0 + sum residue / (x + pole) approximates x^(n/d) on [min, max] */
+typedef struct {
+  int n;
+  int d;
+  int order;
+  double prec;   /* maximum relative error on [min, max] */
+  double min;
+  double max;
+  double a0;
+  double pole[48];
+  double root[48];
+  double residue[48];
+} yh_rational;
+/* Zolotarev's optimal approximation of x^(+-1/2) of the given order
+ * (replaces the table lookup of rational_find, rational.cpp:56-132) */
+int yh_rational_zolotarev(int order, int n, int d, double min, double max, yh_rational* out);
+/* lowest order with maximum relative error below prec (rational_find) */
+int yh_rational_find(int n, int d, double prec, double min, double max, yh_rational* out);
+double yh_rational_eval(const yh_rational* r, double x);
+/* rational_calc (rational.cpp:134-166): dst = a0 src + sum residue_i
+ * (H^2 + pole_i)^-1 src, through one multishift solve; FP64 sum */
+int yh_rational_calc(yh_ctx* ctx, const yh_rational* r, const yh_inv_par* par, yh_field* dst, const yh_field* src,
+                     int* iters);
+/* Jacobi sn(u; k) and K(k), with k given through its complement
+ * kc = sqrt(1 - k^2) (helpers of the Zolotarev construction) */
+double yh_jacobi_sn(double u, double kc);
+double yh_ellip_k(double kc);
+
 /* --- forces (include/forces.h:6-15, library/forces.cpp) -------------------- */
 /* momenta afield_momenta: 4V suNa{av[NC*NC-1]} doubles, index 4*id+mu
  * (include/gaugefield.h:12-18) */
@@ -191,6 +228,11 @@ int yh_bench_hop(yh_ctx* ctx, int prec, int recon, int reps, yh_field* dst, cons
 /* time `reps` plain copies of `bytes` bytes with a vector copy kernel */
 int yh_bench_copy(yh_ctx* ctx, long bytes, int reps, double* ms);
 int yh_synchronize(yh_ctx* ctx);
+/* tuning knobs of the kernels (no counterpart in the reference):
+ * YH_TUNE_XCD_CHUNK = consecutive workgroups each XCD takes (0 = one slab
+ * per XCD; default 1/8 of a t-slice) */
+#define YH_TUNE_XCD_CHUNK 1
+int yh_tune(yh_ctx* ctx, int key, int value);
 
 #ifdef __cplusplus
 }

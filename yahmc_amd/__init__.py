@@ -36,6 +36,16 @@ class YahmcError(RuntimeError):
         self.code = code
 
 
+class Rational(ctypes.Structure):
+    """yh_rational: the fields of rational_app (include/rational.h:7-18)."""
+    _fields_ = [("n", ctypes.c_int), ("d", ctypes.c_int), ("order", ctypes.c_int), ("prec", ctypes.c_double),
+                ("min", ctypes.c_double), ("max", ctypes.c_double), ("a0", ctypes.c_double),
+                ("pole", ctypes.c_double * 48), ("root", ctypes.c_double * 48), ("residue", ctypes.c_double * 48)]
+
+    def __call__(self, x):
+        return lib().yh_rational_eval(ctypes.byref(self), x)
+
+
 class _Lattice(ctypes.Structure):
     _fields_ = [("dims", ctypes.c_int * 4), ("nc", ctypes.c_int), ("repr", ctypes.c_int), ("bc", ctypes.c_int),
                 ("csw", ctypes.c_double), ("cf", ctypes.c_double), ("device", ctypes.c_int)]
@@ -91,6 +101,16 @@ _SIGS = {
     "yh_bicgstab": ([_VP, ctypes.POINTER(InvPar), _VP, _VP, _PI], ctypes.c_int),
     "yh_cg_mshift": ([_VP, ctypes.POINTER(InvPar), ctypes.c_int, _D, ctypes.POINTER(_VP), _VP, _PI], ctypes.c_int),
     "yh_cg_mixed": ([_VP, ctypes.POINTER(InvPar), _VP, _VP, _PI, _PI], ctypes.c_int),
+    "yh_cg_mshift_batched": ([_VP, ctypes.POINTER(InvPar), ctypes.c_int, ctypes.c_int, _D, ctypes.POINTER(_VP),
+                              ctypes.POINTER(_VP), _PI], ctypes.c_int),
+    "yh_rational_zolotarev": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                               ctypes.POINTER(Rational)], ctypes.c_int),
+    "yh_rational_find": ([ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                          ctypes.POINTER(Rational)], ctypes.c_int),
+    "yh_rational_eval": ([ctypes.POINTER(Rational), ctypes.c_double], ctypes.c_double),
+    "yh_rational_calc": ([_VP, ctypes.POINTER(Rational), ctypes.POINTER(InvPar), _VP, _VP, _PI], ctypes.c_int),
+    "yh_jacobi_sn": ([ctypes.c_double, ctypes.c_double], ctypes.c_double),
+    "yh_ellip_k": ([ctypes.c_double], ctypes.c_double),
     "yh_set_momenta": ([_VP, _D], ctypes.c_int),
     "yh_get_momenta": ([_VP, _D], ctypes.c_int),
     "yh_get_clover_force": ([_VP, _D], ctypes.c_int),
@@ -103,11 +123,27 @@ _SIGS = {
     "yh_bench_hop": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP, _VP, _D], ctypes.c_int),
     "yh_bench_copy": ([_VP, ctypes.c_long, ctypes.c_int, _D], ctypes.c_int),
     "yh_synchronize": ([_VP], ctypes.c_int),
+    "yh_tune": ([_VP, ctypes.c_int, ctypes.c_int], ctypes.c_int),
 }
 
 
 def exported_symbols():
     return sorted(_SIGS)
+
+
+def rational_zolotarev(order, n, d, xmin, xmax):
+    """Zolotarev approximation of x^(n/d), n/d = +-1/2 (host computation)."""
+    r = Rational()
+    _check(lib().yh_rational_zolotarev(order, n, d, xmin, xmax, ctypes.byref(r)))
+    return r
+
+
+def rational_find(n, d, prec, xmin, xmax):
+    """Lowest-order approximation of x^(n/d) with max relative error < prec
+    on [xmin, xmax] (mirrors rational_find, library/rational.cpp:56-132)."""
+    r = Rational()
+    _check(lib().yh_rational_find(n, d, prec, xmin, xmax, ctypes.byref(r)))
+    return r
 
 
 def lib():
@@ -321,6 +357,22 @@ class Lattice:
         _check(lib().yh_cg_mshift(self.ctx, ctypes.byref(par), len(dsts), _dp(shifts), arr, s.h, ctypes.byref(it)))
         return it.value
 
+    def cg_mshift_batched(self, par, shifts, dsts, srcs):
+        """dsts: list (per rhs) of lists (per shift) of fields."""
+        shifts = np.ascontiguousarray(shifts, dtype=np.float64)
+        flat = [f.h for row in dsts for f in row]
+        arr = (_VP * len(flat))(*flat)
+        sarr = (_VP * len(srcs))(*[s.h for s in srcs])
+        its = (ctypes.c_int * len(srcs))()
+        _check(lib().yh_cg_mshift_batched(self.ctx, ctypes.byref(par), len(srcs), len(shifts), _dp(shifts), arr, sarr,
+                                          its))
+        return list(its)
+
+    def rational_calc(self, rat, par, d, s):
+        it = ctypes.c_int(0)
+        _check(lib().yh_rational_calc(self.ctx, ctypes.byref(rat), ctypes.byref(par), d.h, s.h, ctypes.byref(it)))
+        return it.value
+
     def cg_mixed(self, par, d, s):
         it = ctypes.c_int(0)
         up = ctypes.c_int(0)
@@ -379,3 +431,6 @@ class Lattice:
 
     def synchronize(self):
         _check(lib().yh_synchronize(self.ctx))
+
+    def tune(self, key, value):
+        _check(lib().yh_tune(self.ctx, key, value))

@@ -180,7 +180,7 @@ int default_recon(Ctx* c, int prec) {
 // side: if non-null receives -D_oe in_e in its odd half (dirac.cpp:743).
 // reduce_slot >= 0 (Hphi_sq only): |Hphi in|^2 into sc->red[reduce_slot]
 int apply_op(Ctx* c, int prec, int recon, int mvm, double mass, Field* out, Field* in, Field* side, int reduce_slot,
-             const int* skip) {
+             const int* skip, double* reduce_ptr) {
   int rc;
   const bool clover = c->csw != 0.0;
   if (clover && (rc = clover_invert(c, 4.0 + mass))) return rc;
@@ -203,7 +203,7 @@ int apply_op(Ctx* c, int prec, int recon, int mvm, double mass, Field* out, Fiel
     h2.self = src; h2.scale = -0.5; h2.epi = EPI_CLOV; h2.gamma5 = g5; h2.skip_flag = skip;
     h2.diag = clover ? 4.0 + mass : (4.0 + mass) * (4.0 + mass);
     h2.zero_self_bc = 1;
-    h2.reduce_norm = red >= 0; h2.reduce_slot = red >= 0 ? red : 0;
+    h2.reduce_norm = red >= 0; h2.reduce_slot = red >= 0 ? red : 0; h2.reduce_ptr = reduce_ptr;
     return hop_launch(c, h2);
   };
   if (mvm == YH_MVM_DPHI) return half_op(out, in, 0, -1, side);
@@ -348,6 +348,10 @@ int yh_create(const yh_lattice* lat, yh_ctx** out) {
   if (lat->bc == YH_BC_OPEN) { c->g.Ta = 1; c->g.Tb = lat->dims[0] - 2; }
   if (lat->bc == YH_BC_SF) { c->g.Ta = 1; c->g.Tb = lat->dims[0] - 3; }
   c->nblk = (c->g.VH + 255) / 256;
+  // XCD chunk: each XCD takes 1/8 of a t-slice in turn, so the t +- 1
+  // neighbours of its sites stay in its own L2 (best of a sweep at 32^4:
+  // +10% over one slab per XCD, +15% over the hardware round robin)
+  c->g.xq = std::max(1, (int)lround((double)c->g.VH / c->g.L[0] / (256.0 * 8.0)));
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMalloc(&c->partials, sizeof(double) * (size_t)(c->nblk + 64) * 8);
   if (e == hipSuccess) e = hipMalloc(&c->ticket, sizeof(unsigned) * 64);
@@ -595,6 +599,17 @@ int yh_cg(yh_ctx* c, const yh_inv_par* par, yh_field* dst, const yh_field* src, 
   int rc = par_check(c, par, dst, src, iters);
   if (rc) return rc;
   return cg_device(c, PREC_F64, RECON_18, par, dst, src, iters, nullptr, nullptr);
+}
+
+int yh_tune(yh_ctx* c, int key, int value) {
+  if (!c) { set_error("null context"); return YH_ERR_ARG; }
+  if (key == YH_TUNE_XCD_CHUNK) {
+    if (value < 0) { set_error("chunk must be >= 0"); return YH_ERR_ARG; }
+    c->g.xq = value;
+    return YH_OK;
+  }
+  set_error("unknown tuning key");
+  return YH_ERR_ARG;
 }
 
 int yh_synchronize(yh_ctx* c) {

@@ -41,6 +41,7 @@ struct Geom {
   int bc;         // boundary condition
   int Ta, Tb;     // time slices carrying the c_f improvement (open / SF)
   double cf;      // c_f (open / SF); the diagonal gets c_f - 1 extra there
+  int xq;         // XCD chunk: consecutive workgroups one XCD takes in a row (0: nblocks / 8)
 };
 
 template <typename R>
@@ -116,15 +117,22 @@ __device__ __forceinline__ double bc_cf_shift(int t, const Geom& g) {
   return 0.0;
 }
 
-// XCD-aware block order: the hardware deals consecutive workgroups round
-// robin to the 8 XCDs; remap so that each XCD sweeps one contiguous slab
-// of the lattice and neighbouring t-slices share that XCD's L2.
-__device__ __forceinline__ int xcd_block(int nblocks) {
-  int b = blockIdx.x;
-  const int nx = 8;
-  int per = nblocks / nx;
-  if (per == 0 || b >= per * nx) return b;
-  return (b % nx) * per + b / nx;
+// XCD-aware block order.  The hardware deals consecutive workgroups round
+// robin to the 8 XCDs (physical block b runs on XCD b % 8 as that XCD's
+// (b / 8)-th block).  Remap so that XCD x takes `q` consecutive logical
+// blocks of every group of 8q: with q = nblocks / 8 each XCD sweeps one
+// contiguous slab; smaller q interleaves the XCDs at a finer grain so the
+// t +- 1 reuse window of each XCD stays small.  The map is a bijection on
+// [0, G*8q) and the identity on the tail.
+__device__ __forceinline__ int xcd_block(int nblocks, int q) {
+  const int b = blockIdx.x;
+  if (q <= 0) q = nblocks / 8;
+  if (q <= 0) return b;
+  const int group = 8 * q;
+  const int G = nblocks / group;
+  if (b >= G * group) return b;
+  const int x = b & 7, s = b >> 3;
+  return (s / q) * group + x * q + (s % q);
 }
 
 // ---------------------------------------------------------------------------

@@ -66,7 +66,7 @@ int hop_launch(Ctx* c, const HopCall& h) {
   a.reduce = h.reduce_norm;
   a.partials = c->partials;
   a.ticket = c->ticket;
-  a.red_out = h.reduce_norm ? &c->sc->red[0] + h.reduce_slot : nullptr;
+  a.red_out = h.reduce_norm ? (h.reduce_ptr ? h.reduce_ptr : &c->sc->red[0] + h.reduce_slot) : nullptr;
   a.skip = h.skip_flag;
   a.g = c->g;
   c->mvm_counter++;

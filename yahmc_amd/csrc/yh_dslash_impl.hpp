@@ -106,12 +106,29 @@ __device__ __forceinline__ void hop_dir(Cx<R> (&acc)[4][D], const HopArgs& a, in
   }
 }
 
+// minimum waves per SIMD requested from the compiler per precision (caps
+// the VGPR budget: 512 / waves).  A sweep on MI355X (32^4) found every
+// tighter bound slower than none: the extra waves do not pay for the spills.
+#ifndef YH_MINW_F64
+#define YH_MINW_F64 1
+#endif
+#ifndef YH_MINW_F32
+#define YH_MINW_F32 1
+#endif
+#ifndef YH_MINW_F16
+#define YH_MINW_F16 1
+#endif
+template <int PREC>
+constexpr int hop_min_waves() {
+  return PREC == PREC_F64 ? YH_MINW_F64 : PREC == PREC_F32 ? YH_MINW_F32 : YH_MINW_F16;
+}
+
 template <int D, int PREC, int RECON, int EPI>
-__global__ void __launch_bounds__(256) k_hop(HopArgs a) {
+__global__ void __launch_bounds__(256, hop_min_waves<PREC>()) k_hop(HopArgs a) {
   typedef typename SpinorStore<PREC, D>::real R;
   if (a.skip && *a.skip) return;
   const int nblk = gridDim.x;
-  const int blk = xcd_block(nblk);
+  const int blk = xcd_block(nblk, a.g.xq);
   const int k = blk * 256 + threadIdx.x;
   const Geom& g = a.g;
   double nrm2 = 0.0;

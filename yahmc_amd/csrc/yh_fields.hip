@@ -84,9 +84,16 @@ int clover_logdet_sum(Ctx* c, double* out) {
 // plain 16-byte vector copy: the calibration kernel for HBM bandwidth and
 // for the FETCH_SIZE / WRITE_SIZE counters
 __global__ void __launch_bounds__(256) k_copy16(const uint4* __restrict__ a, uint4* __restrict__ b, long n) {
-  long i = (long)blockIdx.x * 256 + threadIdx.x;
-  long stride = (long)gridDim.x * 256;
-  for (; i < n; i += stride) b[i] = a[i];
+  // each workgroup copies 4 consecutive 4 KiB chunks; four independent
+  // loads in flight per lane
+  long base = (long)blockIdx.x * 1024 + threadIdx.x;
+  uint4 v[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++)
+    if (base + u * 256 < n) v[u] = a[base + u * 256];
+#pragma unroll
+  for (int u = 0; u < 4; u++)
+    if (base + u * 256 < n) b[base + u * 256] = v[u];
 }
 
 int copy_kernel_bench(Ctx* c, long bytes, int reps, double* ms) {
@@ -96,7 +103,9 @@ int copy_kernel_bench(Ctx* c, long bytes, int reps, double* ms) {
   YH_HIP(hipMalloc(&a, n * 16));
   YH_HIP(hipMalloc(&b, n * 16));
   YH_HIP(hipMemsetAsync(a, 1, n * 16, c->stream));
-  int grid = (int)std::min<long>((n + 255) / 256, 256L * 64);
+  long gl = (n + 1023) / 1024;
+  if (gl > (1L << 30)) { set_error("copy size too large"); return YH_ERR_ARG; }
+  int grid = (int)gl;
   hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, c->stream, a, b, n);
   hipEvent_t e0, e1;
   YH_HIP(hipEventCreate(&e0));

@@ -114,6 +114,7 @@ struct HopCall {
   int zero_self_bc;       // EPI_CLOV: the reference zeroes boundary slices of the input
   int reduce_norm;        // EPI_CLOV: |out|^2 into sc->red[reduce_slot]
   int reduce_slot;
+  double* reduce_ptr;     // if non-null, the reduction goes here instead of sc->red[reduce_slot]
   const int* skip_flag;   // if non-null and *skip_flag, every thread returns
 };
 int hop_launch(Ctx* c, const HopCall& h);
@@ -146,7 +147,9 @@ int clover_force_download(Ctx* c, double* host);
 Field* tmp_field(Ctx* c, int prec, int slot, int* rc);
 int default_recon(Ctx* c, int prec);
 int apply_op(Ctx* c, int prec, int recon, int mvm, double mass, Field* out, Field* in, Field* side, int reduce_slot,
-             const int* skip);
+             const int* skip, double* reduce_ptr = nullptr);
+int mshift_device(Ctx* c, const yh_inv_par* par, int nrhs, int num, const double* shift, Field** x, const Field** b,
+                  int* iters);
 int read_scalars(Ctx* c);
 int write_scalars(Ctx* c);
 int reduce_read(Ctx* c, int which, const Field* a, const Field* b, int par, double* out2);

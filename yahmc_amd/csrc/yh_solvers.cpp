@@ -116,6 +116,29 @@ int yh_bicgstab(yh_ctx* cc, const yh_inv_par* par, yh_field* dptr, const yh_fiel
   return sync(c);
 }
 
+int yh_cg_mshift_batched(yh_ctx* cc, const yh_inv_par* par, int nrhs, int num, const double* shift, yh_field** dptr,
+                         const yh_field** sptr, int* iters) {
+  Ctx* c = cc;
+  if (!c || !par || !shift || !dptr || !sptr || !iters || num < 1 || num > 48 || nrhs < 1 || nrhs > 16) {
+    set_error("bad argument");
+    return YH_ERR_ARG;
+  }
+  if (!c->gauge_set) { set_error("no gauge field"); return YH_ERR_STATE; }
+  if (!(par->prec > 0.0)) { set_error("prec must be positive"); return YH_ERR_ARG; }
+  if (par->mvm != YH_MVM_HPHI_SQ) { set_error("batched multishift needs Hphi_sq"); return YH_ERR_ARG; }
+  std::vector<Field*> x(nrhs * num);
+  std::vector<const Field*> b(nrhs);
+  for (int j = 0; j < nrhs; j++) {
+    if (!sptr[j] || sptr[j]->prec != PREC_F64) { set_error("fields must be FP64"); return YH_ERR_ARG; }
+    b[j] = sptr[j];
+  }
+  for (int i = 0; i < nrhs * num; i++) {
+    if (!dptr[i] || dptr[i]->prec != PREC_F64) { set_error("fields must be FP64"); return YH_ERR_ARG; }
+    x[i] = dptr[i];
+  }
+  return mshift_device(c, par, nrhs, num, shift, x.data(), b.data(), iters);
+}
+
 int yh_cg_mshift(yh_ctx* cc, const yh_inv_par* par, int num, const double* shift, yh_field** dptr,
                  const yh_field* sptr, int* iters) {
   Ctx* c = cc;
@@ -124,6 +147,12 @@ int yh_cg_mshift(yh_ctx* cc, const yh_inv_par* par, int num, const double* shift
   if (!(par->prec > 0.0)) { set_error("prec must be positive"); return YH_ERR_ARG; }
   for (int n = 0; n < num; n++)
     if (!dptr[n] || dptr[n]->prec != PREC_F64) { set_error("solution fields must be FP64"); return YH_ERR_ARG; }
+  if (par->mvm == YH_MVM_HPHI_SQ && num <= 48) {
+    // device-resident recurrences (the usual case: H^2 in RHMC)
+    std::vector<Field*> x(dptr, dptr + num);
+    const Field* b = sptr;
+    return mshift_device(c, par, 1, num, shift, x.data(), &b, iters);
+  }
   int rc;
   Field *r, *p, *q;
   if ((rc = need(c, PREC_F64, 2, &r)) || (rc = need(c, PREC_F64, 3, &p)) || (rc = need(c, PREC_F64, 4, &q))) return rc;
