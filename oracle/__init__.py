@@ -99,6 +99,12 @@ class Oracle:
         lib.or_hmc_action.argtypes = [ctypes.c_double, ctypes.c_double, _D]
         lib.or_hmc_action.restype = ctypes.c_double
         lib.or_spin_matrix.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _D]
+        lib.or_wilson_gauge_force.argtypes = [ctypes.c_double, ctypes.c_double]
+        lib.or_plaquette.restype = ctypes.c_double
+        lib.or_gauge_action.argtypes = [ctypes.c_double]
+        lib.or_gauge_action.restype = ctypes.c_double
+        lib.or_kinetic.restype = ctypes.c_double
+        lib.or_update_links.argtypes = [ctypes.c_double]
 
     # ---- lattice -------------------------------------------------------
     def init(self, dims, bc=BC_ANTIPERIODIC, csw=1.0, cf=1.0):
@@ -298,6 +304,25 @@ class Oracle:
 
     def hmc_action(self, mass, prec, pf):
         return self.lib.or_hmc_action(mass, prec, _p(np.ascontiguousarray(pf)))
+
+    # ---- gauge MD (forces.cpp:262-283, update.cpp:201-247) -------------
+    def wilson_gauge_force(self, dt, beta):
+        self.lib.or_wilson_gauge_force(dt, beta)
+
+    def plaquette(self):
+        return self.lib.or_plaquette()
+
+    def gauge_action(self, beta):
+        return self.lib.or_gauge_action(beta)
+
+    def kinetic(self):
+        return self.lib.or_kinetic()
+
+    def update_links(self, eps):
+        self.lib.or_update_links(eps)
+
+    def reunitarize(self):
+        self.lib.or_reunitarize()
 
     def spin_matrix(self, kind, mu, nu=0):
         out = np.zeros(32)

@@ -1400,3 +1400,170 @@ double or_hmc_action(double mass, double prec, const double *pf)
 	free(y); free(b);
 	return v;
 }
+
+/* ------------------------------------------------------------------ */
+/* gauge MD: library/forces.cpp:262-283 (wilson_gauge_force),            */
+/* library/staples.cpp:82-107 (staples_wilson), library/wilsonloop.cpp:  */
+/* 58-133 (plaquette), library/update.cpp:201-247 (hamiltonian,          */
+/* update_links), library/suNa.cpp:113-134 (exp), library/suNg.cpp:       */
+/* 171-359 (reunitarize).  Periodic gauge boundaries (plaq weights 1).   */
+/* ------------------------------------------------------------------ */
+static void gstore(double *p, gmat M)
+{
+	for (int i = 0; i < NC; i++)
+		for (int j = 0; j < NC; j++) { p[i * NC + j] = creal(M.m[i][j]); p[NC * NC + i * NC + j] = cimag(M.m[i][j]); }
+}
+
+static gmat gadd(gmat a, gmat b)
+{
+	for (int i = 0; i < NC; i++)
+		for (int j = 0; j < NC; j++) a.m[i][j] += b.m[i][j];
+	return a;
+}
+
+/* staples_wilson(id, mu) */
+static gmat staple_wilson(int id, int mu)
+{
+	gmat st;
+	memset(&st, 0, sizeof(st));
+	for (int nu = 0; nu < 4; nu++) {
+		if (nu == mu) continue;
+		int o1 = nb_fw[4 * id + mu], o2 = nb_fw[4 * id + nu];
+		st = gadd(st, gmul(LNK(o1, nu), gdag(gmul(LNK(id, nu), LNK(o2, mu)))));
+		o1 = nb_bk[4 * id + nu];
+		o2 = nb_fw[4 * o1 + mu];
+		st = gadd(st, gmul(gdag(gmul(LNK(o1, mu), LNK(o2, nu))), LNK(o1, nu)));
+	}
+	return st;
+}
+
+/* algebra_project(suNg, coeff): av_a = -(coeff/TF) Re Tr(iT_a M) */
+static void algebra_project_g(gmat M, double coeff, double *av, double sgn)
+{
+	for (int a = 0; a < NG; a++) {
+		double re = 0;
+		for (int i = 0; i < NC; i++)
+			for (int k = 0; k < NC; k++) re += creal(iTfund[a][i][k] * M.m[k][i]);
+		av[a] += sgn * (-(coeff / TF) * re);
+	}
+}
+
+/* wilson_gauge_force: momentum -= algebra_project(U staple, dt beta / NC) */
+void or_wilson_gauge_force(double dt, double beta)
+{
+	double coeff = dt * (beta / NC);
+#pragma omp parallel for
+	for (int id = 0; id < V; id++)
+		for (int mu = 0; mu < 4; mu++) {
+			gmat F = gmul(LNK(id, mu), staple_wilson(id, mu));
+			algebra_project_g(F, coeff, mom + (4 * (size_t)id + mu) * NG, -1.0);
+		}
+}
+
+/* avr_wilson(1,1): plaquette average Re Tr / (NC 6 V) */
+double or_plaquette(void)
+{
+	double sum = 0;
+#pragma omp parallel for reduction(+ : sum)
+	for (int id = 0; id < V; id++)
+		for (int mu = 1; mu < 4; mu++)
+			for (int nu = 0; nu < mu; nu++) {
+				int a = nb_fw[4 * id + mu], b = nb_fw[4 * id + nu];
+				gmat one = gmul(LNK(id, mu), LNK(a, nu));
+				gmat two = gmul(LNK(id, nu), LNK(b, mu));
+				gmat P = gmul(one, gdag(two));
+				for (int i = 0; i < NC; i++) sum += creal(P.m[i][i]);
+			}
+	return sum / (NC * 6.0 * V);
+}
+
+/* MonomialGauge::action for the Wilson action: -beta * plaq * 6 V */
+double or_gauge_action(double beta) { return -beta * or_plaquette() * 6.0 * V; }
+
+/* TF/2 sum |P|^2 (update.cpp:201-216) */
+double or_kinetic(void)
+{
+	double v = 0;
+	for (size_t k = 0; k < (size_t)4 * V * NG; k++) v += mom[k] * mom[k];
+	return v * (TF / 2.0);
+}
+
+static gmat reunitarize_g(gmat u)
+{
+	/* suNg::reunitarize: Gram-Schmidt rows; NC = 3 third row conj(r0 x r1) */
+	cplx r[NC][NC];
+	memcpy(r, u.m, sizeof(r));
+	double n0 = 0;
+	for (int j = 0; j < NC; j++) n0 += creal(r[0][j] * conj(r[0][j]));
+	n0 = sqrt(n0);
+	for (int j = 0; j < NC; j++) r[0][j] /= n0;
+	if (NC == 2) {
+		r[1][0] = -conj(r[0][1]);
+		r[1][1] = conj(r[0][0]);
+	} else {
+		for (int i = 1; i < NC; i++) {
+			if (NC == 3 && i == 2) {
+				r[2][0] = conj(r[0][1] * r[1][2] - r[0][2] * r[1][1]);
+				r[2][1] = conj(r[0][2] * r[1][0] - r[0][0] * r[1][2]);
+				r[2][2] = conj(r[0][0] * r[1][1] - r[0][1] * r[1][0]);
+				break;
+			}
+			for (int k = 0; k < i; k++) {
+				cplx p = 0;
+				for (int j = 0; j < NC; j++) p += conj(r[k][j]) * r[i][j];
+				for (int j = 0; j < NC; j++) r[i][j] -= p * r[k][j];
+			}
+			double n = 0;
+			for (int j = 0; j < NC; j++) n += creal(r[i][j] * conj(r[i][j]));
+			n = sqrt(n);
+			for (int j = 0; j < NC; j++) r[i][j] /= n;
+		}
+	}
+	gmat o;
+	memcpy(o.m, r, sizeof(r));
+	return o;
+}
+
+/* exp(suNa m, coeff): Taylor series until |X^k/k!|^2 < 1e-24, reunitarised */
+static gmat exp_alg(const double *av, double coeff)
+{
+	gmat X = alg_to_mat(av), R, Xk;
+	memset(&R, 0, sizeof(R));
+	memset(&Xk, 0, sizeof(Xk));
+	for (int i = 0; i < NC; i++) { R.m[i][i] = 1; Xk.m[i][i] = 1; }
+	double err = 1;
+	int k = 1;
+	while (err > 1e-24) {
+		Xk = gmul(Xk, X);
+		for (int i = 0; i < NC; i++)
+			for (int j = 0; j < NC; j++) Xk.m[i][j] *= coeff / k;
+		R = gadd(R, Xk);
+		err = 0;
+		for (int i = 0; i < NC; i++)
+			for (int j = 0; j < NC; j++) err += creal(Xk.m[i][j] * conj(Xk.m[i][j]));
+		k++;
+	}
+	return reunitarize_g(R);
+}
+
+/* update_links: U <- exp(eps P) U (update.cpp:228-247) */
+void or_update_links(double eps)
+{
+#pragma omp parallel for
+	for (int id = 0; id < V; id++)
+		for (int mu = 0; mu < 4; mu++) {
+			double *l = gfund + (4 * (size_t)id + mu) * NLG;
+			gmat E = exp_alg(mom + (4 * (size_t)id + mu) * NG, eps);
+			gstore(l, gmul(E, gload(l)));
+		}
+}
+
+void or_reunitarize(void)
+{
+#pragma omp parallel for
+	for (int id = 0; id < V; id++)
+		for (int mu = 0; mu < 4; mu++) {
+			double *l = gfund + (4 * (size_t)id + mu) * NLG;
+			gstore(l, reunitarize_g(gload(l)));
+		}
+}

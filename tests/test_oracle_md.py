@@ -1,0 +1,110 @@
+"""Pins the oracle's gauge molecular dynamics (library/forces.cpp:262-283,
+staples.cpp:82-136, wilsonloop.cpp:58-133, update.cpp:201-247,
+integrator.cpp:42-63, suNa.cpp:113-134):
+  - plaquette of the unit field is 1, and agrees with a numpy loop;
+  - the Wilson gauge force is -(1/TF) times the gradient of
+    S_g = -beta * plaq * 6V (finite differences along random directions);
+  - the link update equals scipy's matrix exponential;
+  - a 2nd-order minimal-norm trajectory conserves H up to O(dt^2) and is
+    reversible."""
+import numpy as np
+import pytest
+from scipy.linalg import expm
+
+from fields import complex_to_mat, eo_geometry, mat_to_complex, random_gauge, unit_gauge
+from oracle import BC_PERIODIC, Oracle
+from test_oracle_forces import alg_to_mat
+
+TF = 2.0
+
+
+def make(seed=4, dims=(4, 4, 4, 4)):
+    o = Oracle("su3_fund").init(dims, bc=BC_PERIODIC, csw=0.0)
+    rng = np.random.default_rng(seed)
+    o.set_gauge(random_gauge(rng, o.V, 3))
+    return o, rng
+
+
+def test_plaquette():
+    o, rng = make()
+    U = mat_to_complex(o.get_gauge(), 3).reshape(o.V, 4, 3, 3)
+    _, fw, _ = eo_geometry(o.dims)
+    s = 0.0
+    for x in range(o.V):
+        for mu in range(4):
+            for nu in range(mu):
+                P = U[x, mu] @ U[fw[x, mu], nu] @ U[fw[x, nu], mu].conj().T @ U[x, nu].conj().T
+                s += np.trace(P).real
+    assert abs(o.plaquette() - s / (3 * 6 * o.V)) < 1e-14
+    o.set_gauge(unit_gauge(o.V, 3))
+    assert abs(o.plaquette() - 1.0) < 1e-15
+
+
+def test_gauge_force_is_minus_gradient():
+    o, rng = make(seed=8)
+    beta = 5.6
+    links = o.get_gauge()
+    o.set_momenta(np.zeros((4 * o.V, 8)))
+    o.wilson_gauge_force(1.0, beta)
+    F = o.get_momenta()
+    Q = rng.normal(size=F.shape)
+    U = mat_to_complex(links, 3)
+    XQ = np.array([alg_to_mat(q, 3) for q in Q])
+
+    def S(tau):
+        o.set_gauge(complex_to_mat(np.array([expm(tau * x) @ u for x, u in zip(XQ, U)])))
+        return o.gauge_action(beta)
+
+    h = 1e-4
+    dS = (8 * (S(h) - S(-h)) - (S(2 * h) - S(-2 * h))) / (12 * h)
+    want = -TF * np.sum(Q * F)
+    scale = TF * np.sum(np.abs(Q * F))
+    assert abs(dS - want) < 1e-8 * scale
+
+
+def test_update_links_is_matrix_exponential():
+    o, rng = make(seed=2)
+    P = rng.normal(size=(4 * o.V, 8))
+    o.set_momenta(P)
+    U0 = mat_to_complex(o.get_gauge(), 3)
+    o.update_links(0.13)
+    U1 = mat_to_complex(o.get_gauge(), 3)
+    for k in range(0, 4 * o.V, 37):
+        assert np.abs(U1[k] - expm(0.13 * alg_to_mat(P[k], 3)) @ U0[k]).max() < 1e-13
+
+
+def o2mn(o, beta, length, steps):
+    """o2mn_multilevel with a single level (integrator.cpp:42-63)."""
+    a = 0.1931833275037836
+    dt = length / steps
+    for n in range(steps):
+        o.wilson_gauge_force(a * dt if n == 0 else 2 * a * dt, beta)
+        o.update_links(dt / 2)
+        o.wilson_gauge_force((1 - 2 * a) * dt, beta)
+        o.update_links(dt / 2)
+    o.wilson_gauge_force(a * dt, beta)
+
+
+def hamiltonian(o, beta):
+    return o.kinetic() + o.gauge_action(beta)
+
+
+def test_o2mn_energy_conservation_scales_as_dt2_and_is_reversible():
+    o, rng = make(seed=3)
+    beta = 5.6
+    U0 = o.get_gauge()
+    P0 = rng.normal(size=(4 * o.V, 8))
+    dH = []
+    for steps in (4, 8):
+        o.set_gauge(U0)
+        o.set_momenta(P0)
+        H0 = hamiltonian(o, beta)
+        o2mn(o, beta, 0.5, steps)
+        dH.append(abs(hamiltonian(o, beta) - H0))
+    ratio = dH[0] / dH[1]
+    assert 3.0 < ratio < 5.5, (dH, ratio)
+    # reversibility: flip the momenta and integrate back
+    o.set_momenta(-o.get_momenta())
+    o2mn(o, beta, 0.5, 8)
+    assert np.abs(o.get_gauge() - U0).max() < 1e-11
+    assert np.abs(o.get_momenta() + P0).max() < 1e-10
