@@ -105,6 +105,8 @@ class Oracle:
         lib.or_gauge_action.restype = ctypes.c_double
         lib.or_kinetic.restype = ctypes.c_double
         lib.or_update_links.argtypes = [ctypes.c_double]
+        lib.or_ptc_layer.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D, _D, ctypes.c_int]
+        lib.or_ptc_wgrad.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D, _D]
 
     # ---- lattice -------------------------------------------------------
     def init(self, dims, bc=BC_ANTIPERIODIC, csw=1.0, cf=1.0):
@@ -324,7 +326,105 @@ class Oracle:
     def reunitarize(self):
         self.lib.or_reunitarize()
 
+    # ---- parallel-transport convolution layers (learned preconditioner) --
+    def ptc_layer(self, W, h, adjoint=False):
+        """W: complex (9, cout, cin, 4, 4); h: complex (V, nin, 4, DR) with
+        nin = cin (forward) or cout (adjoint)."""
+        W = np.ascontiguousarray(W, dtype=np.complex128)
+        _, cout, cin = W.shape[:3]
+        assert max(cin, cout) <= 16
+        nout = cin if adjoint else cout
+        h = np.ascontiguousarray(h, dtype=np.complex128)
+        out = np.zeros((self.V, nout, 4, self.dr), dtype=np.complex128)
+        self.lib.or_ptc_layer(cin, cout, _p(W.view(np.float64)), _p(h.view(np.float64)),
+                              _p(out.view(np.float64)), int(adjoint))
+        return out
+
+    def ptc_wgrad(self, g, h):
+        """sum_{x,a} g_co(x)[s][a] conj((T_p h_ci)(x)[s'][a]) as (9, cout, cin, 4, 4)."""
+        g = np.ascontiguousarray(g, dtype=np.complex128)
+        h = np.ascontiguousarray(h, dtype=np.complex128)
+        cout, cin = g.shape[1], h.shape[1]
+        out = np.zeros((9, cout, cin, 4, 4), dtype=np.complex128)
+        self.lib.or_ptc_wgrad(cin, cout, _p(g.view(np.float64)), _p(h.view(np.float64)),
+                              _p(out.view(np.float64)))
+        return out
+
     def spin_matrix(self, kind, mu, nu=0):
         out = np.zeros(32)
         self.lib.or_spin_matrix(kind, mu, nu, _p(out))
         return (out[0::2] + 1j * out[1::2]).reshape(4, 4)
+
+
+# ---------------------------------------------------------------------------
+# The learned preconditioner's network on the even sublattice, restated in
+# FP64 on top of the layer functions above:
+#   A b = P_e L_n ... L_1 E b      (E: even -> full lattice, P_e: keep even)
+#   A^+ g = P_e L_1^+ ... L_n^+ E g
+#   preconditioner  M = A A^+      (Hermitian, positive semi-definite)
+# Training loss over (source b, solution x) pairs: |A A^+ b - x|^2 / |x|^2.
+# Its weight gradient (d/dRe W + i d/dIm W) is WG(u, e) + WG(v, b) with
+# u = A^+ b, e = 2 (A u - x) / |x|^2, v = A^+ e, where WG(w, y) is the
+# gradient of Re<y, A w>: the per-layer sum of ptc_wgrad(cotangent of the
+# layer output, layer input).
+# ---------------------------------------------------------------------------
+def spinor_to_act(o, s):
+    """Spinor layout (V, 8*DR) doubles -> activation (V, 1, 4, DR) complex."""
+    s = np.asarray(s, dtype=np.float64).reshape(o.V, 2, o.dr, 4)
+    c = s[:, 0] + 1j * s[:, 1]                      # (V, DR, 4)
+    return np.ascontiguousarray(c.transpose(0, 2, 1))[:, None]
+
+
+def act_to_spinor(o, a):
+    c = np.asarray(a)[:, 0].transpose(0, 2, 1)      # (V, DR, 4)
+    return np.ascontiguousarray(np.stack([c.real, c.imag], axis=1).reshape(o.V, 8 * o.dr))
+
+
+class OracleNet:
+    def __init__(self, o, weights):
+        self.o = o
+        self.W = [np.asarray(w, dtype=np.complex128) for w in weights]
+
+    def _even(self, a):
+        a = a.copy()
+        a[self.o.VH:] = 0
+        return a
+
+    def forward(self, h, keep=False):
+        """h: (V, 1, 4, DR) activation; returns P_e A E h (and layer inputs)."""
+        acts = [self._even(h)]
+        for W in self.W:
+            acts.append(self.o.ptc_layer(W, acts[-1]))
+        out = self._even(acts[-1])
+        return (out, acts[:-1]) if keep else out
+
+    def adjoint(self, g, keep=False):
+        """P_e A^+ E g; with keep, the cotangents entering each layer output."""
+        cot = [self._even(g)]
+        for W in reversed(self.W):
+            cot.append(self.o.ptc_layer(W, cot[-1], adjoint=True))
+        out = self._even(cot[-1])
+        # cot[k] enters layer n-1-k; reorder so cots[l] is layer l's output cotangent
+        return (out, cot[:-1][::-1]) if keep else out
+
+    def precondition(self, r):
+        return self.forward(self.adjoint(r))
+
+    def wgrad(self, w, y):
+        """Gradient of Re<y, A w> for every layer."""
+        _, acts = self.forward(w, keep=True)
+        _, cots = self.adjoint(y, keep=True)
+        return [self.o.ptc_wgrad(c, a) for c, a in zip(cots, acts)]
+
+    def loss_and_grad(self, b, x):
+        """b, x: activations (V, 1, 4, DR), even sites."""
+        u = self.adjoint(b)
+        f = self.forward(u)
+        nx = np.vdot(x, x).real
+        d = f - x
+        loss = np.vdot(d, d).real / nx
+        e = 2.0 * d / nx
+        v = self.adjoint(e)
+        g1 = self.wgrad(u, e)
+        g2 = self.wgrad(v, b)
+        return loss, [a + c for a, c in zip(g1, g2)]

@@ -1567,3 +1567,106 @@ void or_reunitarize(void)
 			gstore(l, reunitarize_g(gload(l)));
 		}
 }
+
+/* ------------------------------------------------------------------ */
+/* Gauge-covariant parallel-transport convolution layer (no counterpart */
+/* in the reference: the learned preconditioner).                        */
+/*   (L h)_co(x) = sum_p sum_ci W_p[co][ci] (T_p h_ci)(x)               */
+/* W_p[co][ci] is a complex 4x4 spin matrix; paths p = 0 identity,       */
+/* 1+mu: U(x,mu) h(x+mu), 5+mu: U(x-mu,mu)^+ h(x-mu), with the           */
+/* represented links (boundary sign included) the Dirac operator uses.   */
+/* Activations: [V][C][4 spins][DR colours] complex (re, im doubles).    */
+/* Weights: [9][cout][cin][4][4] complex.                                */
+/*   adjoint: (L^+ g)_ci(x) = sum_p sum_co W_p[co][ci]^H (T_pbar g_co)(x) */
+/*   wgrad:   G_p[co][ci][s][s'] = sum_{x,a} g_co(x)[s][a]                */
+/*                                  conj((T_p h_ci)(x)[s'][a])            */
+/* ------------------------------------------------------------------ */
+static void ptc_transport(int p, int x, const cplx *h, int c, int nch, cplx out[4][DR])
+{
+	const cplx *src;
+	cplx U[DR][DR];
+	if (p == 0) {
+		src = h + ((size_t)x * nch + c) * 4 * DR;
+		for (int s = 0; s < 4; s++)
+			for (int a = 0; a < DR; a++) out[s][a] = src[s * DR + a];
+		return;
+	}
+	int mu = (p - 1) & 3, fw = p <= 4;
+	int y = fw ? nb_fw[4 * x + mu] : nb_bk[4 * x + mu];
+	const double *u = grepr + (4 * (size_t)(fw ? x : y) + mu) * NLF;
+	for (int i = 0; i < DR; i++)
+		for (int j = 0; j < DR; j++) U[i][j] = fw ? fget(u, i, j) : conj(fget(u, j, i));
+	src = h + ((size_t)y * nch + c) * 4 * DR;
+	for (int s = 0; s < 4; s++)
+		for (int a = 0; a < DR; a++) {
+			cplx acc = 0;
+			for (int b = 0; b < DR; b++) acc += U[a][b] * src[s * DR + b];
+			out[s][a] = acc;
+		}
+}
+
+static inline int ptc_bar(int p) { return p == 0 ? 0 : (p <= 4 ? p + 4 : p - 4); }
+
+void or_ptc_layer(int cin, int cout, const double *Wd, const double *ind, double *outd, int adjoint)
+{
+	const cplx *W = (const cplx *)Wd, *in = (const cplx *)ind;
+	cplx *out = (cplx *)outd;
+	int nin = adjoint ? cout : cin, nout = adjoint ? cin : cout;
+#pragma omp parallel for
+	for (int x = 0; x < V; x++) {
+		cplx acc[16][4][DR];
+		for (int o = 0; o < nout; o++)
+			for (int s = 0; s < 4; s++)
+				for (int a = 0; a < DR; a++) acc[o][s][a] = 0;
+		for (int p = 0; p < 9; p++)
+			for (int i = 0; i < nin; i++) {
+				cplx t[4][DR];
+				ptc_transport(adjoint ? ptc_bar(p) : p, x, in, i, nin, t);
+				for (int o = 0; o < nout; o++) {
+					/* forward: W_p[o][i]; adjoint: W_p[i][o]^H */
+					const cplx *w = adjoint ? W + (((size_t)p * cout + i) * cin + o) * 16
+					                        : W + (((size_t)p * cout + o) * cin + i) * 16;
+					for (int s = 0; s < 4; s++)
+						for (int q = 0; q < 4; q++) {
+							cplx ws = adjoint ? conj(w[q * 4 + s]) : w[s * 4 + q];
+							for (int a = 0; a < DR; a++) acc[o][s][a] += ws * t[q][a];
+						}
+				}
+			}
+		for (int o = 0; o < nout; o++)
+			for (int s = 0; s < 4; s++)
+				for (int a = 0; a < DR; a++) out[(((size_t)x * nout + o) * 4 + s) * DR + a] = acc[o][s][a];
+	}
+}
+
+void or_ptc_wgrad(int cin, int cout, const double *gd, const double *hd, double *gradd)
+{
+	const cplx *g = (const cplx *)gd, *h = (const cplx *)hd;
+	cplx *grad = (cplx *)gradd;
+	size_t nw = (size_t)9 * cout * cin * 16;
+	for (size_t k = 0; k < nw; k++) grad[k] = 0;
+#pragma omp parallel
+	{
+		cplx *loc = calloc(nw, sizeof(cplx));
+#pragma omp for
+		for (int x = 0; x < V; x++)
+			for (int p = 0; p < 9; p++)
+				for (int i = 0; i < cin; i++) {
+					cplx t[4][DR];
+					ptc_transport(p, x, h, i, cin, t);
+					for (int o = 0; o < cout; o++) {
+						const cplx *go = g + ((size_t)x * cout + o) * 4 * DR;
+						cplx *gw = loc + (((size_t)p * cout + o) * cin + i) * 16;
+						for (int s = 0; s < 4; s++)
+							for (int q = 0; q < 4; q++) {
+								cplx v = 0;
+								for (int a = 0; a < DR; a++) v += go[s * DR + a] * conj(t[q][a]);
+								gw[s * 4 + q] += v;
+							}
+					}
+				}
+#pragma omp critical
+		for (size_t k = 0; k < nw; k++) grad[k] += loc[k];
+		free(loc);
+	}
+}
