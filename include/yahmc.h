@@ -220,6 +220,60 @@ int yh_wilson_fermion_force_q(yh_ctx* ctx, double dt, double mass, double residu
 /* clover_logdet_force (forces.cpp:109-119) */
 int yh_clover_logdet_force(yh_ctx* ctx, double mass, double residue);
 
+/* --- gauge molecular dynamics (FP64, periodic gauge boundaries: the
+ * periodic and antiperiodic fermion BCs; open / SF return UNSUPPORTED) ----- */
+/* the fundamental links (gfield_fund layout) as they are on the device */
+int yh_get_gauge(yh_ctx* ctx, double* links);
+/* represent_gauge_field (repr.cpp:265-280): rebuild the represented links
+ * and the clover term from the device fundamental links */
+int yh_represent_gauge_field(yh_ctx* ctx);
+/* wilson_gauge_force (forces.cpp:262-283): mom -= proj(U staple, dt beta/NC) */
+int yh_wilson_gauge_force(yh_ctx* ctx, double dt, double beta);
+/* update_links (update.cpp:228-247): U <- exp(eps P) U; the represented
+ * links go stale until yh_represent_gauge_field, as set_gauge_field_state(true)
+ * marks them in the reference */
+int yh_update_links(yh_ctx* ctx, double eps);
+/* reunitarize (update.cpp:309-320) */
+int yh_reunitarize(yh_ctx* ctx);
+/* avr_wilson(1,1) (wilsonloop.cpp:116-133): average plaquette */
+int yh_plaquette(yh_ctx* ctx, double* out);
+/* momentum part of hamiltonian (update.cpp:201-216): (TF/2) sum |P|^2 */
+int yh_momenta_energy(yh_ctx* ctx, double* out);
+
+/* --- learned preconditioner (no counterpart in the reference) --------------
+ * A gauge-covariant parallel-transport convolution network on the even
+ * sublattice, A = P_e L_n ... L_1 E with
+ *   (L h)_o(x) = sum_q sum_i W_q[o][i] (T_q h_i)(x),
+ * q = 0 identity, 1+mu: U(x,mu) h(x+mu), 5+mu: U(x-mu,mu)^+ h(x-mu) (the
+ * Dirac operator's links), W_q[o][i] complex 4x4 spin matrices.  Weights of
+ * layer l: [9][channels[l+1]][channels[l]][4][4] complex (re, im doubles),
+ * layers concatenated.  The preconditioner is M = A A^+.  SU(3)
+ * fundamental only; channels[0] = channels[n] = 1, hidden 4 or 8. */
+typedef struct yh_net yh_net;
+int yh_net_create(yh_ctx* ctx, int nlayers, const int* channels, yh_net** out);
+int yh_net_destroy(yh_net* net);
+long yh_net_num_weights(const yh_net* net);
+int yh_net_set_weights(yh_net* net, const double* w);
+int yh_net_get_weights(yh_net* net, double* w);
+/* dst = A src (adjoint 0) or A^+ src on the even halves */
+int yh_net_apply(yh_net* net, int adjoint, yh_field* dst, const yh_field* src);
+/* dst = A A^+ src */
+int yh_net_precondition(yh_net* net, yh_field* dst, const yh_field* src);
+/* sum over pairs of |A A^+ b_k - x_k|^2 / |x_k|^2 and its gradient
+ * d/dRe W + i d/dIm W (grad optional: nw complex) */
+int yh_net_loss_grad(yh_net* net, int npairs, const yh_field* const* b, const yh_field* const* x, double* loss,
+                     double* grad);
+/* one Adam step with the mean gradient of the last yh_net_loss_grad */
+int yh_net_adam_step(yh_net* net, double lr, double beta1, double beta2, double eps, int npairs);
+/* single-layer access on host arrays [V][C][4][3] complex (testing) */
+int yh_net_layer_host(yh_net* net, int layer, int adjoint, const double* in, double* out);
+int yh_net_wgrad_host(yh_net* net, int layer, const double* g, const double* h, double* grad);
+/* flexible CG on Hphi_sq in FP32, preconditioned by A A^+ (net may be NULL:
+ * plain CG through the same path), inside FP64 reliable updates; stops on
+ * the FP64 true residual |b - H^2 x|^2 < prec |b|^2 */
+int yh_cg_net(yh_ctx* ctx, const yh_inv_par* par, yh_net* net, yh_field* dst, const yh_field* src, int* iters,
+              int* updates);
+
 /* --- measurement support ---------------------------------------------------- */
 /* time `reps` back-to-back hop launches (Dphi_eo shape) of the given
  * precision / recon on the context stream with HIP events; returns the

@@ -1,0 +1,112 @@
+"""Gauge molecular dynamics on the device against the oracle
+(library/forces.cpp:262-283, update.cpp:201-320, wilsonloop.cpp:116-133,
+integrator.cpp:42-63): plaquette, Wilson gauge force, link update,
+reunitarisation, momentum energy, a whole o2mn trajectory, and the
+represented operator after the links moved."""
+import numpy as np
+import pytest
+
+from fields import gaussian_spinor, random_gauge
+from oracle import BC_ANTIPERIODIC, BC_PERIODIC, Oracle
+
+pytestmark = pytest.mark.gpu
+yh = pytest.importorskip("yahmc_amd")
+
+MODELS = [("su3_fund", 3, "fundamental"), ("su2_fund", 2, "fundamental"), ("su4_asym", 4, "antisymmetric")]
+
+
+def setup(model, nc, rep, dims=(4, 4, 4, 8), seed=5, bc="antiperiodic", csw=1.0):
+    obc = BC_ANTIPERIODIC if bc == "antiperiodic" else BC_PERIODIC
+    o = Oracle(model).init(dims, bc=obc, csw=csw)
+    rng = np.random.default_rng(seed)
+    links = random_gauge(rng, o.V, nc)
+    o.set_gauge(links)
+    lat = yh.Lattice(dims, nc=nc, repr=rep, bc=bc, csw=csw)
+    lat.set_gauge(links)
+    return o, lat, rng
+
+
+@pytest.mark.parametrize("model,nc,rep", MODELS)
+def test_plaquette_and_force(model, nc, rep):
+    print(f"\n[case] plaquette/gauge force {model}", flush=True)
+    o, lat, rng = setup(model, nc, rep)
+    assert abs(lat.plaquette() - o.plaquette()) < 1e-14
+    P = rng.normal(size=(4 * o.V, nc * nc - 1))
+    o.set_momenta(P)
+    lat.set_momenta(P)
+    o.wilson_gauge_force(0.37, 5.6)
+    lat.wilson_gauge_force(0.37, 5.6)
+    got, want = lat.get_momenta(), o.get_momenta()
+    assert np.abs(got - want).max() < 1e-13 * np.abs(want).max()
+    assert abs(lat.momenta_energy() - o.kinetic()) < 1e-12 * o.kinetic()
+
+
+@pytest.mark.parametrize("model,nc,rep", MODELS)
+def test_update_links_and_reunitarize(model, nc, rep):
+    print(f"\n[case] update_links {model}", flush=True)
+    o, lat, rng = setup(model, nc, rep, seed=11)
+    P = rng.normal(size=(4 * o.V, nc * nc - 1))
+    o.set_momenta(P)
+    lat.set_momenta(P)
+    o.update_links(0.21)
+    lat.update_links(0.21)
+    assert np.abs(lat.get_gauge() - o.get_gauge()).max() < 1e-13
+    # perturb away from the group, then reunitarise on both sides
+    bent = o.get_gauge() * (1.0 + 1e-3 * rng.normal(size=(4 * o.V, 2 * nc * nc)))
+    o.set_gauge(bent)
+    lat.set_gauge(bent)
+    o.reunitarize()
+    lat.reunitarize()
+    assert np.abs(lat.get_gauge() - o.get_gauge()).max() < 1e-13
+
+
+def test_o2mn_trajectory_matches_oracle_and_operator_follows():
+    """A five-step o2mn gauge trajectory: dH and the final links agree with
+    the oracle; after represent_gauge_field the device Dirac operator is the
+    oracle's on the moved links."""
+    print("\n[case] o2mn trajectory su3_fund", flush=True)
+    o, lat, rng = setup("su3_fund", 3, "fundamental", seed=7)
+    beta, mass = 5.6, -0.3
+    P = rng.normal(size=(4 * o.V, 8))
+    o.set_momenta(P)
+    lat.set_momenta(P)
+
+    def ham(x):
+        return x.momenta_energy() - beta * x.plaquette() * 6 * o.V if x is lat else \
+            x.kinetic() + x.gauge_action(beta)
+
+    h0d, h0o = ham(lat), ham(o)
+    assert abs(h0d - h0o) < 1e-10 * abs(h0o)
+    a, steps, length = 0.1931833275037836, 5, 0.5
+    dt = length / steps
+    for x in (o, lat):
+        for n in range(steps):
+            x.wilson_gauge_force(a * dt if n == 0 else 2 * a * dt, beta)
+            x.update_links(dt / 2)
+            x.wilson_gauge_force((1 - 2 * a) * dt, beta)
+            x.update_links(dt / 2)
+        x.wilson_gauge_force(a * dt, beta)
+    dHd, dHo = ham(lat) - h0d, ham(o) - h0o
+    print(f"dH device {dHd:.6e} oracle {dHo:.6e}")
+    assert abs(dHd - dHo) < 1e-9 * abs(h0o)
+    assert np.abs(lat.get_gauge() - o.get_gauge()).max() < 1e-12
+    assert np.abs(lat.get_momenta() - o.get_momenta()).max() < 1e-11
+    # the operator follows the moved links
+    lat.represent_gauge_field()
+    o.represent()
+    s = gaussian_spinor(rng, o.V, o.dr)
+    want = np.zeros_like(s)
+    o.Dphi(mass, want, s.copy())
+    fs, fd = lat.field("f64").upload(s), lat.field("f64")
+    lat.Dphi(mass, fd, fs)
+    got = fd.download()
+    assert np.abs(got[: o.VH] - want[: o.VH]).max() < 1e-12 * np.abs(want).max()
+
+
+def test_open_boundaries_rejected():
+    lat = yh.Lattice((4, 4, 4, 4), nc=3, repr="fundamental", bc="open", csw=0.0)
+    rng = np.random.default_rng(1)
+    lat.set_gauge(random_gauge(rng, lat.V, 3))
+    with pytest.raises(yh.YahmcError) as e:
+        lat.plaquette()
+    assert e.value.code == -6

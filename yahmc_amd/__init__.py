@@ -114,6 +114,13 @@ _SIGS = {
     "yh_set_momenta": ([_VP, _D], ctypes.c_int),
     "yh_get_momenta": ([_VP, _D], ctypes.c_int),
     "yh_get_clover_force": ([_VP, _D], ctypes.c_int),
+    "yh_get_gauge": ([_VP, _D], ctypes.c_int),
+    "yh_represent_gauge_field": ([_VP], ctypes.c_int),
+    "yh_wilson_gauge_force": ([_VP, ctypes.c_double, ctypes.c_double], ctypes.c_int),
+    "yh_update_links": ([_VP, ctypes.c_double], ctypes.c_int),
+    "yh_reunitarize": ([_VP], ctypes.c_int),
+    "yh_plaquette": ([_VP, _D], ctypes.c_int),
+    "yh_momenta_energy": ([_VP, _D], ctypes.c_int),
     "yh_begin_fermion_force": ([_VP], ctypes.c_int),
     "yh_end_fermion_force": ([_VP, ctypes.c_double], ctypes.c_int),
     "yh_clover_fermion_force": ([_VP, ctypes.c_double, _VP, _VP], ctypes.c_int),
@@ -123,6 +130,19 @@ _SIGS = {
     "yh_bench_hop": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP, _VP, _D], ctypes.c_int),
     "yh_bench_copy": ([_VP, ctypes.c_long, ctypes.c_int, _D], ctypes.c_int),
     "yh_synchronize": ([_VP], ctypes.c_int),
+    "yh_net_create": ([_VP, ctypes.c_int, _PI, ctypes.POINTER(_VP)], ctypes.c_int),
+    "yh_net_destroy": ([_VP], ctypes.c_int),
+    "yh_net_num_weights": ([_VP], ctypes.c_long),
+    "yh_net_set_weights": ([_VP, _D], ctypes.c_int),
+    "yh_net_get_weights": ([_VP, _D], ctypes.c_int),
+    "yh_net_apply": ([_VP, ctypes.c_int, _VP, _VP], ctypes.c_int),
+    "yh_net_precondition": ([_VP, _VP, _VP], ctypes.c_int),
+    "yh_net_loss_grad": ([_VP, ctypes.c_int, ctypes.POINTER(_VP), ctypes.POINTER(_VP), _D, _D], ctypes.c_int),
+    "yh_net_adam_step": ([_VP, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int],
+                         ctypes.c_int),
+    "yh_net_layer_host": ([_VP, ctypes.c_int, ctypes.c_int, _D, _D], ctypes.c_int),
+    "yh_net_wgrad_host": ([_VP, ctypes.c_int, _D, _D, _D], ctypes.c_int),
+    "yh_cg_net": ([_VP, ctypes.c_void_p, _VP, _VP, _VP, _PI, _PI], ctypes.c_int),
     "yh_tune": ([_VP, ctypes.c_int, ctypes.c_int], ctypes.c_int),
 }
 
@@ -379,6 +399,15 @@ class Lattice:
         _check(lib().yh_cg_mixed(self.ctx, ctypes.byref(par), d.h, s.h, ctypes.byref(it), ctypes.byref(up)))
         return it.value, up.value
 
+    def cg_net(self, par, net, d, s):
+        """Flexible CG on Hphi_sq preconditioned by net (None: plain CG
+        through the same FP32 / FP64 reliable-update path)."""
+        it = ctypes.c_int(0)
+        up = ctypes.c_int(0)
+        _check(lib().yh_cg_net(self.ctx, ctypes.byref(par), net.h if net is not None else None, d.h, s.h,
+                               ctypes.byref(it), ctypes.byref(up)))
+        return it.value, up.value
+
     # ---- forces (include/forces.h) ------------------------------------
     @property
     def ng(self):
@@ -417,6 +446,34 @@ class Lattice:
     def clover_logdet_force(self, mass, residue):
         _check(lib().yh_clover_logdet_force(self.ctx, mass, residue))
 
+    # ---- gauge molecular dynamics (forces.cpp:262-283, update.cpp:201-320) --
+    def get_gauge(self):
+        out = np.zeros((4 * self.V, 2 * self.nc ** 2))
+        _check(lib().yh_get_gauge(self.ctx, _dp(out)))
+        return out
+
+    def represent_gauge_field(self):
+        _check(lib().yh_represent_gauge_field(self.ctx))
+
+    def wilson_gauge_force(self, dt, beta):
+        _check(lib().yh_wilson_gauge_force(self.ctx, dt, beta))
+
+    def update_links(self, eps):
+        _check(lib().yh_update_links(self.ctx, eps))
+
+    def reunitarize(self):
+        _check(lib().yh_reunitarize(self.ctx))
+
+    def plaquette(self):
+        out = np.zeros(1)
+        _check(lib().yh_plaquette(self.ctx, _dp(out)))
+        return float(out[0])
+
+    def momenta_energy(self):
+        out = np.zeros(1)
+        _check(lib().yh_momenta_energy(self.ctx, _dp(out)))
+        return float(out[0])
+
     # ---- measurement ---------------------------------------------------
     def bench_hop(self, prec, recon, reps, d, s):
         out = np.zeros(1)
@@ -434,3 +491,113 @@ class Lattice:
 
     def tune(self, key, value):
         _check(lib().yh_tune(self.ctx, key, value))
+
+
+class Net:
+    """The learned preconditioner: a gauge-covariant parallel-transport
+    convolution network A on the even sublattice; M = A A^+."""
+
+    def __init__(self, lat, channels=(1, 8, 8, 1)):
+        self.lat = lat
+        self.channels = tuple(int(c) for c in channels)
+        ch = (ctypes.c_int * len(self.channels))(*self.channels)
+        h = _VP()
+        _check(lib().yh_net_create(lat.ctx, len(self.channels) - 1, ch, ctypes.byref(h)))
+        self.h = h
+        self.nw = lib().yh_net_num_weights(h)
+
+    def shapes(self):
+        c = self.channels
+        return [(9, co, ci, 4, 4) for ci, co in zip(c[:-1], c[1:])]
+
+    def _split(self, flat):
+        out, k = [], 0
+        for shp in self.shapes():
+            n = int(np.prod(shp))
+            out.append(flat[k:k + n].reshape(shp))
+            k += n
+        return out
+
+    def set_weights(self, Ws):
+        flat = np.concatenate([np.asarray(W, dtype=np.complex128).reshape(-1) for W in Ws])
+        assert flat.size == self.nw
+        buf = np.ascontiguousarray(flat).view(np.float64)
+        _check(lib().yh_net_set_weights(self.h, _dp(buf)))
+
+    def get_weights(self):
+        buf = np.zeros(2 * self.nw)
+        _check(lib().yh_net_get_weights(self.h, _dp(buf)))
+        return self._split(buf.view(np.complex128))
+
+    def init_weights(self, rng, scale=0.1, identity=0.5):
+        """Small random weights plus `identity` on the path-0 channel diagonal."""
+        Ws = []
+        for shp in self.shapes():
+            W = scale * (rng.normal(size=shp) + 1j * rng.normal(size=shp)) / np.sqrt(9 * shp[2] * 4)
+            for k in range(min(shp[1], shp[2])):
+                W[0, k, k] += identity * np.eye(4)
+            Ws.append(W)
+        self.set_weights(Ws)
+        return Ws
+
+    def apply(self, d, s, adjoint=False):
+        _check(lib().yh_net_apply(self.h, int(adjoint), d.h, s.h))
+
+    def precondition(self, d, s):
+        _check(lib().yh_net_precondition(self.h, d.h, s.h))
+
+    def loss_grad(self, bs, xs, want_grad=True):
+        n = len(bs)
+        B = (_VP * n)(*[b.h for b in bs])
+        X = (_VP * n)(*[x.h for x in xs])
+        loss = np.zeros(1)
+        g = np.zeros(2 * self.nw) if want_grad else None
+        _check(lib().yh_net_loss_grad(self.h, n, B, X, _dp(loss), _dp(g) if want_grad else None))
+        return float(loss[0]), (self._split(g.view(np.complex128)) if want_grad else None)
+
+    def adam_step(self, lr, npairs, beta1=0.9, beta2=0.999, eps=1e-8):
+        _check(lib().yh_net_adam_step(self.h, lr, beta1, beta2, eps, npairs))
+
+    def train(self, bs, xs, steps, batch=4, lr=2e-3, rng=None, log=None):
+        """Adam on the device gradient of |A A^+ b - x|^2 / |x|^2 over
+        minibatches of (source, solution) pairs; returns the loss history."""
+        rng = rng or np.random.default_rng(0)
+        hist = []
+        for step in range(steps):
+            idx = rng.choice(len(bs), size=min(batch, len(bs)), replace=False)
+            loss, _ = self.loss_grad([bs[i] for i in idx], [xs[i] for i in idx], want_grad=False)
+            self.adam_step(lr, len(idx))
+            hist.append(loss / len(idx))
+            if log and (step + 1) % log == 0:
+                print(f"[net] step {step + 1} loss {hist[-1]:.4f}", flush=True)
+        return hist
+
+    def layer_host(self, layer, h, adjoint=False):
+        """One layer on an activation (V, C, 4, 3) complex (testing)."""
+        ci, co = self.channels[layer], self.channels[layer + 1]
+        nout = ci if adjoint else co
+        h = np.ascontiguousarray(h, dtype=np.complex128)
+        out = np.zeros((self.lat.V, nout, 4, 3), dtype=np.complex128)
+        _check(lib().yh_net_layer_host(self.h, layer, int(adjoint), _dp(h.view(np.float64)),
+                                       _dp(out.view(np.float64))))
+        return out
+
+    def wgrad_host(self, layer, g, h):
+        ci, co = self.channels[layer], self.channels[layer + 1]
+        g = np.ascontiguousarray(g, dtype=np.complex128)
+        h = np.ascontiguousarray(h, dtype=np.complex128)
+        out = np.zeros((9, co, ci, 4, 4), dtype=np.complex128)
+        _check(lib().yh_net_wgrad_host(self.h, layer, _dp(g.view(np.float64)), _dp(h.view(np.float64)),
+                                       _dp(out.view(np.float64))))
+        return out
+
+    def free(self):
+        if self.h:
+            lib().yh_net_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

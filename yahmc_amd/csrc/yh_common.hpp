@@ -104,6 +104,16 @@ __device__ __forceinline__ int half_index(int t, int x, int y, int z, const Geom
   return (((t * g.L[1] + x) * g.L[2] + y) * g.L[3] + z) >> 1;
 }
 
+// full-lattice EO index (parity * VH + half index) and its inverse
+__device__ __forceinline__ int full_index(int t, int x, int y, int z, const Geom& g) {
+  return ((t + x + y + z) & 1) * g.VH + half_index(t, x, y, z, g);
+}
+
+__device__ __forceinline__ void full_coords(int id, const Geom& g, int c[4]) {
+  int p = id >= g.VH;
+  site_coords(id - p * g.VH, p, g, c);
+}
+
 // true where the boundary conditions force spinors to zero
 // (library/bcs.cpp:139-164: open t in {0, T-1}; SF t in {0, T-2, T-1})
 __device__ __forceinline__ bool bc_zero_slice(int t, const Geom& g) {

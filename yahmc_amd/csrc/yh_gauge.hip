@@ -194,15 +194,7 @@ __global__ void k_represent(const double* __restrict__ fund, double2* __restrict
     out[((size_t)(p * 4 + mu) * D * D + e) * g.VHpad + k] = make_double2(R[e / D][e % D].re, R[e / D][e % D].im);
 }
 
-__device__ __forceinline__ int full_index(int t, int x, int y, int z, const Geom& g) {
-  int par = (t + x + y + z) & 1;
-  return par * g.VH + half_index(t, x, y, z, g);
-}
-
-__device__ __forceinline__ void coords_of(int id, const Geom& g, int c[4]) {
-  int p = id >= g.VH;
-  site_coords(id - p * g.VH, p, g, c);
-}
+__device__ __forceinline__ void coords_of(int id, const Geom& g, int c[4]) { full_coords(id, g, c); }
 
 __device__ __forceinline__ int shift(const int c[4], int mu, int d, const Geom& g) {
   int n[4] = {c[0], c[1], c[2], c[3]};

@@ -143,6 +143,18 @@ int momenta_upload(Ctx* c, const double* host);
 int momenta_download(Ctx* c, double* host);
 int clover_force_download(Ctx* c, double* host);
 
+// ---- gauge molecular dynamics ----------------------------------------------
+int md_init_constants(Ctx* c);
+int md_gauge_force(Ctx* c, double dt, double beta);
+int md_update_links(Ctx* c, double eps);
+int md_reunitarize(Ctx* c);
+int md_plaquette(Ctx* c, double* out);
+int md_kinetic(Ctx* c, double* out);
+
+// ---- learned preconditioner ---------------------------------------------------
+struct Net;
+int net_precondition(Net* n, Field* dst, const Field* src);
+
 // ---- host helpers shared by the solver translation units -------------------
 Field* tmp_field(Ctx* c, int prec, int slot, int* rc);
 int default_recon(Ctx* c, int prec);

@@ -1,0 +1,464 @@
+// Gauge molecular dynamics on the device (FP64): Wilson gauge force, link
+// update, reunitarisation, plaquette and kinetic energy.
+//
+// wilson_gauge_force : library/forces.cpp:262-283 + staples.cpp:82-107
+// update_links       : library/update.cpp:228-247 + suNa.cpp:113-134 (exp)
+// reunitarize        : library/suNg.cpp:171-359
+// plaquette          : library/wilsonloop.cpp:58-133 (avr_wilson(1,1))
+// hamiltonian (P^2)  : library/update.cpp:201-216
+// Fundamental links stay in the reference layout (4*id+mu, re plane then im
+// plane); momenta are SoA mom[(mu * NG + a) * V + id].  Periodic gauge
+// boundaries (the plaquette weights of open / SF are not applied here).
+#include <algorithm>
+#include <vector>
+#include "yh_internal.hpp"
+#include "yh_reduce.hpp"
+
+namespace yh {
+
+typedef Cx<double> cd;
+
+template <int NC>
+struct M {
+  cd m[NC][NC];
+};
+
+template <int NC>
+__device__ __forceinline__ M<NC> ld(const double* fund, int id, int mu) {
+  M<NC> u;
+  const double* p = fund + (size_t)(4 * id + mu) * 2 * NC * NC;
+#pragma unroll
+  for (int i = 0; i < NC; i++)
+#pragma unroll
+    for (int j = 0; j < NC; j++) u.m[i][j] = cd(p[i * NC + j], p[NC * NC + i * NC + j]);
+  return u;
+}
+
+template <int NC>
+__device__ __forceinline__ void st(double* fund, int id, int mu, const M<NC>& u) {
+  double* p = fund + (size_t)(4 * id + mu) * 2 * NC * NC;
+#pragma unroll
+  for (int i = 0; i < NC; i++)
+#pragma unroll
+    for (int j = 0; j < NC; j++) {
+      p[i * NC + j] = u.m[i][j].re;
+      p[NC * NC + i * NC + j] = u.m[i][j].im;
+    }
+}
+
+template <int NC>
+__device__ __forceinline__ M<NC> mul(const M<NC>& a, const M<NC>& b) {
+  M<NC> r;
+#pragma unroll
+  for (int i = 0; i < NC; i++)
+#pragma unroll
+    for (int j = 0; j < NC; j++) {
+      cd s(0, 0);
+#pragma unroll
+      for (int k = 0; k < NC; k++) cmac(s, a.m[i][k], b.m[k][j]);
+      r.m[i][j] = s;
+    }
+  return r;
+}
+
+// a * b^+
+template <int NC>
+__device__ __forceinline__ M<NC> muld(const M<NC>& a, const M<NC>& b) {
+  M<NC> r;
+#pragma unroll
+  for (int i = 0; i < NC; i++)
+#pragma unroll
+    for (int j = 0; j < NC; j++) {
+      cd s(0, 0);
+#pragma unroll
+      for (int k = 0; k < NC; k++) cmac(s, a.m[i][k], conj(b.m[j][k]));
+      r.m[i][j] = s;
+    }
+  return r;
+}
+
+// a^+ * b
+template <int NC>
+__device__ __forceinline__ M<NC> dmul(const M<NC>& a, const M<NC>& b) {
+  M<NC> r;
+#pragma unroll
+  for (int i = 0; i < NC; i++)
+#pragma unroll
+    for (int j = 0; j < NC; j++) {
+      cd s(0, 0);
+#pragma unroll
+      for (int k = 0; k < NC; k++) cmac_conj(s, a.m[k][i], b.m[k][j]);
+      r.m[i][j] = s;
+    }
+  return r;
+}
+
+__device__ __forceinline__ int mshift(const int c[4], int mu, int d, const Geom& g) {
+  int n[4] = {c[0], c[1], c[2], c[3]};
+  n[mu] = (n[mu] + d + g.L[mu]) % g.L[mu];
+  return full_index(n[0], n[1], n[2], n[3], g);
+}
+
+// anti-Hermitian matrix of an algebra vector (library/suNg.cpp:15-58)
+template <int NC>
+__device__ __forceinline__ M<NC> alg(const double* mom, int V, int mu, int id) {
+  constexpr int NG = NC * NC - 1;
+  M<NC> r;
+#pragma unroll
+  for (int i = 0; i < NC; i++)
+#pragma unroll
+    for (int j = 0; j < NC; j++) r.m[i][j] = cd(0, 0);
+  int n = 0;
+#pragma unroll
+  for (int i = 0; i < NC; i++)
+#pragma unroll
+    for (int j = 0; j < NC; j++) {
+      if (i > j) {
+        double v = mom[(size_t)(mu * NG + n) * V + id];
+        r.m[i][j].im += v;
+        r.m[j][i].im += v;
+        n++;
+      } else if (i < j) {
+        double v = mom[(size_t)(mu * NG + n) * V + id];
+        r.m[i][j].re += v;
+        r.m[j][i].re -= v;
+        n++;
+      } else if (i > 0) {
+        double v = mom[(size_t)(mu * NG + n) * V + id];
+        double w = sqrt(2.0 / (i * (i + 1)));
+        for (int k = 0; k < i; k++) r.m[k][k].im += w * v;
+        r.m[i][i].im -= i * w * v;
+        n++;
+      }
+    }
+  return r;
+}
+
+// row-by-row unitarisation (library/suNg.cpp:171-359)
+template <int NC>
+__device__ __forceinline__ void reunit(M<NC>& u) {
+  double n0 = 0;
+#pragma unroll
+  for (int j = 0; j < NC; j++) n0 += u.m[0][j].re * u.m[0][j].re + u.m[0][j].im * u.m[0][j].im;
+  n0 = 1.0 / sqrt(n0);
+#pragma unroll
+  for (int j = 0; j < NC; j++) u.m[0][j] = n0 * u.m[0][j];
+  if constexpr (NC == 2) {
+    u.m[1][0] = cd(-u.m[0][1].re, u.m[0][1].im);
+    u.m[1][1] = conj(u.m[0][0]);
+  } else {
+#pragma unroll
+    for (int i = 1; i < NC; i++) {
+      if (NC == 3 && i == 2) {
+        u.m[2][0] = conj(u.m[0][1] * u.m[1][2] - u.m[0][2] * u.m[1][1]);
+        u.m[2][1] = conj(u.m[0][2] * u.m[1][0] - u.m[0][0] * u.m[1][2]);
+        u.m[2][2] = conj(u.m[0][0] * u.m[1][1] - u.m[0][1] * u.m[1][0]);
+        break;
+      }
+#pragma unroll
+      for (int k = 0; k < i; k++) {
+        cd p(0, 0);
+#pragma unroll
+        for (int j = 0; j < NC; j++) cmac_conj(p, u.m[k][j], u.m[i][j]);
+#pragma unroll
+        for (int j = 0; j < NC; j++) u.m[i][j] = u.m[i][j] - p * u.m[k][j];
+      }
+      double n = 0;
+#pragma unroll
+      for (int j = 0; j < NC; j++) n += u.m[i][j].re * u.m[i][j].re + u.m[i][j].im * u.m[i][j].im;
+      n = 1.0 / sqrt(n);
+#pragma unroll
+      for (int j = 0; j < NC; j++) u.m[i][j] = n * u.m[i][j];
+    }
+  }
+}
+
+__constant__ double2 c_itf[15 * 16];  // fundamental generators iT_a, NC x NC
+
+// Wilson gauge force: mom -= algebra_project(U staple, dt beta / NC)
+template <int NC>
+__global__ void __launch_bounds__(128) k_gauge_force(const double* __restrict__ fund, double* __restrict__ mom,
+                                                     double coeff, Geom g) {
+  int t = blockIdx.x * 128 + threadIdx.x;
+  if (t >= 4 * g.V) return;
+  int id = t >> 2, mu = t & 3;
+  constexpr int NG = NC * NC - 1;
+  int c[4];
+  full_coords(id, g, c);
+  M<NC> S;
+#pragma unroll
+  for (int i = 0; i < NC; i++)
+#pragma unroll
+    for (int j = 0; j < NC; j++) S.m[i][j] = cd(0, 0);
+  int o1m = mshift(c, mu, 1, g);
+  for (int nu = 0; nu < 4; nu++) {
+    if (nu == mu) continue;
+    int o2 = mshift(c, nu, 1, g);
+    M<NC> a = mul<NC>(ld<NC>(fund, id, nu), ld<NC>(fund, o2, mu));
+    M<NC> s1 = muld<NC>(ld<NC>(fund, o1m, nu), a);
+    int b = mshift(c, nu, -1, g);
+    int cb[4];
+    full_coords(b, g, cb);
+    int b2 = mshift(cb, mu, 1, g);
+    M<NC> q = mul<NC>(ld<NC>(fund, b, mu), ld<NC>(fund, b2, nu));
+    M<NC> s2 = dmul<NC>(q, ld<NC>(fund, b, nu));
+#pragma unroll
+    for (int i = 0; i < NC; i++)
+#pragma unroll
+      for (int j = 0; j < NC; j++) S.m[i][j] = S.m[i][j] + s1.m[i][j] + s2.m[i][j];
+  }
+  M<NC> F = mul<NC>(ld<NC>(fund, id, mu), S);
+#pragma unroll
+  for (int a = 0; a < NG; a++) {
+    double re = 0;
+#pragma unroll
+    for (int i = 0; i < NC; i++)
+#pragma unroll
+      for (int k = 0; k < NC; k++) {
+        double2 tg = c_itf[a * 16 + i * NC + k];
+        re += tg.x * F.m[k][i].re - tg.y * F.m[k][i].im;
+      }
+    mom[(size_t)(mu * NG + a) * g.V + id] -= -(coeff * 0.5) * re;
+  }
+}
+
+// U <- exp(eps P) U, Taylor series to |X^k/k!|^2 < 1e-24 then reunitarised
+template <int NC>
+__global__ void __launch_bounds__(128) k_update_links(double* __restrict__ fund, const double* __restrict__ mom,
+                                                      double eps, Geom g) {
+  int t = blockIdx.x * 128 + threadIdx.x;
+  if (t >= 4 * g.V) return;
+  int id = t >> 2, mu = t & 3;
+  M<NC> X = alg<NC>(mom, g.V, mu, id);
+  M<NC> R, Xk;
+#pragma unroll
+  for (int i = 0; i < NC; i++)
+#pragma unroll
+    for (int j = 0; j < NC; j++) {
+      R.m[i][j] = cd(i == j ? 1.0 : 0.0, 0);
+      Xk.m[i][j] = R.m[i][j];
+    }
+  double err = 1.0;
+  for (int k = 1; k < 64 && err > 1e-24; k++) {
+    Xk = mul<NC>(Xk, X);
+    double f = eps / k;
+    err = 0;
+#pragma unroll
+    for (int i = 0; i < NC; i++)
+#pragma unroll
+      for (int j = 0; j < NC; j++) {
+        Xk.m[i][j] = f * Xk.m[i][j];
+        R.m[i][j] = R.m[i][j] + Xk.m[i][j];
+        err += Xk.m[i][j].re * Xk.m[i][j].re + Xk.m[i][j].im * Xk.m[i][j].im;
+      }
+  }
+  reunit<NC>(R);
+  st<NC>(fund, id, mu, mul<NC>(R, ld<NC>(fund, id, mu)));
+}
+
+template <int NC>
+__global__ void __launch_bounds__(128) k_reunitarize(double* __restrict__ fund, Geom g) {
+  int t = blockIdx.x * 128 + threadIdx.x;
+  if (t >= 4 * g.V) return;
+  M<NC> u = ld<NC>(fund, t >> 2, t & 3);
+  reunit<NC>(u);
+  st<NC>(fund, t >> 2, t & 3, u);
+}
+
+// sum over planes of Re Tr of the plaquette
+template <int NC>
+__global__ void __launch_bounds__(256) k_plaquette(const double* __restrict__ fund, double* partials, unsigned* ticket,
+                                                   double* out, Geom g) {
+  int id = blockIdx.x * 256 + threadIdx.x;
+  double v[1] = {0.0};
+  if (id < g.V) {
+    int c[4];
+    full_coords(id, g, c);
+    for (int mu = 1; mu < 4; mu++)
+      for (int nu = 0; nu < mu; nu++) {
+        int a = mshift(c, mu, 1, g), b = mshift(c, nu, 1, g);
+        M<NC> one = mul<NC>(ld<NC>(fund, id, mu), ld<NC>(fund, a, nu));
+        M<NC> two = mul<NC>(ld<NC>(fund, id, nu), ld<NC>(fund, b, mu));
+        M<NC> P = muld<NC>(one, two);
+#pragma unroll
+        for (int i = 0; i < NC; i++) v[0] += P.m[i][i].re;
+      }
+  }
+  block_reduce_finish<1, 256>(v, partials, gridDim.x, blockIdx.x, ticket, out);
+}
+
+__global__ void __launch_bounds__(256) k_sumsq(const double* __restrict__ a, long n, double* partials, unsigned* ticket,
+                                               double* out) {
+  long i0 = (long)blockIdx.x * 256 + threadIdx.x;
+  double v[1] = {0.0};
+  for (long i = i0; i < n; i += (long)gridDim.x * 256) v[0] += a[i] * a[i];
+  block_reduce_finish<1, 256>(v, partials, gridDim.x, blockIdx.x, ticket, out);
+}
+
+// ---------------------------------------------------------------------------
+int md_init_constants(Ctx* c) {
+  int nc = c->nc, ng = nc * nc - 1;
+  std::vector<double2> h(15 * 16, make_double2(0, 0));
+  for (int a = 0; a < ng; a++) {
+    int k = 0;
+    for (int i = 0; i < nc; i++)
+      for (int j = 0; j < nc; j++) {
+        if (i > j) {
+          if (k == a) { h[a * 16 + i * nc + j].y += 1; h[a * 16 + j * nc + i].y += 1; }
+          k++;
+        } else if (i < j) {
+          if (k == a) { h[a * 16 + i * nc + j].x += 1; h[a * 16 + j * nc + i].x -= 1; }
+          k++;
+        } else if (i > 0) {
+          if (k == a) {
+            double w = sqrt(2.0 / (i * (i + 1)));
+            for (int q = 0; q < i; q++) h[a * 16 + q * nc + q].y += w;
+            h[a * 16 + i * nc + i].y -= i * w;
+          }
+          k++;
+        }
+      }
+  }
+  YH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_itf), h.data(), sizeof(double2) * h.size()));
+  return YH_OK;
+}
+
+#define YH_NC_SWITCH(...)                                     \
+  switch (c->nc) {                                            \
+    case 2: { constexpr int NC_ = 2; __VA_ARGS__; } break;    \
+    case 3: { constexpr int NC_ = 3; __VA_ARGS__; } break;    \
+    case 4: { constexpr int NC_ = 4; __VA_ARGS__; } break;    \
+    default: set_error("unsupported NC"); return YH_ERR_UNSUPPORTED; \
+  }
+
+int md_gauge_force(Ctx* c, double dt, double beta) {
+  int rc;
+  if ((rc = force_alloc(c))) return rc;
+  double coeff = dt * beta / c->nc;
+  dim3 grid((4 * c->g.V + 127) / 128);
+  YH_NC_SWITCH(hipLaunchKernelGGL((k_gauge_force<NC_>), grid, dim3(128), 0, c->stream, c->fund, c->mom, coeff, c->g));
+  YH_HIP(hipGetLastError());
+  return YH_OK;
+}
+
+int md_update_links(Ctx* c, double eps) {
+  int rc;
+  if ((rc = force_alloc(c))) return rc;
+  dim3 grid((4 * c->g.V + 127) / 128);
+  YH_NC_SWITCH(hipLaunchKernelGGL((k_update_links<NC_>), grid, dim3(128), 0, c->stream, c->fund, c->mom, eps, c->g));
+  YH_HIP(hipGetLastError());
+  return YH_OK;
+}
+
+int md_reunitarize(Ctx* c) {
+  dim3 grid((4 * c->g.V + 127) / 128);
+  YH_NC_SWITCH(hipLaunchKernelGGL((k_reunitarize<NC_>), grid, dim3(128), 0, c->stream, c->fund, c->g));
+  YH_HIP(hipGetLastError());
+  return YH_OK;
+}
+
+int md_plaquette(Ctx* c, double* out) {
+  dim3 grid((c->g.V + 255) / 256);
+  double* dst = &c->sc->red[0];
+  YH_NC_SWITCH(hipLaunchKernelGGL((k_plaquette<NC_>), grid, dim3(256), 0, c->stream, c->fund, c->partials, c->ticket,
+                                  dst, c->g));
+  YH_HIP(hipGetLastError());
+  double v;
+  YH_HIP(hipMemcpyAsync(&v, dst, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  YH_HIP(hipStreamSynchronize(c->stream));
+  *out = v / (c->nc * 6.0 * c->g.V);
+  return YH_OK;
+}
+
+int md_kinetic(Ctx* c, double* out) {
+  int rc;
+  if ((rc = force_alloc(c))) return rc;
+  long n = 4L * c->g.V * (c->nc * c->nc - 1);
+  int grid = (int)std::min<long>((n + 255) / 256, (long)c->nblk);
+  double* dst = &c->sc->red[0];
+  hipLaunchKernelGGL(k_sumsq, dim3(grid), dim3(256), 0, c->stream, c->mom, n, c->partials, c->ticket, dst);
+  YH_HIP(hipGetLastError());
+  double v;
+  YH_HIP(hipMemcpyAsync(&v, dst, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  YH_HIP(hipStreamSynchronize(c->stream));
+  *out = v;  // TF / 2 = 1
+  return YH_OK;
+}
+
+}  // namespace yh
+
+// ---------------------------------------------------------------------------
+using namespace yh;
+
+static int md_check(Ctx* c) {
+  if (!c) { set_error("null argument"); return YH_ERR_ARG; }
+  if (!c->gauge_set) { set_error("no gauge field"); return YH_ERR_STATE; }
+  if (c->g.bc == YH_BC_OPEN || c->g.bc == YH_BC_SF) {
+    set_error("gauge MD is built for periodic gauge boundaries");
+    return YH_ERR_UNSUPPORTED;
+  }
+  YH_HIP(hipSetDevice(c->device));
+  return YH_OK;
+}
+
+static int md_sync(Ctx* c, int rc) {
+  if (rc) return rc;
+  YH_HIP(hipStreamSynchronize(c->stream));
+  return YH_OK;
+}
+
+extern "C" {
+
+int yh_get_gauge(yh_ctx* c, double* links) {
+  if (!c || !links) { set_error("null argument"); return YH_ERR_ARG; }
+  if (!c->gauge_set) { set_error("no gauge field"); return YH_ERR_STATE; }
+  YH_HIP(hipSetDevice(c->device));
+  size_t n = (size_t)4 * c->g.V * 2 * c->nc * c->nc;
+  YH_HIP(hipMemcpyAsync(links, c->fund, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  YH_HIP(hipStreamSynchronize(c->stream));
+  return YH_OK;
+}
+
+int yh_represent_gauge_field(yh_ctx* c) {
+  if (!c) { set_error("null argument"); return YH_ERR_ARG; }
+  if (!c->gauge_set) { set_error("no gauge field"); return YH_ERR_STATE; }
+  YH_HIP(hipSetDevice(c->device));
+  int rc;
+  if ((rc = gauge_represent(c))) return rc;
+  if ((rc = gauge_derive_links(c))) return rc;
+  if ((rc = clover_build(c))) return rc;
+  return md_sync(c, YH_OK);
+}
+
+int yh_wilson_gauge_force(yh_ctx* c, double dt, double beta) {
+  int rc = md_check(c);
+  if (rc) return rc;
+  return md_sync(c, md_gauge_force(c, dt, beta));
+}
+
+int yh_update_links(yh_ctx* c, double eps) {
+  int rc = md_check(c);
+  if (rc) return rc;
+  return md_sync(c, md_update_links(c, eps));
+}
+
+int yh_reunitarize(yh_ctx* c) {
+  int rc = md_check(c);
+  if (rc) return rc;
+  return md_sync(c, md_reunitarize(c));
+}
+
+int yh_plaquette(yh_ctx* c, double* out) {
+  if (!out) { set_error("null argument"); return YH_ERR_ARG; }
+  int rc = md_check(c);
+  if (rc) return rc;
+  return md_plaquette(c, out);
+}
+
+int yh_momenta_energy(yh_ctx* c, double* out) {
+  if (!c || !out) { set_error("null argument"); return YH_ERR_ARG; }
+  YH_HIP(hipSetDevice(c->device));
+  return md_kinetic(c, out);
+}
+
+}  // extern "C"

@@ -237,6 +237,7 @@ int mshift_device(Ctx* c, const yh_inv_par* par, int nrhs, int num, const double
     if ((rc = reduce_read(c, 0, b[j], nullptr, 0, nb))) break;
     MsState& S = hs[j];
     memset(&S, 0, sizeof(S));
+    S.rr = nb[0];  // |r|^2 of the starting residual r = b
     S.beta = 1.0;
     S.alpha = 0.0;
     S.norm = nb[0];
