@@ -7,9 +7,16 @@ lattice with all inputs resident in HBM.  `value` is the achieved HBM
 bandwidth of that kernel under the compulsory-traffic model (each input
 spinor, each link and each output spinor moves once per hop), measured with
 HIP events on the stream the kernel runs on.  The line also carries the
-other precisions, the CG time-to-solution (plain FP64 vs mixed FP32 / FP16
-reliable-update CG to an FP64 true residual of 1e-10), the roofline object
-and a CPU baseline (the oracle on the host cores).
+other precisions, the CG time-to-solution to an FP64 true residual of 1e-10
+(plain FP64 CG; mixed FP32 / FP16 reliable-update CG; flexible CG with the
+learned gauge-covariant preconditioner, trained on the device on (source,
+solution) pairs first), the roofline object and a CPU baseline (the oracle
+on the host cores).
+
+The gauge field is prepared as the 32^4 workload specifies: seeded cold
+start, then 10 pure-gauge o2mn trajectories (tau = 1, 10 steps) at
+beta = 5.6, every trajectory kept (MD-only: from the cold start the first
+trajectories have dH of O(V) and Metropolis would reject them).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -73,6 +80,62 @@ def cpu_baseline(threads, budget_s=12.0):
                       f"{dt * 1e3:.2f} ms each; same compulsory-bytes model as the GPU value"}
 
 
+def true_residual(lat, fx, fb):
+    ax = lat.field("f64")
+    lat.Hphi_sq(MASS, ax, fx)
+    lat.spinor_mulr_add_assign(ax, -1.0, fb)
+    res = lat.spinor_sqnorm(ax) / lat.spinor_sqnorm(fb)
+    ax.free()
+    return float(np.sqrt(res))
+
+
+def learned_solves(yh, lat, fb, prec, train_steps):
+    """Train the gauge-covariant network on (Gaussian source, solution)
+    pairs on the device, then time the flexible CG on H^2 with M = A A^+
+    (FP32 inner, FP64 reliable updates) against the same solver without it."""
+    from yahmc_amd.synthetic import gaussian_source
+    out = {}
+    t0 = time.perf_counter()
+    bs, xs = [], []
+    for k in range(8):
+        b = lat.field("f64").upload(gaussian_source(DIMS, 3, seed=32300 + k))
+        x = lat.field("f64")
+        lat.cg_mixed(lat.inv_par(MASS, 1e-16, "Hphi_sq", sloppy="f32", max_iter=20000), x, b)
+        bs.append(b)
+        xs.append(x)
+    t_pairs = time.perf_counter() - t0
+    net = yh.Net(lat, (1, 8, 8, 1))
+    rng = np.random.default_rng(32400)
+    net.init_weights(rng)
+    lat.synchronize()
+    t0 = time.perf_counter()
+    hist = net.train(bs, xs, steps=train_steps, batch=4, lr=2e-3, rng=rng)
+    lat.synchronize()
+    t_train = time.perf_counter() - t0
+    train = {"pairs": len(bs), "pair_seconds": round(t_pairs, 2), "steps": train_steps, "batch": 4,
+             "train_seconds": round(t_train, 2), "loss_first10": round(float(np.mean(hist[:10])), 4),
+             "loss_last10": round(float(np.mean(hist[-10:])), 4), "channels": [1, 8, 8, 1]}
+    log(f"network training: {train}")
+    for name, n in (("fcg_plain_f32", None), ("learned_f32", net)):
+        fx = lat.field("f64")
+        lat.synchronize()
+        t = time.perf_counter()
+        it, upd = lat.cg_net(lat.inv_par(MASS, prec, "Hphi_sq", max_iter=20000), n, fx, fb)
+        lat.synchronize()
+        dt = time.perf_counter() - t
+        out[name] = {"seconds": round(dt, 4), "iterations": it, "reliable_updates": upd,
+                     "true_rel_residual": true_residual(lat, fx, fb)}
+        log(f"cg {name}: {out[name]}")
+        fx.free()
+    out["learned_f32"]["training"] = train
+    out["learned_f32"]["iteration_ratio_vs_fcg_plain"] = round(out["fcg_plain_f32"]["iterations"] /
+                                                               max(1, out["learned_f32"]["iterations"]), 2)
+    net.free()
+    for f in bs + xs:
+        f.free()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,6 +145,11 @@ def main():
     ap.add_argument("--skip-cg", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--only-hop", action="store_true", help="time the headline hop only (for profiling)")
+    ap.add_argument("--no-smooth", action="store_true", help="keep the hot random field (no gauge trajectories)")
+    ap.add_argument("--skip-learned", action="store_true")
+    ap.add_argument("--train-steps", type=int, default=120)
+    ap.add_argument("--only-learned", action="store_true",
+                    help="smoothing, network training and the two flexible-CG solves only (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
@@ -98,10 +166,24 @@ def main():
     lat = yh.Lattice(DIMS, nc=3, repr="fundamental", bc="antiperiodic", csw=CSW)
     lat.set_gauge(links)
     log("gauge field, representation and clover term built on device")
+    smoothing = None
+    if not args.no_smooth and not args.only_hop:
+        from yahmc_amd.md import smooth
+        t0 = time.perf_counter()
+        traj = smooth(lat, beta=5.6, trajectories=10, length=1.0, steps=10, seed=32246)
+        smoothing = {"start": "cold", "beta": 5.6, "trajectories": 10, "tau": 1.0, "md_steps": 10,
+                     "integrator": "o2mn", "metropolis": False,
+                     "dH": [round(t["dH"], 3) for t in traj], "plaquette": [round(t["plaquette"], 5) for t in traj],
+                     "seconds": round(time.perf_counter() - t0, 2)}
+        log(f"smoothing: {smoothing}")
 
     hop = {}
     variants = [(args.prec, 0 if args.prec == "f64" else 1)] if args.only_hop else \
         [("f64", 0), ("f64", 1), ("f32", 1), ("f16", 1)]
+    if args.only_learned:
+        fb = lat.field("f64").upload(gaussian_source(DIMS, 3, seed=32246))
+        print(json.dumps(learned_solves(yh, lat, fb, 1e-20, args.train_steps)))
+        return
     for prec, recon in variants:
         fs = lat.field(prec).upload(src)
         fd = lat.field(prec)
@@ -136,17 +218,14 @@ def main():
             lat.synchronize()
             dt = time.perf_counter() - t
             # FP64 true residual with the FP64 GPU operator
-            ax = lat.field("f64")
-            lat.Hphi_sq(MASS, ax, fx)
-            lat.spinor_mulr_add_assign(ax, -1.0, fb)
-            res = lat.spinor_sqnorm(ax) / lat.spinor_sqnorm(fb)
             cg[name] = {"seconds": round(dt, 4), "iterations": it, "reliable_updates": upd,
-                        "true_rel_residual": float(np.sqrt(res))}
+                        "true_rel_residual": true_residual(lat, fx, fb)}
             log(f"cg {name}: {cg[name]}")
             fx.free()
-            ax.free()
+        if not args.skip_learned:
+            cg.update(learned_solves(yh, lat, fb, prec, args.train_steps))
         if "fp64" in cg:
-            for k in ("mixed_f32", "mixed_f16"):
+            for k in ("mixed_f32", "mixed_f16", "fcg_plain_f32", "learned_f32"):
                 if k in cg:
                     cg[k]["speedup_vs_fp64"] = round(cg["fp64"]["seconds"] / cg[k]["seconds"], 2)
 
@@ -170,11 +249,12 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"f64": "f64", "f32": "f32", "f16": "f16-storage/f32-arith"}[args.prec],
-        "data": "synthetic (seeded random SU(3) links, Gaussian source)",
+        "data": "synthetic (seeded SU(3) field smoothed by pure-gauge trajectories, Gaussian sources)"
+                if smoothing else "synthetic (seeded random SU(3) links, Gaussian source)",
         "config": {"workload": f"hop D_eo, {args.prec} recon-{18 if args.prec == 'f64' else 12}, 32^4 SU(3) fundamental "
                                "clover, antiperiodic time",
                    "lattice": list(DIMS), "sites_per_step": lat.VH, "mass": MASS, "csw": CSW,
-                   "gauge": "random SU(3) links (hot); the 10-trajectory pure-gauge smoothing is not applied"},
+                   "gauge": smoothing if smoothing else "random SU(3) links (hot), not smoothed"},
         "roofline": {"bound": "hbm", "achieved": head["GB_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(head["GB_s"] / HBM_PEAK_GBS, 4), "traffic": TRAFFIC_PER_LAUNCH.get(key),
                      "algorithmic_bytes_per_launch": algo_bytes},

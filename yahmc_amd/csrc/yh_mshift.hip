@@ -134,19 +134,18 @@ __global__ void __launch_bounds__(256) k_ms_k(MsState* st, const MsFields* __res
     }
   }
   // last workgroup: clear the update marks once every workgroup has used them
+  // (every wave of this workgroup has read the marks before the barrier;
+  // the ticket is a relaxed agent-scope atomic, as in block_reduce_finish)
   __shared__ int last;
   __syncthreads();
   if (threadIdx.x == 0) {
-    __threadfence();
-    unsigned t = atomicAdd(ticket, 1u);
+    unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     last = (t == gridDim.x - 1);
   }
   __syncthreads();
   if (last && threadIdx.x == 0) {
-    __threadfence();
     for (int s = 0; s < ns; s++) st->upd[s] = 0;
-    atomicExch(ticket, 0u);
-    __threadfence();
+    __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 

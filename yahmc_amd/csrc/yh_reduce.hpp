@@ -2,6 +2,15 @@
 // workgroup -> the last workgroup to arrive (ticket counter) sums the
 // partials in a fixed order and writes the result.  The result therefore
 // does not depend on scheduling, and no host round trip is needed.
+//
+// Publication without a device-scope fence: on CDNA3/4 each XCD has its own
+// L2, and an agent-scope release fence writes back every dirty line of the
+// XCD's L2 - in a kernel that streams hundreds of MB of stores that stalls
+// each workgroup for the whole write-back.  Instead each partial is stored
+// with an agent-scope atomic store (coherent across XCDs, written through
+// the L2), the wave waits for the store to complete (s_waitcnt), and only
+// then takes its ticket with a relaxed agent-scope atomic; the last
+// workgroup reads the partials with agent-scope atomic loads.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -33,15 +42,14 @@ __device__ bool block_reduce_finish(double (&v)[NR], double* partials, int nblk,
       double s = 0.0;
 #pragma unroll
       for (int q = 0; q < NT / 64; q++) s += sm[r][q];
-      partials[(size_t)r * nblk + blk] = s;
+      __hip_atomic_store(&partials[(size_t)r * nblk + blk], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    __threadfence();
-    unsigned t = atomicAdd(ticket, 1u);
+    __builtin_amdgcn_s_waitcnt(0);  // the partial stores have completed
+    unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     is_last = (t == (unsigned)(nblk - 1));
   }
   __syncthreads();
   if (!is_last) return false;
-  __threadfence();
 #pragma unroll
   for (int r = 0; r < NR; r++) {
     double s = 0.0;
@@ -57,10 +65,7 @@ __device__ bool block_reduce_finish(double (&v)[NR], double* partials, int nblk,
       out[r] = tot;
     }
   }
-  if (threadIdx.x == 0) {
-    atomicExch(ticket, 0u);
-    __threadfence();
-  }
+  if (threadIdx.x == 0) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __syncthreads();
   return true;
 }
