@@ -286,6 +286,10 @@ int yh_synchronize(yh_ctx* ctx);
  * YH_TUNE_XCD_CHUNK = consecutive workgroups each XCD takes (0 = one slab
  * per XCD; default 1/8 of a t-slice) */
 #define YH_TUNE_XCD_CHUNK 1
+/* YH_TUNE_NET_PREFETCH = network layer kernels issue the next path's loads
+ * before the current path's arithmetic (1) or not (0, default: the extra
+ * registers cost more occupancy than the prefetch gains) */
+#define YH_TUNE_NET_PREFETCH 2
 int yh_tune(yh_ctx* ctx, int key, int value);
 
 #ifdef __cplusplus

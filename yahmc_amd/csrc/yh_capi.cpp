@@ -609,6 +609,10 @@ int yh_tune(yh_ctx* c, int key, int value) {
     c->g.xq = value;
     return YH_OK;
   }
+  if (key == YH_TUNE_NET_PREFETCH) {
+    c->net_prefetch = value != 0;
+    return YH_OK;
+  }
   set_error("unknown tuning key");
   return YH_ERR_ARG;
 }

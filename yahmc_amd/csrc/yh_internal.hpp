@@ -77,6 +77,8 @@ struct Ctx {
   // molecular dynamics: momenta (SoA, FP64) and clover force matrices
   double* mom = nullptr;
   void* clf = nullptr;
+  // learned preconditioner: layer kernels prefetch the next path (tunable)
+  int net_prefetch = 0;
 };
 
 // ---- fields ---------------------------------------------------------------

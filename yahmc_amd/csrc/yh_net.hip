@@ -112,43 +112,110 @@ struct PtcShape {
 
 // ---------------------------------------------------------------------------
 // forward layer (also the adjoint, with the adjoint weight matrix)
-// in: [CIN][V][3][4], out: [COUT][V][3][4]; tiles of 16 sites from tile0
+// in: [CIN][V][3][4], out: [COUT][V][3][4]; tiles of 16 sites
 // ---------------------------------------------------------------------------
-template <int CIN, int COUT>
+typedef __attribute__((ext_vector_type(8))) float f8;
+
+// A-operand fragments (one per colour) of a transported spinor t[a][s]:
+// k order within the 8 = (spin, re/im); hardware RNE conversion to BF16
+__device__ __forceinline__ void make_frags(Frag (&A)[ND], const float2 (&t)[ND][4]) {
+#pragma unroll
+  for (int a = 0; a < ND; a++) {
+    f8 v = {t[a][0].x, t[a][0].y, t[a][1].x, t[a][1].y, t[a][2].x, t[a][2].y, t[a][3].x, t[a][3].y};
+    A[a].v = __builtin_convertvector(v, bf16x8);
+  }
+}
+
+// neighbour and link of path q (1..8) for a site with coordinates c
+struct PathRef {
+  int y;           // full index of the neighbour
+  const float2* l; // link base (element 0)
+  bool fw, flip;
+};
+
+__device__ __forceinline__ PathRef path_ref(const int (&c)[4], int x, int q, const float2* links, const Geom& g) {
+  PathRef P;
+  const int mu = (q - 1) & 3;
+  P.fw = q <= 4;
+  int n[4] = {c[0], c[1], c[2], c[3]};
+  n[mu] = P.fw ? (c[mu] + 1 == g.L[mu] ? 0 : c[mu] + 1) : (c[mu] == 0 ? g.L[mu] - 1 : c[mu] - 1);
+  P.y = full_index(n[0], n[1], n[2], n[3], g);
+  const int ls = P.fw ? x : P.y;
+  const int lp = ls >= g.VH;
+  P.l = links + (size_t)(lp * 4 + mu) * 9 * g.VHpad + (ls - lp * g.VH);
+  P.flip = g.bc == BC_ANTIPERIODIC && mu == 0 && (P.fw ? c[0] : n[0]) == 0;
+  return P;
+}
+
+__device__ __forceinline__ void load_h(float4 (&h)[6], const float2* base, int y) {
+  const float4* p = (const float4*)(base + (size_t)y * 12);
+#pragma unroll
+  for (int k = 0; k < 6; k++) h[k] = p[k];
+}
+
+__device__ __forceinline__ void load_u(float2 (&u)[9], const float2* l, int VHpad) {
+#pragma unroll
+  for (int e = 0; e < 9; e++) u[e] = l[(size_t)e * VHpad];
+}
+
+// t = sign * U h (fw) or sign * U^+ h; h as 6 float4 in [colour][spin] order
+__device__ __forceinline__ void apply_link(float2 (&t)[ND][4], const float2 (&u)[9], const float4 (&h)[6], bool fw,
+                                           bool flip) {
+  float2 hv[ND][4];
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    hv[(2 * k) / 4][(2 * k) % 4] = make_float2(h[k].x, h[k].y);
+    hv[(2 * k + 1) / 4][(2 * k + 1) % 4] = make_float2(h[k].z, h[k].w);
+  }
+  const float sg = flip ? -1.f : 1.f;
+#pragma unroll
+  for (int a = 0; a < ND; a++)
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+      float re = 0.f, im = 0.f;
+#pragma unroll
+      for (int b = 0; b < ND; b++) {
+        float2 w = fw ? u[a * 3 + b] : make_float2(u[b * 3 + a].x, -u[b * 3 + a].y);
+        re += w.x * hv[b][s].x - w.y * hv[b][s].y;
+        im += w.x * hv[b][s].y + w.y * hv[b][s].x;
+      }
+      t[a][s] = make_float2(sg * re, sg * im);
+    }
+}
+
+__device__ __forceinline__ void unpack_h(float2 (&t)[ND][4], const float4 (&h)[6]) {
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    t[(2 * k) / 4][(2 * k) % 4] = make_float2(h[k].x, h[k].y);
+    t[(2 * k + 1) / 4][(2 * k + 1) % 4] = make_float2(h[k].z, h[k].w);
+  }
+}
+
+template <int CIN, int COUT, bool PF>
 __global__ void __launch_bounds__(256) k_ptc(const float2* __restrict__ in, float2* __restrict__ out,
                                              const uint4* __restrict__ Bg, const float2* __restrict__ links, Geom g,
-                                             int tile0, int ntiles) {
+                                             int ntiles) {
   typedef PtcShape<CIN, COUT> S;
   extern __shared__ uint4 Bs[];
+  // output staging: per wave, one output channel x 16 sites x 24 floats
+  // (6 KiB per workgroup, so that two workgroups with the 72 KiB weight
+  // matrix of an 8 -> 8 layer still share a CU)
+  __shared__ __align__(16) float stage[4][16][24];
   for (int i = threadIdx.x; i < S::BFRAG; i += 256) Bs[i] = Bg[i];
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int r = lane & 15, grp = lane >> 4;
   for (int tile = blockIdx.x * 4 + wave; tile < ntiles; tile += gridDim.x * 4) {
-    const int xb = (tile0 + tile) * 16;
+    const int xb = tile * 16;
     const int x = xb + r;
+    int c[4];
+    full_coords(x, g, c);
     f4 acc[ND][S::NB];
 #pragma unroll
     for (int a = 0; a < ND; a++)
 #pragma unroll
       for (int nb = 0; nb < S::NB; nb++) acc[a][nb] = f4{0.f, 0.f, 0.f, 0.f};
-    for (int ks = 0; ks < S::KS; ks++) {
-      const int j = ks * 4 + grp;
-      float2 t[ND][4];
-      if (j < S::NP) {
-        const int q = j / CIN, i = j % CIN;
-        net_transport(t, in + (size_t)i * g.V * 12, links, x, q, g);
-      } else {
-#pragma unroll
-        for (int a = 0; a < ND; a++)
-#pragma unroll
-          for (int s = 0; s < 4; s++) t[a][s] = make_float2(0.f, 0.f);
-      }
-      Frag A[ND];
-#pragma unroll
-      for (int a = 0; a < ND; a++)
-        A[a].u = make_uint4(pack2(t[a][0].x, t[a][0].y), pack2(t[a][1].x, t[a][1].y), pack2(t[a][2].x, t[a][2].y),
-                            pack2(t[a][3].x, t[a][3].y));
+    auto mfma_step = [&](int ks, const Frag (&A)[ND]) {
 #pragma unroll
       for (int nb = 0; nb < S::NB; nb++) {
         Frag B;
@@ -156,21 +223,127 @@ __global__ void __launch_bounds__(256) k_ptc(const float2* __restrict__ in, floa
 #pragma unroll
         for (int a = 0; a < ND; a++) acc[a][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[a].v, B.v, acc[a][nb], 0, 0, 0);
       }
-    }
-    // lane holds column r of each N-block, rows 4 grp + i
-    float* o = (float*)out;
+    };
+    if constexpr (CIN % 4 == 0 && PF) {
+      // every lane group takes the same path in a K-step: one link per path,
+      // CIN/4 channels per lane; the next path's loads are issued before
+      // this path's arithmetic
+      constexpr int CPL = CIN / 4;
+      float4 h[CPL][6];
+      float2 u[9];
 #pragma unroll
-    for (int nb = 0; nb < S::NB; nb++) {
-      const int n = nb * 16 + r;
-      const int oc = n >> 3, s = (n & 7) >> 1, ri = n & 1;
-      if (oc >= COUT) continue;
+      for (int cc = 0; cc < CPL; cc++) load_h(h[cc], in + (size_t)(4 * cc + grp) * g.V * 12, x);
+      PathRef P{};
 #pragma unroll
-      for (int i = 0; i < 4; i++) {
-        const int xs = xb + grp * 4 + i;
+      for (int q = 0; q < 9; q++) {
+        float4 hn[CPL][6];
+        float2 un[9];
+        PathRef Pn{};
+        if (q < 8) {
+          Pn = path_ref(c, x, q + 1, links, g);
+          load_u(un, Pn.l, g.VHpad);
 #pragma unroll
-        for (int a = 0; a < ND; a++) o[((size_t)oc * g.V + xs) * 24 + (a * 4 + s) * 2 + ri] = acc[a][nb][i];
+          for (int cc = 0; cc < CPL; cc++) load_h(hn[cc], in + (size_t)(4 * cc + grp) * g.V * 12, Pn.y);
+        }
+#pragma unroll
+        for (int cc = 0; cc < CPL; cc++) {
+          float2 t[ND][4];
+          if (q == 0) unpack_h(t, h[cc]);
+          else apply_link(t, u, h[cc], P.fw, P.flip);
+          Frag A[ND];
+          make_frags(A, t);
+          mfma_step(q * CPL + cc, A);
+        }
+        if (q < 8) {
+#pragma unroll
+          for (int cc = 0; cc < CPL; cc++)
+#pragma unroll
+            for (int k = 0; k < 6; k++) h[cc][k] = hn[cc][k];
+#pragma unroll
+          for (int e = 0; e < 9; e++) u[e] = un[e];
+          P = Pn;
+        }
+      }
+    } else if constexpr (CIN % 4 == 0) {
+      // same order without the prefetch (fewer registers)
+      constexpr int CPL = CIN / 4;
+      for (int q = 0; q < 9; q++) {
+        PathRef P{};
+        float2 u[9];
+        if (q > 0) {
+          P = path_ref(c, x, q, links, g);
+          load_u(u, P.l, g.VHpad);
+        }
+        const int y = q > 0 ? P.y : x;
+#pragma unroll
+        for (int cc = 0; cc < CPL; cc++) {
+          float4 h[6];
+          load_h(h, in + (size_t)(4 * cc + grp) * g.V * 12, y);
+          float2 t[ND][4];
+          if (q == 0) unpack_h(t, h);
+          else apply_link(t, u, h, P.fw, P.flip);
+          Frag A[ND];
+          make_frags(A, t);
+          mfma_step(q * CPL + cc, A);
+        }
+      }
+    } else {
+      // CIN < 4: the lane groups of a K-step take different (path, channel) pairs
+      for (int ks = 0; ks < S::KS; ks++) {
+        const int j = ks * 4 + grp;
+        float2 t[ND][4];
+        if (j < S::NP) {
+          const int q = j / CIN, i = j % CIN;
+          float4 h[6];
+          if (q == 0) {
+            load_h(h, in + (size_t)i * g.V * 12, x);
+            unpack_h(t, h);
+          } else {
+            PathRef P = path_ref(c, x, q, links, g);
+            float2 u[9];
+            load_u(u, P.l, g.VHpad);
+            load_h(h, in + (size_t)i * g.V * 12, P.y);
+            apply_link(t, u, h, P.fw, P.flip);
+          }
+        } else {
+#pragma unroll
+          for (int a = 0; a < ND; a++)
+#pragma unroll
+            for (int s = 0; s < 4; s++) t[a][s] = make_float2(0.f, 0.f);
+        }
+        Frag A[ND];
+        make_frags(A, t);
+        mfma_step(ks, A);
       }
     }
+    // epilogue: lane holds column r of each N-block for rows (sites) 4 grp + i;
+    // stage one output channel at a time in LDS, then write its 16 sites x
+    // 96 B with coalesced float4 stores
+#pragma unroll
+    for (int nb = 0; nb < S::NB; nb++)
+#pragma unroll
+      for (int ch = 0; ch < 2; ch++) {
+        const int oc = 2 * nb + ch;
+        if (oc >= COUT) break;
+        const int n = nb * 16 + r;
+        if ((n >> 3) == oc) {
+#pragma unroll
+          for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int a = 0; a < ND; a++) stage[wave][grp * 4 + i][a * 8 + (n & 7)] = acc[a][nb][i];
+        }
+        // the staging area is private to this wave: order its LDS writes
+        // before the reads (LDS executes one wave's operations in order)
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        for (int idx = lane; idx < 96; idx += 64) {
+          const int site = idx / 6, part = idx % 6;
+          const float4 v = *(const float4*)&stage[wave][site][part * 4];
+          *(float4*)((float*)out + ((size_t)oc * g.V + xb + site) * 24 + part * 4) = v;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+      }
   }
 }
 
@@ -483,11 +656,16 @@ static int net_layer(Net* n, int l, int adjoint, const float2* in, float2* out, 
     size_t lds = sizeof(uint4) * PtcShape<CI_, CO_>::BFRAG;
     static bool lds_set = false;  // the weight matrix can exceed 64 KiB of LDS
     if (!lds_set) {
-      YH_HIP(hipFuncSetAttribute((const void*)k_ptc<CI_, CO_>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      YH_HIP(hipFuncSetAttribute((const void*)k_ptc<CI_, CO_, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      YH_HIP(hipFuncSetAttribute((const void*)k_ptc<CI_, CO_, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
       lds_set = true;
     }
-    hipLaunchKernelGGL((k_ptc<CI_, CO_>), dim3(grid), dim3(256), lds, c->stream, in, out,
-                       adjoint ? n->Ba[l] : n->Bf[l], links, c->g, 0, ntiles);
+    if (c->net_prefetch)
+      hipLaunchKernelGGL((k_ptc<CI_, CO_, true>), dim3(grid), dim3(256), lds, c->stream, in, out,
+                         adjoint ? n->Ba[l] : n->Bf[l], links, c->g, ntiles);
+    else
+      hipLaunchKernelGGL((k_ptc<CI_, CO_, false>), dim3(grid), dim3(256), lds, c->stream, in, out,
+                         adjoint ? n->Ba[l] : n->Bf[l], links, c->g, ntiles);
   });
   YH_HIP(hipGetLastError());
   return YH_OK;
