@@ -143,6 +143,37 @@ def learned_solves(yh, lat, fb, prec, train_steps):
     return out
 
 
+def hmc_trajectory(yh):
+    """One HMC trajectory on 24^3 x 32 with two flavours of sextet (two-index
+    symmetric SU(3)) clover fermions: tau = 1, o2mn with the Wilson gauge
+    monomial on level 0 (1 step) and the fermion monomial on level 1
+    (10 steps), mixed-precision CG to an FP64 true residual of 1e-12, FP64
+    link update and Hamiltonian.  Seeded random SU(3) links (hot start);
+    beta = 5.6 and m = -0.75 (values the workload leaves open)."""
+    from yahmc_amd.hmc import HMC, FermionMonomial, GaugeMonomial
+    from yahmc_amd.synthetic import random_gauge
+    dims = (32, 24, 24, 24)
+    lat = yh.Lattice(dims, nc=3, repr="symmetric", bc="antiperiodic", csw=1.0)
+    lat.set_gauge(random_gauge(dims, 3, seed=32244))
+    plaq0 = lat.plaquette()
+    fm = FermionMonomial(-0.75, prec=1e-24, level=1, sloppy="f32")
+    hmc = HMC(lat, [GaugeMonomial(5.6), fm], [("o2mn", 1), ("o2mn", 10)])
+    lat.synchronize()
+    t0 = time.perf_counter()
+    rec = hmc.trajectory(np.random.default_rng(32247), length=1.0, metropolis=True)
+    lat.synchronize()
+    dt = time.perf_counter() - t0
+    out = {"lattice": list(dims), "repr": "sextet", "beta": 5.6, "mass": -0.75, "csw": 1.0,
+           "integrator": "o2mn, gauge level 0 x 1 step, fermion level 1 x 10 steps", "tau": 1.0,
+           "seconds": round(dt, 3), "dH": rec["dH"], "accepted": rec["accepted"],
+           "plaquette_before": round(plaq0, 6), "plaquette_after": round(rec["plaquette"], 6),
+           "cg_solves": len(fm.iterations), "cg_iterations_total": int(sum(fm.iterations)),
+           "cg": "mixed FP32 reliable-update CG, FP64 true residual 1e-12"}
+    log(f"hmc trajectory: {out}")
+    lat.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,6 +186,8 @@ def main():
     ap.add_argument("--no-smooth", action="store_true", help="keep the hot random field (no gauge trajectories)")
     ap.add_argument("--skip-learned", action="store_true")
     ap.add_argument("--train-steps", type=int, default=120)
+    ap.add_argument("--skip-hmc", action="store_true")
+    ap.add_argument("--only-hmc", action="store_true", help="the sextet HMC trajectory only")
     ap.add_argument("--only-learned", action="store_true",
                     help="smoothing, network training and the two flexible-CG solves only (for profiling)")
     args = ap.parse_args()
@@ -166,6 +199,9 @@ def main():
     import yahmc_amd as yh
     from yahmc_amd.synthetic import gaussian_source, random_gauge
 
+    if args.only_hmc:
+        print(json.dumps(hmc_trajectory(yh)))
+        return
     t0 = time.perf_counter()
     links = random_gauge(DIMS, 3, seed=32244)
     src = gaussian_source(DIMS, 3, seed=32245, parity="full")
@@ -236,6 +272,11 @@ def main():
                 if k in cg:
                     cg[k]["speedup_vs_fp64"] = round(cg["fp64"]["seconds"] / cg[k]["seconds"], 2)
 
+    hmc = None
+    if not args.skip_hmc and not args.only_hop:
+        lat.close()
+        hmc = hmc_trajectory(yh)
+
     cpu = None
     if not args.skip_cpu and not args.only_hop:
         threads = min(16, os.cpu_count() or 1)
@@ -269,6 +310,7 @@ def main():
         "dslash": hop,
         "copy_kernel_GB_s": round(copy_gbs, 1),
         "cg_time_to_1e-10": cg,
+        "hmc_trajectory": hmc,
     }
     print(json.dumps(line))
 

@@ -251,6 +251,8 @@ class Lattice:
         self.ctx = h
         self.dims = tuple(int(d) for d in dims)
         self.nc = nc
+        self.csw = float(csw)
+        self.bc = lt.bc
         self.dr = lib().yh_repr_dim(h)
         self.V = int(np.prod(self.dims))
         self.VH = self.V // 2

@@ -1,0 +1,204 @@
+"""Hybrid Monte Carlo driven through the device kernels.
+
+Monomials (library/monomials/gauge.cpp, hmc.cpp, list.cpp), the nested
+integrators o2lf / o2mn / o4mn (library/integrator.cpp:6-96) and the
+trajectory with its Metropolis step (library/update.cpp:284-339).  Level 0
+is the innermost integrator and carries the gauge monomial; fermion
+monomials sit on higher levels, whose field update runs the next lower
+integrator and then re-represents the links (set_gauge_field_state,
+library/repr.cpp:282-298).
+
+The two-flavour monomial solves with CG on H^2 instead of the reference's
+two BiCGstab solves on D (MonomialHMC::update, hmc.cpp:67-93): the force
+needs Y = D^-1 g5 D^-1 pf = (H^2)^-1 g5 pf, and the action
+|D^-1 pf|^2 = <g5 pf, (H^2)^-1 g5 pf>, both from the same solve (FP32
+reliable-update CG reaching the FP64 true residual `prec`).
+
+Random numbers (momenta, pseudofermion sources) are drawn on the host with
+numpy and the reference's normalisation (N(0, 1/2) per real component,
+include/random.h:15); the GSL generator itself is not reproduced."""
+import numpy as np
+
+from .md import cold_start  # noqa: F401  (re-exported for drivers)
+
+O2MN = 0.1931833275037836                      # integrator.cpp:44
+O4MN = (0.08398315262876693, 0.2539785108410595, 0.6822365335719091, -0.03230286765269967)  # :67-70
+
+
+class GaugeMonomial:
+    """Wilson plaquette action (MonomialGauge with c0 = 1, gauge.cpp)."""
+
+    def __init__(self, beta, level=0):
+        if level != 0:
+            raise ValueError("the gauge monomial must be on integrator level 0 (update.cpp)")
+        self.beta = beta
+        self.level = 0
+
+    def reset(self, lat, rng):
+        pass
+
+    def action(self, lat):
+        return -self.beta * lat.plaquette() * 6.0 * lat.V
+
+    def update(self, lat, dt):
+        lat.wilson_gauge_force(dt, self.beta)
+
+
+class FermionMonomial:
+    """Two flavours of EO Wilson-clover fermions (MonomialHMC, hmc.cpp)."""
+
+    def __init__(self, mass, dm=0.0, prec=1e-24, level=1, sloppy="f32", max_iter=100000):
+        if level == 0:
+            raise ValueError("fermion monomials cannot be on integrator level 0 (update.cpp)")
+        self.omass = mass
+        self.smass = mass + dm
+        self.prec = prec
+        self.level = level
+        self.sloppy = sloppy
+        self.max_iter = max_iter
+        self.pf = None
+        self.la = None
+        self.iterations = []
+
+    def _fields(self, lat):
+        if self.pf is None:
+            self.pf = lat.field("f64")
+            self.g5pf = lat.field("f64")
+            self.y = lat.field("f64")
+            self.eta = lat.field("f64")
+
+    def reset(self, lat, rng):
+        """pf = D eta, eta Gaussian on the even sites; the first action call
+        returns |eta|^2 - 2 log det D_oo (hmc.cpp:27-40)."""
+        self._fields(lat)
+        eta = np.zeros((lat.V, 8 * lat.dr))
+        eta[: lat.V // 2] = rng.normal(0.0, np.sqrt(0.5), size=(lat.V // 2, 8 * lat.dr))
+        self.eta.upload(eta)
+        self.la = float(np.sum(eta * eta))
+        if lat.csw != 0.0:
+            self.la -= 2.0 * lat.clover_logdet(self.omass)
+        lat.Dphi(self.smass, self.pf, self.eta)
+        self.g5pf.copy_from(self.pf)
+        lat.spinor_g5(self.g5pf)
+
+    def _solve(self, lat):
+        self.y.zero()
+        par = lat.inv_par(self.smass, self.prec, "Hphi_sq", sloppy=self.sloppy, max_iter=self.max_iter)
+        it, _ = lat.cg_mixed(par, self.y, self.g5pf)
+        self.iterations.append(it)
+        return self.y
+
+    def action(self, lat):
+        if self.la is not None:
+            v, self.la = self.la, None
+            return v
+        y = self._solve(lat)
+        v = lat.spinor_product_re(self.g5pf, y)
+        if lat.csw != 0.0:
+            v -= 2.0 * lat.clover_logdet(self.omass)
+        return v
+
+    def update(self, lat, dt):
+        y = self._solve(lat)
+        lat.wilson_fermion_force_q(dt, self.smass, 1.0, y)
+        if lat.csw != 0.0:
+            lat.clover_logdet_force(self.omass, 2.0)
+
+
+class HMC:
+    """integrators: [(type, steps)] from level 0 (innermost) upwards, type in
+    {"o2lf", "o2mn", "o4mn"}; monomials carry their level."""
+
+    def __init__(self, lat, monomials, integrators):
+        self.lat = lat
+        self.monomials = list(monomials)
+        self.integrators = list(integrators)
+        self.dirty = False
+        for m in self.monomials:
+            if m.level >= len(self.integrators):
+                raise ValueError(f"monomial level {m.level} has no integrator")
+
+    # ---- update_momenta / update_fields (integrator.cpp:6-17, list.cpp) --
+    def _momenta(self, dt, level):
+        lat = self.lat
+        if level:
+            lat.begin_fermion_force()
+        for m in self.monomials:
+            if m.level == level:
+                m.update(lat, dt)
+        if level:
+            lat.end_fermion_force(dt)
+
+    def _fields(self, dt, level):
+        if level == 0:
+            self.lat.update_links(dt)
+            self.dirty = True
+        else:
+            self._integrate(dt, level - 1)
+            if self.dirty:
+                self.lat.represent_gauge_field()
+                self.dirty = False
+
+    def _integrate(self, length, level):
+        kind, steps = self.integrators[level]
+        dt = length / steps
+        M, F = self._momenta, self._fields
+        if kind == "o2lf":
+            for n in range(steps):
+                M(dt / 2 if n == 0 else dt, level)
+                F(dt, level)
+            M(dt / 2, level)
+        elif kind == "o2mn":
+            a = O2MN
+            for n in range(steps):
+                M(a * dt if n == 0 else 2 * a * dt, level)
+                F(dt / 2, level)
+                M((1 - 2 * a) * dt, level)
+                F(dt / 2, level)
+            M(a * dt, level)
+        elif kind == "o4mn":
+            a, b, c, d = O4MN
+            for n in range(steps):
+                M(a * dt if n == 0 else 2 * a * dt, level)
+                F(b * dt, level)
+                M(c * dt, level)
+                F(d * dt, level)
+                M((0.5 - a - c) * dt, level)
+                F((1 - 2 * (b + d)) * dt, level)
+                M((0.5 - a - c) * dt, level)
+                F(d * dt, level)
+                M(c * dt, level)
+                F(b * dt, level)
+            M(a * dt, level)
+        else:
+            raise ValueError(f"unknown integrator {kind}")
+
+    def hamiltonian(self):
+        """TF/2 sum P^2 + sum of monomial actions (update.cpp:201-226)."""
+        return self.lat.momenta_energy() + sum(m.action(self.lat) for m in self.monomials)
+
+    def trajectory(self, rng, length=1.0, metropolis=True):
+        """One trajectory (update.cpp:284-339): refresh momenta and
+        pseudofermions, integrate, Metropolis on dH, then reunitarise or
+        restore the links."""
+        lat = self.lat
+        U0 = lat.get_gauge()
+        ng = lat.nc * lat.nc - 1
+        lat.set_momenta(rng.normal(0.0, np.sqrt(0.5), size=(4 * lat.V, ng)))
+        for m in self.monomials:
+            m.reset(lat, rng)
+        h0 = self.hamiltonian()
+        self._integrate(length, len(self.integrators) - 1)
+        if self.dirty:
+            lat.represent_gauge_field()
+            self.dirty = False
+        dh = self.hamiltonian() - h0
+        accepted = True
+        if metropolis:
+            accepted = bool(dh < 0 or rng.uniform() < np.exp(-dh))
+        if accepted:
+            lat.reunitarize()
+        else:
+            lat.set_gauge(U0)
+        lat.represent_gauge_field()
+        return {"dH": float(dh), "accepted": accepted, "plaquette": lat.plaquette()}
