@@ -47,7 +47,7 @@ FLOP_PER_SITE = 1320
 # HBM-side bytes per launch of each hop variant, from two rocprofv3 --pmc
 # passes (FETCH_SIZE, WRITE_SIZE) scaled by the same counters on the copy
 # kernel; see profiles/hop_f32r12_32x4_traffic.md.  None = not measured.
-TRAFFIC_PER_LAUNCH = {"f32_r12": 406281886}
+TRAFFIC_PER_LAUNCH = {"f32_r12": 348221239}
 
 
 def log(msg):
