@@ -290,7 +290,15 @@ int yh_synchronize(yh_ctx* ctx);
  * before the current path's arithmetic (1) or not (0, default: the extra
  * registers cost more occupancy than the prefetch gains) */
 #define YH_TUNE_NET_PREFETCH 2
+/* YH_TUNE_GRAPHS = the multishift solver captures its 16-iteration chunk
+ * in a HIP graph once and replays it (1, default) or launches eagerly (0) */
+#define YH_TUNE_GRAPHS 3
 int yh_tune(yh_ctx* ctx, int key, int value);
+/* autotuner (no counterpart in the reference): times the hop of every
+ * precision / reconstruction this context holds for each XCD chunk size
+ * (`reps` launches each) and keeps the fastest per variant; chosen[6]
+ * (optional) receives them, index prec * 2 + recon (-1: variant absent) */
+int yh_autotune(yh_ctx* ctx, int reps, int* chosen);
 
 #ifdef __cplusplus
 }

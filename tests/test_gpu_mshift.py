@@ -90,3 +90,61 @@ def test_rational_calc_inverse_square_root():
     got = spinor_to_complex(fd.download(), 3)[:VH].reshape(-1)
     print(f"order={rat.order} prec={rat.prec:.2e} iterations={it} range=[{lo:.3e}, {hi:.3e}]")
     assert np.linalg.norm(got - want) / np.linalg.norm(want) < 5 * rat.prec + 1e-12
+
+
+def test_graph_replay_matches_eager():
+    """The multishift chunk captured in a HIP graph and replayed gives the
+    iterates of the eager launches bit for bit, over two solves in a row
+    with other device work and host synchronisation in between."""
+    print("\n[case] mshift graph replay", flush=True)
+    o, lat, rng = setup("su3_fund", 3, "fundamental", (4, 4, 4, 8), seed=21)
+    mass, prec = -0.2, 1e-24
+    shifts = [0.7, 0.05, 0.002]
+    srcs = []
+    for _ in range(2):
+        b = gaussian_spinor(rng, o.V, o.dr)
+        b[o.VH:] = 0
+        srcs.append(lat.field("f64").upload(b))
+    results = {}
+    for graphs in (0, 1):
+        lat.tune(yh.TUNE_GRAPHS, graphs)
+        runs = []
+        for rep in range(2):
+            dst = [[lat.field("f64") for _ in shifts] for _ in srcs]
+            its = lat.cg_mshift_batched(lat.inv_par(mass, prec, "Hphi_sq"), shifts, dst, srcs)
+            # eager device work and a synchronisation between the replays
+            t = lat.field("f64")
+            lat.Hphi_sq(mass, t, srcs[0])
+            lat.synchronize()
+            runs.append((its, [[f.download() for f in row] for row in dst]))
+        assert runs[0][0] == runs[1][0]
+        for r0, r1 in zip(runs[0][1], runs[1][1]):
+            for a, b in zip(r0, r1):
+                assert np.array_equal(a, b)
+        results[graphs] = runs[0]
+        print(f"graphs={graphs}: iterations {runs[0][0]}")
+    assert max(results[1][0]) > 16  # more than one chunk: the graph was replayed
+    assert results[0][0] == results[1][0]
+    for r0, r1 in zip(results[0][1], results[1][1]):
+        for a, b in zip(r0, r1):
+            assert np.array_equal(a, b)
+
+
+def test_autotune_keeps_results():
+    """The autotuner only changes the workgroup -> XCD order: the hop output
+    is bit-identical before and after, and every chunk it picks is valid."""
+    print("\n[case] autotune", flush=True)
+    o, lat, rng = setup("su3_fund", 3, "fundamental", (8, 8, 8, 8), seed=23)
+    s = gaussian_spinor(rng, o.V, 3)
+    s[: o.VH] = 0
+    fs = lat.field("f64").upload(s)
+    d0 = lat.field("f64")
+    lat.Dphi_eo(d0, fs)
+    before = d0.download()
+    chosen = lat.autotune(reps=5)
+    print(f"chosen XCD chunks {chosen}")
+    assert ("f64", 18) in chosen and ("f32", 12) in chosen
+    assert all(v >= 0 for v in chosen.values())
+    d1 = lat.field("f64")
+    lat.Dphi_eo(d1, fs)
+    assert np.array_equal(d1.download(), before)

@@ -19,6 +19,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "lib", "libyahmc.so")
 
 YH_OK = 0
+TUNE_XCD_CHUNK, TUNE_NET_PREFETCH, TUNE_GRAPHS = 1, 2, 3
 ERRORS = {-1: "invalid argument", -2: "HIP error", -3: "bad state", -4: "not converged", -5: "non-finite",
           -6: "unsupported"}
 
@@ -144,6 +145,7 @@ _SIGS = {
     "yh_net_wgrad_host": ([_VP, ctypes.c_int, _D, _D, _D], ctypes.c_int),
     "yh_cg_net": ([_VP, ctypes.c_void_p, _VP, _VP, _VP, _PI, _PI], ctypes.c_int),
     "yh_tune": ([_VP, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "yh_autotune": ([_VP, ctypes.c_int, _PI], ctypes.c_int),
 }
 
 
@@ -493,6 +495,14 @@ class Lattice:
 
     def tune(self, key, value):
         _check(lib().yh_tune(self.ctx, key, value))
+
+    def autotune(self, reps=20):
+        """Pick the fastest XCD chunk per hop variant (yh_autotune); returns
+        {(prec, recon): chunk} for the variants this context holds."""
+        out = (ctypes.c_int * 6)()
+        _check(lib().yh_autotune(self.ctx, reps, out))
+        names = ("f64", "f32", "f16")
+        return {(names[k // 2], 18 if k % 2 == 0 else 12): out[k] for k in range(6) if out[k] >= 0}
 
 
 class Net:

@@ -613,6 +613,10 @@ int yh_tune(yh_ctx* c, int key, int value) {
     c->net_prefetch = value != 0;
     return YH_OK;
   }
+  if (key == YH_TUNE_GRAPHS) {
+    c->use_graphs = value != 0;
+    return YH_OK;
+  }
   set_error("unknown tuning key");
   return YH_ERR_ARG;
 }
@@ -643,6 +647,47 @@ int yh_bench_hop(yh_ctx* c, int prec, int recon, int reps, yh_field* d, const yh
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   *ms = t / reps;
+  return YH_OK;
+}
+
+// Online autotuner: for every hop variant this context can run, time the
+// plain hop over the even sites for a set of XCD chunk sizes and keep the
+// fastest (the block -> XCD tiling is the shape that matters for this
+// HBM-bound stencil: it sets which t-slices share an XCD's L2).
+int yh_autotune(yh_ctx* c, int reps, int* chosen) {
+  if (!c || reps < 1) { set_error("bad argument"); return YH_ERR_ARG; }
+  if (!c->gauge_set) { set_error("no gauge field"); return YH_ERR_STATE; }
+  YH_HIP(hipSetDevice(c->device));
+  const int nblocks = (c->g.VH + 255) / 256;
+  std::vector<int> cand = {0};
+  for (int q = 1; 8 * q <= nblocks; q *= 2) cand.push_back(q);
+  for (int prec = 0; prec < 3; prec++)
+    for (int recon = 0; recon < 2; recon++) {
+      if (chosen) chosen[prec * 2 + recon] = -1;
+      if (!c->links[prec][recon]) continue;
+      if (prec == PREC_F16 && c->dr > 4) continue;
+      int rc;
+      Field *s = nullptr, *d = nullptr;
+      if ((rc = field_alloc(c, prec, &s)) || (rc = field_alloc(c, prec, &d))) {
+        field_free(s);
+        return rc;
+      }
+      double best = 1e30;
+      int bq = -1;
+      for (int q : cand) {
+        c->xq_tab[prec][recon] = q;
+        double ms;
+        rc = yh_bench_hop(static_cast<yh_ctx*>(c), prec, recon, reps, static_cast<yh_field*>(d),
+                          static_cast<const yh_field*>(s), &ms);
+        if (rc) break;
+        if (ms < best) { best = ms; bq = q; }
+      }
+      field_free(s);
+      field_free(d);
+      if (rc) { c->xq_tab[prec][recon] = -1; return rc; }
+      c->xq_tab[prec][recon] = bq;
+      if (chosen) chosen[prec * 2 + recon] = bq;
+    }
   return YH_OK;
 }
 

@@ -69,6 +69,7 @@ int hop_launch(Ctx* c, const HopCall& h) {
   a.red_out = h.reduce_norm ? (h.reduce_ptr ? h.reduce_ptr : &c->sc->red[0] + h.reduce_slot) : nullptr;
   a.skip = h.skip_flag;
   a.g = c->g;
+  if (c->xq_tab[h.prec][h.recon] >= 0) a.g.xq = c->xq_tab[h.prec][h.recon];
   c->mvm_counter++;
   switch (c->dr) {
     case 2: return hop_dispatch_d<2>(c, a, h.prec, h.recon, h.epi);

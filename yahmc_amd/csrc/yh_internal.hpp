@@ -79,6 +79,10 @@ struct Ctx {
   void* clf = nullptr;
   // learned preconditioner: layer kernels prefetch the next path (tunable)
   int net_prefetch = 0;
+  // per (precision, reconstruction) XCD chunk chosen by the autotuner (-1: g.xq)
+  int xq_tab[3][2] = {{-1, -1}, {-1, -1}, {-1, -1}};
+  // solver chunks are captured once into a HIP graph and replayed
+  int use_graphs = 1;
 };
 
 // ---- fields ---------------------------------------------------------------

@@ -258,17 +258,52 @@ int mshift_device(Ctx* c, const yh_inv_par* par, int nrhs, int num, const double
   }
   const int chunk = 16;
   int done_iters = 0;
-  while (!rc) {
-    for (int i = 0; i < chunk && !rc; i++)
-      for (int j = 0; j < nrhs && !rc; j++) {
+  // one chunk of iterations for every right-hand side; all its launches take
+  // their scalars from device memory, so the same sequence can be replayed
+  auto enqueue_chunk = [&]() -> int {
+    int e = YH_OK;
+    for (int i = 0; i < chunk && !e; i++)
+      for (int j = 0; j < nrhs && !e; j++) {
         Rhs& R = rh[j];
         // q = H^2 p, and pap = |H p|^2 into st[j].pap via the hop reduction
-        if ((rc = apply_op(c, PREC_F64, recon, YH_MVM_HPHI_SQ, par->mass, R.q, R.p, nullptr, 0, &st[j].done, &st[j].pap)))
+        if ((e = apply_op(c, PREC_F64, recon, YH_MVM_HPHI_SQ, par->mass, R.q, R.p, nullptr, 0, &st[j].done, &st[j].pap)))
           break;
-        if ((rc = ms_x(c, st + j, mf + j, R.r, R.q))) break;
-        rc = ms_k(c, st + j, mf + j, R.r, R.p);
+        if ((e = ms_x(c, st + j, mf + j, R.r, R.q))) break;
+        e = ms_k(c, st + j, mf + j, R.r, R.p);
       }
-    if (rc) break;
+    return e;
+  };
+  // The first chunk runs eagerly (it allocates the operator's scratch fields
+  // and the clover inverse); later chunks replay a HIP graph of one chunk.
+  // The chunk records no memset node.  If capture fails the solver stays eager.
+  hipGraphExec_t exec = nullptr;
+  bool graphs = c->use_graphs != 0;
+  long mvm_per_chunk = 0;
+  while (!rc) {
+    if (done_iters > 0 && graphs && !exec) {
+      hipGraph_t gr = nullptr;
+      long mvm0 = c->mvm_counter;
+      hipError_t ge = hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed);
+      int crc = ge == hipSuccess ? enqueue_chunk() : YH_ERR_HIP;
+      if (ge == hipSuccess) ge = hipStreamEndCapture(c->stream, &gr);
+      if (ge == hipSuccess && crc == YH_OK) ge = hipGraphInstantiate(&exec, gr, nullptr, nullptr, 0);
+      if (gr) (void)hipGraphDestroy(gr);
+      mvm_per_chunk = c->mvm_counter - mvm0;
+      c->mvm_counter = mvm0;
+      if (ge != hipSuccess || crc != YH_OK || !exec) {
+        (void)hipGetLastError();
+        if (exec) (void)hipGraphExecDestroy(exec);
+        exec = nullptr;
+        graphs = false;
+      }
+    }
+    if (exec) {
+      he = hipGraphLaunch(exec, c->stream);
+      if (he != hipSuccess) { set_error(hipGetErrorString(he)); rc = YH_ERR_HIP; break; }
+      c->mvm_counter += mvm_per_chunk;
+    } else if ((rc = enqueue_chunk())) {
+      break;
+    }
     done_iters += chunk;
     he = hipMemcpyAsync(hs.data(), st, sizeof(MsState) * nrhs, hipMemcpyDeviceToHost, c->stream);
     if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
@@ -289,6 +324,7 @@ int mshift_device(Ctx* c, const yh_inv_par* par, int nrhs, int num, const double
   if (par->log && !rc)
     fprintf(stderr, "[INVERTER][NOTICE] cg_mshift (device, %d rhs, %d shifts): iterations = %d\n", nrhs, num, itmax);
   (void)hipStreamSynchronize(c->stream);
+  if (exec) (void)hipGraphExecDestroy(exec);
   cleanup();
   (void)hipFree(st);
   (void)hipFree(mf);
