@@ -157,6 +157,22 @@ int yh_bicgstab(yh_ctx* ctx, const yh_inv_par* par, yh_field* dst, const yh_fiel
 /* cg_mshift (library/cg.cpp:11-130): (mvm + shift[n]) dst[n] = src */
 int yh_cg_mshift(yh_ctx* ctx, const yh_inv_par* par, int num, const double* shift, yh_field** dst,
                  const yh_field* src, int* iters);
+/* minres_inverter (library/minres.cpp:12-118): Hphi or Hphi_sq; dst holds
+ * the initial guess */
+int yh_minres(yh_ctx* ctx, const yh_inv_par* par, yh_field* dst, const yh_field* src, int* iters);
+/* auto_inverter (library/inverter.cpp:9-55): by par->mvm, BiCGstab on Dphi
+ * (two solves with gamma5 for Hphi_sq, one for Hphi and Dphi); src is
+ * restored on return */
+int yh_auto_inverter(yh_ctx* ctx, const yh_inv_par* par, yh_field* dst, yh_field* src, int* iters);
+/* MRE initial guess from past solutions (library/mre.cpp:15-117,
+ * include/mre.h:10-21): mre_init / mre_guess / mre_store / mre_reset.
+ * max = 0 disables it (guess and store do nothing), max is capped at 15. */
+typedef struct yh_mre yh_mre;
+int yh_mre_create(yh_ctx* ctx, int max, int mvm, double mass, yh_mre** out);
+int yh_mre_destroy(yh_mre* mre);
+int yh_mre_reset(yh_mre* mre);
+int yh_mre_store(yh_mre* mre, const yh_field* x);
+int yh_mre_guess(yh_mre* mre, yh_field* dst, const yh_field* src);
 /* project-new: FP64 reliable-update CG with an FP32 / FP16 inner operator;
  * reaches the same FP64 true residual as yh_cg.  iters = sloppy iterations,
  * updates = FP64 reliable updates */

@@ -105,6 +105,8 @@ class Oracle:
         lib.or_gauge_action.restype = ctypes.c_double
         lib.or_kinetic.restype = ctypes.c_double
         lib.or_update_links.argtypes = [ctypes.c_double]
+        lib.or_minres.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _D, _D, _D]
+        lib.or_minres.restype = ctypes.c_int
         lib.or_ptc_layer.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D, _D, ctypes.c_int]
         lib.or_ptc_wgrad.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D, _D]
 
@@ -252,6 +254,12 @@ class Oracle:
         r = np.zeros(1)
         it = self.lib.or_bicgstab(mvm, mass, prec, _p(d), _p(s), _p(r))
         return it, r[0]
+
+    def minres(self, mvm, mass, prec, d, s):
+        """minres_inverter (library/minres.cpp:12-118); d holds the guess."""
+        res = np.zeros(1)
+        it = self.lib.or_minres(mvm, mass, prec, _p(d), _p(s), _p(res))
+        return it, float(res[0])
 
     def auto_inverter(self, mvm, mass, prec, d, s):
         return self.lib.or_auto_inverter(mvm, mass, prec, _p(d), _p(s))
@@ -428,3 +436,52 @@ class OracleNet:
         g1 = self.wgrad(u, e)
         g2 = self.wgrad(v, b)
         return loss, [a + c for a, c in zip(g1, g2)]
+
+
+class OracleMRE:
+    """mre_init / mre_store / mre_guess (library/mre.cpp:15-117) restated
+    over the oracle's operators: Gram-Schmidt of the stored solutions with
+    the reference's product sum a conj(b) and update s_j -= r_ij s_i
+    (mre.cpp:15-33), Galerkin matrix A[j][i] = <M s_i, s_j>, b_i = <b, s_i>
+    (mre.cpp:66-77), dense solve, x0 = sum x_k s_k."""
+
+    def __init__(self, o, max_past, mvm, mass):
+        self.o, self.max, self.mvm, self.mass = o, min(max_past, 15), mvm, mass
+        self.s = []
+
+    def store(self, x):
+        self.s.insert(0, np.array(x, dtype=np.float64, copy=True))
+        self.s = self.s[: self.max]
+
+    def _c(self, a):
+        return spinor_to_act(self.o, a)[: self.o.VH].reshape(-1)
+
+    def guess(self, b):
+        o = self.o
+        n = len(self.s)
+        out = np.zeros((o.V, 8 * o.dr))
+        if n == 0:
+            return out
+        S = [self._c(v) for v in self.s]
+        for i in range(n):
+            S[i] = S[i] / np.sqrt(np.vdot(S[i], S[i]).real)
+            for j in range(i + 1, n):
+                rij = np.sum(S[i] * np.conj(S[j]))
+                S[j] = S[j] - rij * S[i]
+        A = np.zeros((n, n), dtype=complex)
+        rhs = np.zeros(n, dtype=complex)
+        bc = self._c(b)
+        for i in range(n):
+            si = np.zeros((o.V, 1, 4, o.dr), dtype=complex)
+            si[: o.VH] = S[i].reshape(o.VH, 1, 4, o.dr)
+            d = np.zeros((o.V, 8 * o.dr))
+            o.mvm(self.mvm, self.mass, d, act_to_spinor(o, si))
+            mi = self._c(d)
+            for j in range(n):
+                A[j, i] = np.sum(mi * np.conj(S[j]))
+            rhs[i] = np.sum(bc * np.conj(S[i]))
+        x = np.linalg.solve(A, rhs)
+        g = sum(xk * sk for xk, sk in zip(x, S))
+        ga = np.zeros((o.V, 1, 4, o.dr), dtype=complex)
+        ga[: o.VH] = g.reshape(o.VH, 1, 4, o.dr)
+        return act_to_spinor(o, ga)

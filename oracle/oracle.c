@@ -1670,3 +1670,52 @@ void or_ptc_wgrad(int cin, int cout, const double *gd, const double *hd, double 
 		free(loc);
 	}
 }
+
+/* ------------------------------------------------------------------ */
+/* minres_inverter: library/minres.cpp:12-118 (Lanczos + Givens, for a  */
+/* Hermitian mvm; real Lanczos coefficients)                            */
+/* ------------------------------------------------------------------ */
+int or_minres(int mvm, double mass, double prec, double *dptr, const double *sptr, double *resid_out)
+{
+	size_t n = (size_t)V * NSP;
+	double *v0 = calloc(n, sizeof(double)), *v1 = calloc(n, sizeof(double));
+	double *w0 = calloc(n, sizeof(double)), *w1 = calloc(n, sizeof(double)), *t = calloc(n, sizeof(double));
+	double g0 = 1, g1 = 1, s0 = 0, s1 = 0;
+	int it = 0;
+	apply_mvm(mvm, mass, t, dptr);
+	copy_full(v1, sptr);
+	mulr_add(v1, -1.0, t);
+	double norm = or_sqnorm(sptr), residue = or_sqnorm(v1);
+	double beta0 = sqrt(residue), eta = beta0;
+	if (residue < norm * prec) it = -1;
+	while (++it) {
+		mulr(v1, 1.0 / beta0);
+		apply_mvm(mvm, mass, t, v1);
+		double alpha = or_product_re(v1, t);
+		mulr_add(t, -alpha, v1);
+		mulr_add(t, -beta0, v0);
+		/* rotate (v0, v1, t) <- (v1, t, v0) */
+		double *p = v0; v0 = v1; v1 = t; t = p;
+		double beta1 = sqrt(or_sqnorm(v1));
+		double delta = g1 * alpha - g0 * s1 * beta0;
+		double rho1 = sqrt(delta * delta + beta1 * beta1);
+		double rho2 = s1 * alpha + g0 * g1 * beta0;
+		double rho3 = s0 * beta0;
+		g0 = g1; g1 = delta / rho1;
+		s0 = s1; s1 = beta1 / rho1;
+		/* t = (v0 - rho3 w0 - rho2 w1) / rho1 */
+		memset(t, 0, sizeof(double) * n);
+		mulr_add(t, 1.0 / rho1, v0);
+		mulr_add(t, -rho3 / rho1, w0);
+		mulr_add(t, -rho2 / rho1, w1);
+		p = w0; w0 = w1; w1 = t; t = p;
+		mulr_add(dptr, g1 * eta, w1);
+		eta = -s1 * eta;
+		beta0 = beta1;
+		residue = eta * eta;
+		if (residue < norm * prec) break;
+	}
+	if (resid_out) *resid_out = residue / norm;
+	free(v0); free(v1); free(w0); free(w1); free(t);
+	return it;
+}

@@ -146,6 +146,13 @@ _SIGS = {
     "yh_cg_net": ([_VP, ctypes.c_void_p, _VP, _VP, _VP, _PI, _PI], ctypes.c_int),
     "yh_tune": ([_VP, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "yh_autotune": ([_VP, ctypes.c_int, _PI], ctypes.c_int),
+    "yh_minres": ([_VP, ctypes.c_void_p, _VP, _VP, _PI], ctypes.c_int),
+    "yh_auto_inverter": ([_VP, ctypes.c_void_p, _VP, _VP, _PI], ctypes.c_int),
+    "yh_mre_create": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.POINTER(_VP)], ctypes.c_int),
+    "yh_mre_destroy": ([_VP], ctypes.c_int),
+    "yh_mre_reset": ([_VP], ctypes.c_int),
+    "yh_mre_store": ([_VP, _VP], ctypes.c_int),
+    "yh_mre_guess": ([_VP, _VP, _VP], ctypes.c_int),
 }
 
 
@@ -397,6 +404,21 @@ class Lattice:
         _check(lib().yh_rational_calc(self.ctx, ctypes.byref(rat), ctypes.byref(par), d.h, s.h, ctypes.byref(it)))
         return it.value
 
+    def minres_inverter(self, par, d, s):
+        """minres_inverter (library/minres.cpp:12-118); d holds the guess."""
+        it = ctypes.c_int(0)
+        _check(lib().yh_minres(self.ctx, ctypes.byref(par), d.h, s.h, ctypes.byref(it)))
+        return it.value
+
+    def auto_inverter(self, par, d, s):
+        """auto_inverter (library/inverter.cpp:9-55)."""
+        it = ctypes.c_int(0)
+        _check(lib().yh_auto_inverter(self.ctx, ctypes.byref(par), d.h, s.h, ctypes.byref(it)))
+        return it.value
+
+    def mre(self, max_past, mvm="Dphi", mass=0.0):
+        return MRE(self, max_past, mvm, mass)
+
     def cg_mixed(self, par, d, s):
         it = ctypes.c_int(0)
         up = ctypes.c_int(0)
@@ -606,6 +628,38 @@ class Net:
     def free(self):
         if self.h:
             lib().yh_net_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class MRE:
+    """mre_par with mre_init / mre_guess / mre_store / mre_reset
+    (library/mre.cpp, include/mre.h)."""
+
+    def __init__(self, lat, max_past, mvm="Dphi", mass=0.0):
+        self.lat = lat
+        h = _VP()
+        _check(lib().yh_mre_create(lat.ctx, max_past, MVM[mvm] if isinstance(mvm, str) else mvm, mass,
+                                   ctypes.byref(h)))
+        self.h = h
+
+    def guess(self, d, s):
+        _check(lib().yh_mre_guess(self.h, d.h, s.h))
+
+    def store(self, x):
+        _check(lib().yh_mre_store(self.h, x.h))
+
+    def reset(self):
+        _check(lib().yh_mre_reset(self.h))
+
+    def free(self):
+        if self.h:
+            lib().yh_mre_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -384,7 +384,8 @@ int yh_destroy(yh_ctx* c) {
     if (c->clov[p]) (void)hipFree(c->clov[p]);
     if (c->clinv[p]) (void)hipFree(c->clinv[p]);
   }
-  if (c->tmp[5]) { (void)hipFree(c->tmp[5]->mem); delete static_cast<yh_field*>(c->tmp[5]); }
+  for (int i = 4; i < 6; i++)
+    if (c->tmp[i]) { (void)hipFree(c->tmp[i]->mem); delete static_cast<yh_field*>(c->tmp[i]); }
   if (c->fund) (void)hipFree(c->fund);
   if (c->logdet_site) (void)hipFree(c->logdet_site);
   if (c->mom) (void)hipFree(c->mom);

@@ -1,6 +1,10 @@
-// BiCGstab, multishift CG and the mixed-precision reliable-update CG.
+// BiCGstab, MINRES, auto_inverter, MRE, multishift CG and the
+// mixed-precision reliable-update CG.
 //
 // bicgstab_inverter : library/bicgstab.cpp:9-97
+// minres_inverter   : library/minres.cpp:12-118
+// auto_inverter     : library/inverter.cpp:9-55
+// mre_*             : library/mre.cpp:15-117 (+ the pivoted LU of solver.cpp:4-95)
 // cg_mshift         : library/cg.cpp:11-130
 // cg (mixed)        : project-new; FP64 true residual b - A x recomputed with
 //                     the FP64 operator at every reliable update, sloppy
@@ -325,6 +329,262 @@ int yh_cg_mixed(yh_ctx* cc, const yh_inv_par* par, yh_field* dptr, const yh_fiel
   if (par->log)
     fprintf(stderr, "[INVERTER][NOTICE] cg_mixed: iterations = %d, updates = %d, precision = %1.4e\n", total, nupd,
             rr / norm_b);
+  return sync(c);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int yh_minres(yh_ctx* cc, const yh_inv_par* par, yh_field* dptr, const yh_field* sptr, int* iters) {
+  Ctx* c = cc;
+  if (!c || !par || !dptr || !sptr || !iters) { set_error("null argument"); return YH_ERR_ARG; }
+  if (!c->gauge_set) { set_error("no gauge field"); return YH_ERR_STATE; }
+  if (dptr->prec != PREC_F64 || sptr->prec != PREC_F64) { set_error("solver fields must be FP64"); return YH_ERR_ARG; }
+  if (!(par->prec > 0.0)) { set_error("prec must be positive"); return YH_ERR_ARG; }
+  if (par->mvm == YH_MVM_DPHI) { set_error("minres needs a Hermitian operator (Hphi or Hphi_sq)"); return YH_ERR_ARG; }
+  int rc;
+  Field *v0, *v1, *w0, *w1, *t;
+  if ((rc = need(c, PREC_F64, 2, &v0)) || (rc = need(c, PREC_F64, 3, &v1)) || (rc = need(c, PREC_F64, 4, &w0)) ||
+      (rc = need(c, PREC_F64, 5, &w1)) || (rc = need(c, PREC_F64, 6, &t)))
+    return rc;
+  const int recon = default_recon(c, PREC_F64);
+  Field* x = dptr;
+  // v1 = b - A x ; v0 = w0 = w1 = 0
+  if ((rc = apply_op(c, PREC_F64, recon, par->mvm, par->mass, t, x, nullptr, -1, nullptr))) return rc;
+  if ((rc = blas_copy(c, v1, sptr, 0)) || (rc = blas_axpby(c, v1, -1.0, t, 1.0, 0))) return rc;
+  if ((rc = field_zero(c, v0, YH_EVEN)) || (rc = field_zero(c, w0, YH_EVEN)) || (rc = field_zero(c, w1, YH_EVEN)))
+    return rc;
+  const double norm = sqnorm(c, sptr, &rc);
+  if (rc) return rc;
+  double residue = sqnorm(c, v1, &rc);
+  if (rc) return rc;
+  double beta0 = sqrt(residue), eta = beta0;
+  double g0 = 1, g1 = 1, s0 = 0, s1 = 0;
+  int it = 0;
+  if (residue < norm * par->prec) it = -1;
+  while (++it) {
+    // Lanczos step: v1 <- v1 / beta0, t = A v1 - alpha v1 - beta0 v0
+    if ((rc = blas_axpby(c, v1, 0.0, v1, 1.0 / beta0, 0))) return rc;
+    if ((rc = apply_op(c, PREC_F64, recon, par->mvm, par->mass, t, v1, nullptr, -1, nullptr))) return rc;
+    double alpha = dot_re(c, v1, t, &rc);
+    if (rc) return rc;
+    if ((rc = blas_axpby(c, t, -alpha, v1, 1.0, 0)) || (rc = blas_axpby(c, t, -beta0, v0, 1.0, 0))) return rc;
+    std::swap(v0, v1);  // (v0, v1, t) <- (v1, t, v0)
+    std::swap(v1, t);
+    double beta1 = sqrt(sqnorm(c, v1, &rc));
+    if (rc) return rc;
+    // Givens rotation
+    double delta = g1 * alpha - g0 * s1 * beta0;
+    double rho1 = sqrt(delta * delta + beta1 * beta1);
+    double rho2 = s1 * alpha + g0 * g1 * beta0;
+    double rho3 = s0 * beta0;
+    g0 = g1;
+    g1 = delta / rho1;
+    s0 = s1;
+    s1 = beta1 / rho1;
+    // t = (v0 - rho3 w0 - rho2 w1) / rho1 ; (w0, w1, t) <- (w1, t, w0)
+    if ((rc = blas_axpby(c, t, 1.0 / rho1, v0, 0.0, 0)) || (rc = blas_axpby(c, t, -rho3 / rho1, w0, 1.0, 0)) ||
+        (rc = blas_axpby(c, t, -rho2 / rho1, w1, 1.0, 0)))
+      return rc;
+    std::swap(w0, w1);
+    std::swap(w1, t);
+    if ((rc = blas_axpby(c, x, g1 * eta, w1, 1.0, 0))) return rc;
+    eta = -s1 * eta;
+    beta0 = beta1;
+    residue = eta * eta;
+    if (!(residue == residue)) { set_error("minres: non-finite residual"); return YH_ERR_NONFINITE; }
+    if (residue < norm * par->prec) break;
+    if (par->max_iter > 0 && it >= par->max_iter) {
+      *iters = it;
+      set_error("minres: max_iter reached");
+      return YH_ERR_NOCONV;
+    }
+  }
+  *iters = it;
+  if (par->log) fprintf(stderr, "[INVERTER][NOTICE] minres: iterations = %d, precision = %1.4e\n", it, residue / norm);
+  return sync(c);
+}
+
+// auto_inverter: Hphi_sq -> two BiCGstab solves on Dphi with gamma5,
+// Hphi -> one, Dphi -> one.  The intermediate field persists between calls
+// and is the first solve's initial guess, as the reference's static
+// scratch field is.
+int yh_auto_inverter(yh_ctx* cc, const yh_inv_par* par, yh_field* dptr, yh_field* sptr, int* iters) {
+  Ctx* c = cc;
+  if (!c || !par || !dptr || !sptr || !iters) { set_error("null argument"); return YH_ERR_ARG; }
+  if (!c->gauge_set) { set_error("no gauge field"); return YH_ERR_STATE; }
+  int rc;
+  if (!c->tmp[4]) {
+    Field* f;
+    if ((rc = field_alloc(c, PREC_F64, &f))) return rc;
+    c->tmp[4] = f;
+  }
+  Field* tmp = c->tmp[4];
+  yh_inv_par p = *par;
+  p.mvm = YH_MVM_DPHI;
+  int it = 0, n = 0;
+  if (par->mvm == YH_MVM_HPHI_SQ) {
+    if ((rc = blas_g5(c, sptr, 0))) return rc;
+    rc = yh_bicgstab(cc, &p, static_cast<yh_field*>(tmp), sptr, &n);
+    it += n;
+    if (!rc) rc = blas_g5(c, tmp, 0);
+    if (!rc) rc = yh_bicgstab(cc, &p, dptr, static_cast<yh_field*>(tmp), &n);
+    it += n;
+    int rc2 = blas_g5(c, sptr, 0);
+    if (rc) return rc;
+    if (rc2) return rc2;
+  } else if (par->mvm == YH_MVM_HPHI) {
+    if ((rc = blas_g5(c, sptr, 0))) return rc;
+    rc = yh_bicgstab(cc, &p, dptr, sptr, &n);
+    it += n;
+    int rc2 = blas_g5(c, sptr, 0);
+    if (rc) return rc;
+    if (rc2) return rc2;
+  } else if (par->mvm == YH_MVM_DPHI) {
+    if ((rc = yh_bicgstab(cc, &p, dptr, sptr, &n))) return rc;
+    it += n;
+  } else {
+    set_error("auto_inverter: operator not supported");
+    return YH_ERR_ARG;
+  }
+  *iters = it;
+  return sync(c);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// MRE: minimal residual extrapolation of the initial guess from the last
+// `max` solutions (Brower, Ivanenko, Levkova, Orginos, hep-lat/9509012):
+// x0 = sum_k x_k s_k with the Galerkin condition s_j^+ (b - M x0) = 0 on
+// the span of the stored solutions.
+// ---------------------------------------------------------------------------
+struct yh_mre {
+  Ctx* c = nullptr;
+  int max = 0, num = 0, mvm = YH_MVM_DPHI;
+  double mass = 0;
+  std::vector<Field*> s;  // s[0] is the most recent solution
+  Field* out = nullptr;
+};
+
+namespace {
+
+// Gaussian elimination with partial pivoting on a small dense complex system
+int small_solve(int n, std::vector<zd> A, std::vector<zd> b, std::vector<zd>& x) {
+  for (int k = 0; k < n; k++) {
+    int piv = k;
+    for (int i = k + 1; i < n; i++)
+      if (std::abs(A[i * n + k]) > std::abs(A[piv * n + k])) piv = i;
+    if (std::abs(A[piv * n + k]) < 1e-14) { set_error("mre: singular Galerkin matrix"); return YH_ERR_NOCONV; }
+    if (piv != k) {
+      for (int j = 0; j < n; j++) std::swap(A[k * n + j], A[piv * n + j]);
+      std::swap(b[k], b[piv]);
+    }
+    for (int i = k + 1; i < n; i++) {
+      zd f = A[i * n + k] / A[k * n + k];
+      for (int j = k; j < n; j++) A[i * n + j] -= f * A[k * n + j];
+      b[i] -= f * b[k];
+    }
+  }
+  x.assign(n, zd(0, 0));
+  for (int i = n - 1; i >= 0; i--) {
+    zd v = b[i];
+    for (int j = i + 1; j < n; j++) v -= A[i * n + j] * x[j];
+    x[i] = v / A[i * n + i];
+  }
+  return YH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int yh_mre_create(yh_ctx* cc, int max, int mvm, double mass, yh_mre** out) {
+  Ctx* c = cc;
+  if (!c || !out || max < 0 || mvm < YH_MVM_DPHI || mvm > YH_MVM_HPHI_SQ) { set_error("bad argument"); return YH_ERR_ARG; }
+  yh_mre* m = new yh_mre();
+  m->c = c;
+  m->max = std::min(max, 15);  // MRE_MAX (include/mre.h:8)
+  m->mvm = mvm;
+  m->mass = mass;
+  int rc = YH_OK;
+  for (int i = 0; i < m->max && !rc; i++) {
+    Field* f = nullptr;
+    rc = field_alloc(c, PREC_F64, &f);
+    if (!rc) m->s.push_back(f);
+  }
+  if (!rc && m->max > 0) rc = field_alloc(c, PREC_F64, &m->out);
+  if (rc) {
+    yh_mre_destroy(m);
+    return rc;
+  }
+  *out = m;
+  return YH_OK;
+}
+
+int yh_mre_destroy(yh_mre* m) {
+  if (!m) return YH_OK;
+  for (Field* f : m->s) field_free(f);
+  field_free(m->out);
+  delete m;
+  return YH_OK;
+}
+
+int yh_mre_reset(yh_mre* m) {
+  if (!m) { set_error("null argument"); return YH_ERR_ARG; }
+  m->num = 0;
+  return YH_OK;
+}
+
+// mre_store: the oldest slot becomes the newest and receives x (even half)
+int yh_mre_store(yh_mre* m, const yh_field* x) {
+  if (!m || !x) { set_error("null argument"); return YH_ERR_ARG; }
+  if (m->max == 0) return YH_OK;
+  std::rotate(m->s.begin(), m->s.end() - 1, m->s.end());
+  int rc = blas_copy(m->c, m->s[0], x, 0);
+  if (rc) return rc;
+  m->num++;
+  return sync(m->c);
+}
+
+// mre_guess: dst = extrapolated guess for M dst = b (dst = 0 without history)
+int yh_mre_guess(yh_mre* m, yh_field* dst, const yh_field* b) {
+  if (!m || !dst || !b) { set_error("null argument"); return YH_ERR_ARG; }
+  Ctx* c = m->c;
+  if (m->max == 0) return YH_OK;
+  int rc;
+  const int n = std::min(m->num, m->max);
+  if ((rc = field_zero(c, dst, YH_EVEN))) return rc;
+  if (n == 0) return sync(c);
+  // Gram-Schmidt on the stored solutions (mre.cpp:15-33); the span, and so
+  // the Galerkin solution, is what matters
+  for (int i = 0; i < n; i++) {
+    double nn = sqnorm(c, m->s[i], &rc);
+    if (rc) return rc;
+    if ((rc = blas_cscale(c, m->s[i], 1.0 / sqrt(nn), 0.0, 0))) return rc;
+    for (int j = i + 1; j < n; j++) {
+      zd r = dot(c, m->s[i], m->s[j], &rc);  // sum s_i conj(s_j)
+      if (rc) return rc;
+      // subtract the projection of s_j on s_i: (s_i^+ s_j) s_i = conj(r) s_i
+      if ((rc = blas_caxpy(c, m->s[j], -r.real(), r.imag(), m->s[i], 0))) return rc;
+    }
+  }
+  const int recon = default_recon(c, PREC_F64);
+  std::vector<zd> A(n * n), rhs(n), xs;
+  for (int i = 0; i < n; i++) {
+    if ((rc = apply_op(c, PREC_F64, recon, m->mvm, m->mass, m->out, m->s[i], nullptr, -1, nullptr))) return rc;
+    for (int j = 0; j < n; j++) {
+      A[j * n + i] = dot(c, m->out, m->s[j], &rc);  // s_j^+ M s_i, stored as sum (M s_i) conj(s_j)
+      if (rc) return rc;
+    }
+    rhs[i] = dot(c, b, m->s[i], &rc);  // s_i^+ b
+    if (rc) return rc;
+  }
+  if ((rc = small_solve(n, A, rhs, xs))) return rc;
+  for (int k = 0; k < n; k++)
+    if ((rc = blas_caxpy(c, dst, xs[k].real(), xs[k].imag(), m->s[k], 0))) return rc;
   return sync(c);
 }
 
