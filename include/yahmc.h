@@ -256,6 +256,26 @@ int yh_plaquette(yh_ctx* ctx, double* out);
 /* momentum part of hamiltonian (update.cpp:201-216): (TF/2) sum |P|^2 */
 int yh_momenta_energy(yh_ctx* ctx, double* out);
 
+/* --- gauge configuration files (host only; no device involved) ------------
+ * YH_CFG_NATIVE: gauge_field_save / gauge_field_load (library/gaugefield.cpp:
+ *   44-179; header_t of include/gaugefield.h:20-36, host byte order);
+ * YH_CFG_HIREP: gauge_field_save_hirep / gauge_field_load_hirep
+ *   (library/hirep.cpp:56-212, big-endian).
+ * links: 4V suNg in the gfield_fund layout.  yh_config_read with links ==
+ * NULL reads the header only (to size the array).  HiRep files carry no
+ * representation (repr = -1) and take num_cnfg from the last "n%d" of the
+ * file name, as the reference does. */
+#define YH_CFG_NATIVE 0
+#define YH_CFG_HIREP 1
+typedef struct {
+  int ident, version;
+  int dims[4]; /* T, X, Y, Z */
+  int nc, repr, num_cnfg, num_accept;
+  double plaquette;
+} yh_config_info;
+int yh_config_write(const char* path, int format, const yh_config_info* info, const double* links);
+int yh_config_read(const char* path, int format, yh_config_info* info, double* links);
+
 /* --- learned preconditioner (no counterpart in the reference) --------------
  * A gauge-covariant parallel-transport convolution network on the even
  * sublattice, A = P_e L_n ... L_1 E with

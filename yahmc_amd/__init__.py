@@ -37,6 +37,16 @@ class YahmcError(RuntimeError):
         self.code = code
 
 
+class ConfigInfo(ctypes.Structure):
+    """yh_config_info: the fields of header_t (include/gaugefield.h:24-36)."""
+    _fields_ = [("ident", ctypes.c_int), ("version", ctypes.c_int), ("dims", ctypes.c_int * 4), ("nc", ctypes.c_int),
+                ("repr", ctypes.c_int), ("num_cnfg", ctypes.c_int), ("num_accept", ctypes.c_int),
+                ("plaquette", ctypes.c_double)]
+
+
+CFG = {"native": 0, "hirep": 1}
+
+
 class Rational(ctypes.Structure):
     """yh_rational: the fields of rational_app (include/rational.h:7-18)."""
     _fields_ = [("n", ctypes.c_int), ("d", ctypes.c_int), ("order", ctypes.c_int), ("prec", ctypes.c_double),
@@ -146,6 +156,8 @@ _SIGS = {
     "yh_cg_net": ([_VP, ctypes.c_void_p, _VP, _VP, _VP, _PI, _PI], ctypes.c_int),
     "yh_tune": ([_VP, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "yh_autotune": ([_VP, ctypes.c_int, _PI], ctypes.c_int),
+    "yh_config_write": ([ctypes.c_char_p, ctypes.c_int, ctypes.c_void_p, _D], ctypes.c_int),
+    "yh_config_read": ([ctypes.c_char_p, ctypes.c_int, ctypes.c_void_p, _D], ctypes.c_int),
     "yh_minres": ([_VP, ctypes.c_void_p, _VP, _VP, _PI], ctypes.c_int),
     "yh_auto_inverter": ([_VP, ctypes.c_void_p, _VP, _VP, _PI], ctypes.c_int),
     "yh_mre_create": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.POINTER(_VP)], ctypes.c_int),
@@ -158,6 +170,30 @@ _SIGS = {
 
 def exported_symbols():
     return sorted(_SIGS)
+
+
+def config_write(path, links, dims, nc, repr=0, plaquette=0.0, num_cnfg=0, num_accept=0, format="native"):
+    """gauge_field_save / gauge_field_save_hirep: links (4V, 2 NC^2) in the
+    gfield_fund layout (EO order)."""
+    info = ConfigInfo()
+    for i in range(4):
+        info.dims[i] = int(dims[i])
+    info.nc, info.repr, info.num_cnfg, info.num_accept, info.plaquette = nc, repr, num_cnfg, num_accept, plaquette
+    links = np.ascontiguousarray(links, dtype=np.float64)
+    assert links.size == 4 * int(np.prod(dims)) * 2 * nc * nc
+    _check(lib().yh_config_write(str(path).encode(), CFG[format], ctypes.byref(info), _dp(links)))
+
+
+def config_read(path, format="native"):
+    """gauge_field_load / gauge_field_load_hirep: returns (info dict, links)."""
+    info = ConfigInfo()
+    _check(lib().yh_config_read(str(path).encode(), CFG[format], ctypes.byref(info), None))
+    dims = tuple(info.dims)
+    nc = info.nc
+    links = np.zeros((4 * int(np.prod(dims)), 2 * nc * nc))
+    _check(lib().yh_config_read(str(path).encode(), CFG[format], ctypes.byref(info), _dp(links)))
+    return {"dims": dims, "nc": nc, "repr": info.repr, "num_cnfg": info.num_cnfg, "num_accept": info.num_accept,
+            "plaquette": info.plaquette}, links
 
 
 def rational_zolotarev(order, n, d, xmin, xmax):
@@ -262,6 +298,7 @@ class Lattice:
         self.nc = nc
         self.csw = float(csw)
         self.bc = lt.bc
+        self.repr_id = lt.repr
         self.dr = lib().yh_repr_dim(h)
         self.V = int(np.prod(self.dims))
         self.VH = self.V // 2
@@ -403,6 +440,23 @@ class Lattice:
         it = ctypes.c_int(0)
         _check(lib().yh_rational_calc(self.ctx, ctypes.byref(rat), ctypes.byref(par), d.h, s.h, ctypes.byref(it)))
         return it.value
+
+    def save_config(self, path, format="native", num_cnfg=0, num_accept=0):
+        """gauge_field_save (gaugefield.cpp:44-115) of the device links."""
+        config_write(path, self.get_gauge(), self.dims, self.nc, self.repr_id, self.plaquette(), num_cnfg, num_accept,
+                     format)
+
+    def load_config(self, path, format="native"):
+        """gauge_field_load (gaugefield.cpp:117-179): checks the header
+        against this lattice, uploads and represents the links; returns the
+        header and |stored plaquette - recomputed plaquette|."""
+        info, links = config_read(path, format)
+        bad = tuple(info["dims"]) != self.dims or info["nc"] != self.nc or \
+            (format == "native" and info["repr"] != self.repr_id)
+        if bad:
+            raise YahmcError(-1, f"configuration {path} is not consistent with this lattice")
+        self.set_gauge(links)
+        return info, abs(info["plaquette"] - self.plaquette())
 
     def minres_inverter(self, par, d, s):
         """minres_inverter (library/minres.cpp:12-118); d holds the guess."""
