@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from fields import random_gauge
-from oracle import BC_ANTIPERIODIC, MVM_HPHI_SQ, Oracle
+from oracle import BC_ANTIPERIODIC, MVM_HPHI, MVM_HPHI_SQ, Oracle
 
 pytestmark = pytest.mark.gpu
 yh = pytest.importorskip("yahmc_amd")
@@ -108,3 +108,154 @@ def test_trajectory_matches_oracle(model, nc, rep):
     o.set_gauge(links_o)
     o.reunitarize()
     assert np.abs(lat_links - o.get_gauge()).max() < 1e-10
+
+
+class OracleHasenbusch:
+    """MonomialHB (monomials/hasenbusch.cpp, CLOVER_TERM_EO branch) on the
+    oracle, inverses by auto_inverter as the reference does."""
+
+    def __init__(self, o, mass, dm, prec):
+        self.o, self.om, self.sm, self.prec = o, mass, mass + dm, prec
+
+    def reset(self, eta):
+        o = self.o
+        self.la = float(np.sum(eta * eta))
+        a = np.zeros_like(eta)
+        o.Hphi(self.om, a, eta.copy())
+        self.pf = np.zeros_like(eta)
+        o.auto_inverter(MVM_HPHI, self.sm, self.prec, self.pf, a)
+        return self.la
+
+    def action(self):
+        o = self.o
+        a, b = np.zeros_like(self.pf), np.zeros_like(self.pf)
+        o.Hphi(self.sm, a, self.pf.copy())
+        o.auto_inverter(MVM_HPHI, self.om, self.prec, b, a)
+        return o.sqnorm(b)
+
+    def update(self, dt):
+        o = self.o
+        a, y = np.zeros_like(self.pf), np.zeros_like(self.pf)
+        o.Hphi(self.sm, a, self.pf.copy())
+        o.auto_inverter(MVM_HPHI_SQ, self.om, self.prec, y, a)
+        o.wilson_fermion_force_q(dt, self.om, 1.0, y.copy())
+        o.wilson_fermion_force(dt, self.sm, -1.0, self.pf, y)
+
+
+def test_hasenbusch_trajectory_matches_oracle():
+    """Gauge + two-flavour monomial at m + dm + Hasenbusch ratio (m, dm) on
+    level 1: dH and plaquette against the oracle restatement."""
+    print("\n[case] hmc + hasenbusch trajectory", flush=True)
+    from oracle import MVM_HPHI  # noqa: F401
+    from yahmc_amd.hmc import HasenbuschMonomial
+    dims = (4, 4, 4, 8)
+    beta, mass, dm, prec, length, nf, ng = 5.6, -0.25, 0.2, 1e-26, 0.3, 2, 1
+    o = Oracle("su3_fund").init(dims, bc=BC_ANTIPERIODIC, csw=1.0)
+    rng = np.random.default_rng(29)
+    links = random_gauge(rng, o.V, 3)
+    P = rng.normal(0.0, np.sqrt(0.5), size=(4 * o.V, 8))
+    eta1 = np.zeros((o.V, 24))
+    eta1[: o.VH] = rng.normal(0.0, np.sqrt(0.5), size=(o.VH, 24))
+    eta2 = np.zeros((o.V, 24))
+    eta2[: o.VH] = rng.normal(0.0, np.sqrt(0.5), size=(o.VH, 24))
+
+    # oracle: HMC monomial at m + dm and Hasenbusch (m, dm)
+    o.set_gauge(links)
+    o.represent()
+    o.set_momenta(P)
+    hm = mass + dm
+    la = float(np.sum(eta1 * eta1)) - 2.0 * o.clover_logdet(hm)
+    pf = np.zeros_like(eta1)
+    o.Dphi(hm, pf, eta1.copy())
+    g5pf = pf.copy()
+    o.g5(g5pf)
+    hb = OracleHasenbusch(o, mass, dm, prec)
+    h0 = o.kinetic() + o.gauge_action(beta) + la + hb.reset(eta2)
+
+    def momenta(dt, level):
+        if level == 0:
+            o.wilson_gauge_force(dt, beta)
+            return
+        o.begin_fermion_force()
+        y = np.zeros_like(eta1)
+        o.cg(MVM_HPHI_SQ, hm, prec, y, g5pf.copy())
+        o.wilson_fermion_force_q(dt, hm, 1.0, y)
+        o.clover_logdet_force(hm, 2.0)
+        hb.update(dt)
+        o.end_fermion_force(dt)
+
+    def integrate(length, level):
+        steps = ng if level == 0 else nf
+        dt = length / steps
+        for n in range(steps):
+            momenta(A * dt if n == 0 else 2 * A * dt, level)
+            fields(dt / 2, level)
+            momenta((1 - 2 * A) * dt, level)
+            fields(dt / 2, level)
+        momenta(A * dt, level)
+
+    def fields(dt, level):
+        if level == 0:
+            o.update_links(dt)
+        else:
+            integrate(dt, level - 1)
+            o.represent()
+
+    integrate(length, 1)
+    o.represent()
+    h1 = o.kinetic() + o.gauge_action(beta) + o.hmc_action(hm, prec, pf.copy()) + hb.action()
+    dh_o = h1 - h0
+
+    class Rng:
+        def __init__(self):
+            self.k = 0
+
+        def normal(self, loc, scale, size):
+            if size == P.shape:
+                return P
+            self.k += 1
+            return (eta1 if self.k == 1 else eta2)[: size[0]]
+
+        def uniform(self):
+            return 0.0
+
+    lat = yh.Lattice(dims, nc=3, repr="fundamental", bc="antiperiodic", csw=1.0)
+    lat.set_gauge(links)
+    hmc = HMC(lat, [GaugeMonomial(beta), FermionMonomial(hm, prec=prec), HasenbuschMonomial(mass, dm, prec=prec)],
+              [("o2mn", ng), ("o2mn", nf)])
+    rec = hmc.trajectory(Rng(), length=length, metropolis=False)
+    print(f"dH device {rec['dH']:.10e} oracle {dh_o:.10e}")
+    assert abs(rec["dH"] - dh_o) < 1e-7 * max(1.0, abs(dh_o))
+    assert abs(rec["plaquette"] - o.plaquette()) < 1e-10
+
+
+def test_evlimits_bracket_dense_spectrum():
+    """evlimits (library/evlimit.cpp): power and inverse power iteration on
+    H^2 bracket the dense spectrum of a tiny lattice from inside, within
+    10% at the reference's stopping rule (relative change below 1e-3)."""
+    print("\n[case] evlimits", flush=True)
+    from fields import complex_to_spinor, spinor_to_complex
+    from yahmc_amd.hmc import evlimits
+    dims = (2, 2, 2, 4)
+    o = Oracle("su3_fund").init(dims, bc=BC_ANTIPERIODIC, csw=1.0)
+    rng = np.random.default_rng(41)
+    links = random_gauge(rng, o.V, 3)
+    o.set_gauge(links)
+    o.represent()
+    mass = -0.2
+    cols = []
+    for k in range(o.VH * 12):
+        e = np.zeros((o.V, 4, 3), dtype=complex)
+        e[: o.VH].reshape(-1)[k] = 1.0
+        h = np.zeros((o.V, 24))
+        o.Hphi_sq(mass, h, complex_to_spinor(e))
+        cols.append(spinor_to_complex(h, 3)[: o.VH].reshape(-1))
+    w = np.linalg.eigvalsh(np.stack(cols, axis=1))
+    lat = yh.Lattice(dims, nc=3, repr="fundamental", bc="antiperiodic", csw=1.0)
+    lat.set_gauge(links)
+    lo, hi = evlimits(lat, mass, np.random.default_rng(3))
+    print(f"evlimits [{lo:.6e}, {hi:.6e}], dense [{w.min():.6e}, {w.max():.6e}]")
+    # |H^2 b| <= lambda_max |b| and 1/|H^-2 b| >= lambda_min |b|: the
+    # estimates approach the ends from inside
+    assert 0.9 * w.max() < hi <= w.max() * (1 + 1e-9)
+    assert w.min() * (1 - 1e-4) <= lo < 1.1 * w.min()

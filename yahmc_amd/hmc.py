@@ -105,6 +105,112 @@ class FermionMonomial:
             lat.clover_logdet_force(self.omass, 2.0)
 
 
+class HasenbuschMonomial:
+    """Mass-preconditioning ratio det(H(m)^2) / det(H(m + dm)^2) for EO
+    clover fermions (MonomialHB, monomials/hasenbusch.cpp:8-96, the
+    CLOVER_TERM_EO branch).  Inverses of Hphi are taken as
+    H^-1 b = (H^2)^-1 H b with the mixed-precision CG on H^2."""
+
+    def __init__(self, mass, dm, prec=1e-24, level=1, sloppy="f32", max_iter=100000):
+        if level == 0:
+            raise ValueError("fermion monomials cannot be on integrator level 0 (update.cpp)")
+        self.omass = mass
+        self.smass = mass + dm
+        self.prec = prec
+        self.level = level
+        self.sloppy = sloppy
+        self.max_iter = max_iter
+        self.pf = None
+        self.la = -1.0
+        self.iterations = []
+
+    def _fields(self, lat):
+        if self.pf is None:
+            self.pf, self.a, self.b, self.eta = (lat.field("f64") for _ in range(4))
+
+    def _h2_inverse(self, lat, mass, dst, src):
+        dst.zero()
+        par = lat.inv_par(mass, self.prec, "Hphi_sq", sloppy=self.sloppy, max_iter=self.max_iter)
+        it, _ = lat.cg_mixed(par, dst, src)
+        self.iterations.append(it)
+
+    def _h_inverse(self, lat, mass, dst, src):
+        """dst = Hphi(mass)^-1 src = (H^2)^-1 H src"""
+        lat.Hphi(mass, self.b, src)
+        self._h2_inverse(lat, mass, dst, self.b)
+
+    def reset(self, lat, rng):
+        """pf = H(m + dm)^-1 H(m) eta (hasenbusch.cpp:29-42)."""
+        self._fields(lat)
+        eta = np.zeros((lat.V, 8 * lat.dr))
+        eta[: lat.V // 2] = rng.normal(0.0, np.sqrt(0.5), size=(lat.V // 2, 8 * lat.dr))
+        self.eta.upload(eta)
+        self.la = float(np.sum(eta * eta))
+        lat.Hphi(self.omass, self.a, self.eta)
+        self._h_inverse(lat, self.smass, self.pf, self.a)
+
+    def action(self, lat):
+        """|H(m)^-1 H(m + dm) pf|^2 (hasenbusch.cpp:44-68)."""
+        if self.la > 0:
+            v, self.la = self.la, -1.0
+            return v
+        lat.Hphi(self.smass, self.eta, self.pf)
+        self._h_inverse(lat, self.omass, self.a, self.eta)
+        return lat.spinor_sqnorm(self.a)
+
+    def update(self, lat, dt):
+        """hasenbusch.cpp:70-83: Y = H(m)^-2 H(m + dm) pf; force_q(m, +1, Y)
+        and force(m + dm, -1, pf, Y)."""
+        lat.Hphi(self.smass, self.eta, self.pf)
+        self._h2_inverse(lat, self.omass, self.a, self.eta)
+        lat.wilson_fermion_force_q(dt, self.omass, 1.0, self.a)
+        lat.wilson_fermion_force(dt, self.smass, -1.0, self.pf, self.a)
+
+
+def largest_eigenvalue(lat, mass, rng, mvm="Hphi_sq", tol=1e-3):
+    """Power iteration (library/evlimit.cpp:52-84)."""
+    b = lat.field("f64").upload(_gaussian_even(lat, rng))
+    t = lat.field("f64")
+    op = getattr(lat, mvm)
+    eig, old, rel = 0.0, 0.0, 1.0
+    while rel > tol:
+        op(mass, t, b)
+        nrm = np.sqrt(lat.spinor_sqnorm(t))
+        old, eig = eig, nrm
+        b.zero()
+        lat.spinor_mulr_add_assign(b, 1.0 / nrm, t)
+        rel = abs((eig - old) / eig)
+    return eig
+
+
+def smallest_eigenvalue(lat, mass, rng, mvm="Hphi_sq", tol=1e-3):
+    """Inverse power iteration with auto_inverter at precision 1e-6
+    (library/evlimit.cpp:9-50)."""
+    b = lat.field("f64").upload(_gaussian_even(lat, rng))
+    t = lat.field("f64")
+    par = lat.inv_par(mass, 1e-6, mvm)
+    eig, old, rel = 0.0, 0.0, 1.0
+    while rel > tol:
+        lat.auto_inverter(par, t, b)
+        nrm = np.sqrt(lat.spinor_sqnorm(t))
+        old, eig = eig, nrm
+        b.zero()
+        lat.spinor_mulr_add_assign(b, 1.0 / nrm, t)
+        rel = abs((eig - old) / eig)
+    return 1.0 / eig
+
+
+def evlimits(lat, mass, rng, mvm="Hphi_sq"):
+    """Spectral range [min, max] of mvm (evlimit.cpp:86-91)."""
+    return smallest_eigenvalue(lat, mass, rng, mvm), largest_eigenvalue(lat, mass, rng, mvm)
+
+
+def _gaussian_even(lat, rng):
+    s = np.zeros((lat.V, 8 * lat.dr))
+    s[: lat.V // 2] = rng.normal(0.0, np.sqrt(0.5), size=(lat.V // 2, 8 * lat.dr))
+    return s
+
+
 class HMC:
     """integrators: [(type, steps)] from level 0 (innermost) upwards, type in
     {"o2lf", "o2mn", "o4mn"}; monomials carry their level."""
