@@ -228,6 +228,7 @@ def main():
     ap.add_argument("--no-smooth", action="store_true", help="keep the hot random field (no gauge trajectories)")
     ap.add_argument("--skip-learned", action="store_true")
     ap.add_argument("--train-steps", type=int, default=120)
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default XCD chunk instead of tuning it")
     ap.add_argument("--skip-hmc", action="store_true")
     ap.add_argument("--skip-rhmc", action="store_true")
     ap.add_argument("--only-rhmc", action="store_true", help="the batched multishift workload only")
@@ -267,6 +268,11 @@ def main():
                      "seconds": round(time.perf_counter() - t0, 2)}
         log(f"smoothing: {smoothing}")
 
+    tuned = None
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        tuned = {f"{k[0]}_r{k[1]}": v for k, v in lat.autotune(reps=20).items()}
+        log(f"autotuned XCD chunks {tuned} ({time.perf_counter() - t0:.1f}s)")
     hop = {}
     variants = [(args.prec, 0 if args.prec == "f64" else 1)] if args.only_hop else \
         [("f64", 0), ("f64", 1), ("f32", 1), ("f16", 1)]
@@ -352,7 +358,8 @@ def main():
         "config": {"workload": f"hop D_eo, {args.prec} recon-{18 if args.prec == 'f64' else 12}, 32^4 SU(3) fundamental "
                                "clover, antiperiodic time",
                    "lattice": list(DIMS), "sites_per_step": lat.VH, "mass": MASS, "csw": CSW,
-                   "gauge": smoothing if smoothing else "random SU(3) links (hot), not smoothed"},
+                   "gauge": smoothing if smoothing else "random SU(3) links (hot), not smoothed",
+                   "xcd_chunk": tuned if tuned else "default (1/8 of a t-slice)"},
         "roofline": {"bound": "hbm", "achieved": head["GB_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(head["GB_s"] / HBM_PEAK_GBS, 4), "traffic": TRAFFIC_PER_LAUNCH.get(key),
                      "algorithmic_bytes_per_launch": algo_bytes},

@@ -146,6 +146,41 @@ __device__ __forceinline__ int xcd_block(int nblocks, int q) {
 }
 
 // ---------------------------------------------------------------------------
+// non-temporal (streaming) accesses for data a kernel touches once: the
+// links of a hop and its output field
+// ---------------------------------------------------------------------------
+template <typename V>
+__device__ __forceinline__ V ld_stream(const V* p) {
+  if constexpr (sizeof(V) == 16 && alignof(V) >= 16) {
+    typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+    u4 r = __builtin_nontemporal_load((const u4*)p);
+    V v;
+    __builtin_memcpy(&v, &r, 16);
+    return v;
+  } else if constexpr (sizeof(V) == 8) {
+    typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+    u2 r = __builtin_nontemporal_load((const u2*)p);
+    V v;
+    __builtin_memcpy(&v, &r, 8);
+    return v;
+  } else {
+    return *p;
+  }
+}
+
+template <typename V>
+__device__ __forceinline__ void st_stream(V* p, const V& v) {
+  if constexpr (sizeof(V) == 16 && alignof(V) >= 16) {
+    typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+    u4 r;
+    __builtin_memcpy(&r, &v, 16);
+    __builtin_nontemporal_store(r, (u4*)p);
+  } else {
+    *p = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // spinor storage traits
 // ---------------------------------------------------------------------------
 template <int PREC, int D>
@@ -211,18 +246,26 @@ __device__ __forceinline__ void load_spinor(Cx<R> (&s)[4][D], const void* base, 
   }
 }
 
-template <int PREC, int D, typename R>
+// NT: write with non-temporal stores (a field no later access of the same
+// kernel reads)
+template <int PREC, int D, typename R, bool NT = false>
 __device__ __forceinline__ void store_spinor(void* base, float* nrm, int site, int stride, const Cx<R> (&s)[4][D]) {
   if constexpr (PREC == PREC_F64) {
     double2* p = (double2*)base;
 #pragma unroll
-    for (int k = 0; k < 4 * D; k++) p[(size_t)k * stride + site] = make_double2(s[k / D][k % D].re, s[k / D][k % D].im);
+    for (int k = 0; k < 4 * D; k++) {
+      const double2 v = make_double2(s[k / D][k % D].re, s[k / D][k % D].im);
+      if constexpr (NT) st_stream(&p[(size_t)k * stride + site], v);
+      else p[(size_t)k * stride + site] = v;
+    }
   } else if constexpr (PREC == PREC_F32) {
     float4* p = (float4*)base;
 #pragma unroll
     for (int k = 0; k < 2 * D; k++) {
       Cx<R> a = s[(2 * k) / D][(2 * k) % D], b = s[(2 * k + 1) / D][(2 * k + 1) % D];
-      p[(size_t)k * stride + site] = make_float4((float)a.re, (float)a.im, (float)b.re, (float)b.im);
+      const float4 v = make_float4((float)a.re, (float)a.im, (float)b.re, (float)b.im);
+      if constexpr (NT) st_stream(&p[(size_t)k * stride + site], v);
+      else p[(size_t)k * stride + site] = v;
     }
   } else {
     uint4* p = (uint4*)base;
@@ -242,7 +285,9 @@ __device__ __forceinline__ void store_spinor(void* base, float* nrm, int site, i
         int c = 4 * k + q;
         w[q] = (unsigned)f2h((float)s[c / D][c % D].re * inv) | ((unsigned)f2h((float)s[c / D][c % D].im * inv) << 16);
       }
-      p[(size_t)k * stride + site] = make_uint4(w[0], w[1], w[2], w[3]);
+      const uint4 v = make_uint4(w[0], w[1], w[2], w[3]);
+      if constexpr (NT) st_stream(&p[(size_t)k * stride + site], v);
+      else p[(size_t)k * stride + site] = v;
     }
   }
 }
@@ -274,14 +319,19 @@ __device__ __forceinline__ void su3_third_row(Cx<R> (&u)[3][3]) {
   u[2][2] = conj(u[0][0] * u[1][1] - u[0][1] * u[1][0]);
 }
 
-template <int PREC, int RECON, int D, typename R>
+// NT: non-temporal loads (each link is read once per hop)
+template <int PREC, int RECON, int D, typename R, bool NT = false>
 __device__ __forceinline__ void load_link(Cx<R> (&u)[D][D], const void* base, int site, int stride) {
   typedef typename LinkStore<PREC, RECON, D>::vec V;
   const V* p = (const V*)base;
+  auto ld = [&](size_t i) -> V {
+    if constexpr (NT) return ld_stream(&p[i]);
+    else return p[i];
+  };
   if constexpr (RECON == RECON_18) {
 #pragma unroll
     for (int k = 0; k < D * D; k++) {
-      V v = p[(size_t)k * stride + site];
+      V v = ld((size_t)k * stride + site);
       if constexpr (PREC == PREC_F64 || PREC == PREC_F32) u[k / D][k % D] = Cx<R>((R)v.x, (R)v.y);
       else u[k / D][k % D] = Cx<R>((R)h2f(v & 0xffff), (R)h2f(v >> 16));
     }
@@ -291,20 +341,20 @@ __device__ __forceinline__ void load_link(Cx<R> (&u)[D][D], const void* base, in
     if constexpr (PREC == PREC_F64) {
 #pragma unroll
       for (int k = 0; k < 6; k++) {
-        double2 v = p[(size_t)k * stride + site];
+        double2 v = ld((size_t)k * stride + site);
         w[k / 3][k % 3] = Cx<R>((R)v.x, (R)v.y);
       }
     } else if constexpr (PREC == PREC_F32) {
 #pragma unroll
       for (int k = 0; k < 3; k++) {
-        float4 v = p[(size_t)k * stride + site];
+        float4 v = ld((size_t)k * stride + site);
         w[(2 * k) / 3][(2 * k) % 3] = Cx<R>((R)v.x, (R)v.y);
         w[(2 * k + 1) / 3][(2 * k + 1) % 3] = Cx<R>((R)v.z, (R)v.w);
       }
     } else {
 #pragma unroll
       for (int k = 0; k < 3; k++) {
-        uint2 v = p[(size_t)k * stride + site];
+        uint2 v = ld((size_t)k * stride + site);
         w[(2 * k) / 3][(2 * k) % 3] = Cx<R>((R)h2f(v.x & 0xffff), (R)h2f(v.x >> 16));
         w[(2 * k + 1) / 3][(2 * k + 1) % 3] = Cx<R>((R)h2f(v.y & 0xffff), (R)h2f(v.y >> 16));
       }

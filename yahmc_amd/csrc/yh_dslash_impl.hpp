@@ -75,7 +75,11 @@ __device__ __forceinline__ void hop_dir(Cx<R> (&acc)[4][D], const HopArgs& a, in
   }
   const LV* lbase = (const LV*)(FWD ? a.lf : a.lb) + (size_t)MU * LNV * a.g.VHpad;
   Cx<R> u[D][D];
-  load_link<PREC, RECON, D, R>(u, lbase, lidx, a.g.VHpad);
+  // FP64 hops stream the links (each is read once per hop) with non-temporal
+  // loads and write their output with non-temporal stores: measured faster
+  // for FP64 (326 -> 301 us, 18 reals, 32^4), neutral for FP32 and slower
+  // for FP16, which keep ordinary accesses
+  load_link<PREC, RECON, D, R, PREC == PREC_F64>(u, lbase, lidx, a.g.VHpad);
   Cx<R> r[2][D];
 #pragma unroll
   for (int i = 0; i < D; i++) {
@@ -180,7 +184,7 @@ __global__ void __launch_bounds__(256, hop_min_waves<PREC>()) k_hop(HopArgs a) {
       for (int i = 0; i < D; i++) acc[s][i] = sc * acc[s][i];
 
     if constexpr (EPI == EPI_PLAIN) {
-      store_spinor<PREC, D, R>(a.out, a.out_nrm, k, g.VHpad, acc);
+      store_spinor<PREC, D, R, PREC == PREC_F64>(a.out, a.out_nrm, k, g.VHpad, acc);
     } else if constexpr (EPI == EPI_CLOVINV) {
       if (a.aux) store_spinor<PREC, D, R>(a.aux, a.aux_nrm, k, g.VHpad, acc);
       constexpr int N = 2 * D;
@@ -196,7 +200,7 @@ __global__ void __launch_bounds__(256, hop_min_waves<PREC>()) k_hop(HopArgs a) {
 #pragma unroll
         for (int i = 0; i < D; i++) { o[2 * blk2][i] = yout[i]; o[2 * blk2 + 1][i] = yout[D + i]; }
       }
-      store_spinor<PREC, D, R>(a.out, a.out_nrm, k, g.VHpad, o);
+      store_spinor<PREC, D, R, PREC == PREC_F64>(a.out, a.out_nrm, k, g.VHpad, o);
     } else {
       constexpr int N = 2 * D;
       Cx<R> xs[4][D];
@@ -227,7 +231,7 @@ __global__ void __launch_bounds__(256, hop_min_waves<PREC>()) k_hop(HopArgs a) {
           acc[3][i] = Cx<R>(-acc[3][i].re, -acc[3][i].im);
         }
       }
-      store_spinor<PREC, D, R>(a.out, a.out_nrm, k, g.VHpad, acc);
+      store_spinor<PREC, D, R, PREC == PREC_F64>(a.out, a.out_nrm, k, g.VHpad, acc);
       if (a.reduce) {
         if constexpr (PREC == PREC_F16) {
           // reduce what was stored (the rounded field), as a consumer sees it
