@@ -256,6 +256,19 @@ int yh_plaquette(yh_ctx* ctx, double* out);
 /* momentum part of hamiltonian (update.cpp:201-216): (TF/2) sum |P|^2 */
 int yh_momenta_energy(yh_ctx* ctx, double* out);
 
+/* --- spectroscopy (observables/propagator.cpp, observables/mesons.cpp) ------ */
+/* calc_propagator (propagator.cpp:75-166): full-lattice D dst = src through
+ * the EO Schur complement; the even solve is the mixed-precision CG on H^2
+ * to par->prec (par->mvm is ignored); both parities of src are read */
+int yh_calc_propagator(yh_ctx* ctx, const yh_inv_par* par, yh_field* dst, yh_field* src, int* iters);
+/* contract_all_correlators (mesons.cpp:55-147) for one source slice tau:
+ * corr[16][T] (overwritten), channels id, g1, g2, g3, g5, g5g1, g5g2, g5g3,
+ * g0g1, g0g2, g0g3, g0g5, g0g5g1, g0g5g2, g0g5g3, g5_g0g5; the time index is
+ * (t - tau) mod T; no sign or normalisation (print_correlator applies them) */
+#define YH_MESON_WALL 1  /* SEMWALL: 4 propagators, one per source spin */
+#define YH_MESON_POINT 2 /* POINT: 4 D_R propagators, index spin * D_R + colour */
+int yh_meson_contract(yh_ctx* ctx, int method, int nprop, const yh_field* const* props, int tau, double* corr);
+
 /* --- gauge configuration files (host only; no device involved) ------------
  * YH_CFG_NATIVE: gauge_field_save / gauge_field_load (library/gaugefield.cpp:
  *   44-179; header_t of include/gaugefield.h:20-36, host byte order);

@@ -158,6 +158,8 @@ _SIGS = {
     "yh_autotune": ([_VP, ctypes.c_int, _PI], ctypes.c_int),
     "yh_config_write": ([ctypes.c_char_p, ctypes.c_int, ctypes.c_void_p, _D], ctypes.c_int),
     "yh_config_read": ([ctypes.c_char_p, ctypes.c_int, ctypes.c_void_p, _D], ctypes.c_int),
+    "yh_calc_propagator": ([_VP, ctypes.c_void_p, _VP, _VP, _PI], ctypes.c_int),
+    "yh_meson_contract": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_VP), ctypes.c_int, _D], ctypes.c_int),
     "yh_minres": ([_VP, ctypes.c_void_p, _VP, _VP, _PI], ctypes.c_int),
     "yh_auto_inverter": ([_VP, ctypes.c_void_p, _VP, _VP, _PI], ctypes.c_int),
     "yh_mre_create": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.POINTER(_VP)], ctypes.c_int),
@@ -440,6 +442,20 @@ class Lattice:
         it = ctypes.c_int(0)
         _check(lib().yh_rational_calc(self.ctx, ctypes.byref(rat), ctypes.byref(par), d.h, s.h, ctypes.byref(it)))
         return it.value
+
+    def calc_propagator(self, par, d, s):
+        """calc_propagator (observables/propagator.cpp:75-166): full-lattice
+        D d = s; returns the CG iterations of the even solve."""
+        it = ctypes.c_int(0)
+        _check(lib().yh_calc_propagator(self.ctx, ctypes.byref(par), d.h, s.h, ctypes.byref(it)))
+        return it.value
+
+    def meson_contract(self, method, props, tau):
+        """16 x T correlators of one source slice (mesons.cpp:55-147)."""
+        arr = (_VP * len(props))(*[p.h for p in props])
+        out = np.zeros((16, self.dims[0]))
+        _check(lib().yh_meson_contract(self.ctx, method, len(props), arr, tau, _dp(out)))
+        return out
 
     def save_config(self, path, format="native", num_cnfg=0, num_accept=0):
         """gauge_field_save (gaugefield.cpp:44-115) of the device links."""

@@ -368,6 +368,7 @@ int yh_create(const yh_lattice* lat, yh_ctx** out) {
   int rc = gauge_init_basis(c);
   if (!rc) rc = force_init_constants(c);
   if (!rc) rc = md_init_constants(c);
+  if (!rc) rc = meson_init_tables();
   if (rc) { yh_destroy(c); return rc; }
   *out = c;
   return YH_OK;

@@ -157,6 +157,9 @@ int md_reunitarize(Ctx* c);
 int md_plaquette(Ctx* c, double* out);
 int md_kinetic(Ctx* c, double* out);
 
+// ---- spectroscopy -----------------------------------------------------------
+int meson_init_tables();
+
 // ---- learned preconditioner ---------------------------------------------------
 struct Net;
 int net_precondition(Net* n, Field* dst, const Field* src);
