@@ -203,7 +203,13 @@ typedef struct {
 /* Zolotarev's optimal approximation of x^(+-1/2) of the given order
  * (replaces the table lookup of rational_find, rational.cpp:56-132) */
 int yh_rational_zolotarev(int order, int n, int d, double min, double max, yh_rational* out);
-/* lowest order with maximum relative error below prec (rational_find) */
+/* best (minimax relative error) approximation of x^(n/d), 0 < |n/d| < 1,
+ * of the given order by the Remez exchange, e.g. the x^(+-1/4) entries of
+ * the reference's table (include/approx.h); positive powers are the
+ * reciprocal of the negative one (rational.cpp:111-122) */
+int yh_rational_remez(int order, int n, int d, double min, double max, yh_rational* out);
+/* lowest order with maximum relative error below prec (rational_find,
+ * rational.cpp:56-132): Zolotarev for x^(+-1/2), Remez otherwise */
 int yh_rational_find(int n, int d, double prec, double min, double max, yh_rational* out);
 double yh_rational_eval(const yh_rational* r, double x);
 /* rational_calc (rational.cpp:134-166): dst = a0 src + sum residue_i

@@ -259,3 +259,99 @@ def test_evlimits_bracket_dense_spectrum():
     # estimates approach the ends from inside
     assert 0.9 * w.max() < hi <= w.max() * (1 + 1e-9)
     assert w.min() * (1 - 1e-4) <= lo < 1.1 * w.min()
+
+
+class OracleRHMC:
+    """MonomialRHMC (monomials/rhmc.cpp:30-106) on the oracle: rational_calc
+    (rational.cpp:134-166) and the force from the oracle's multishift CG,
+    with the rational approximations the device monomial chose."""
+
+    def __init__(self, o, mass, rpf, rfc, prec):
+        self.o, self.m, self.rpf, self.rfc, self.prec = o, mass, rpf, rfc, prec
+
+    def _calc(self, r, s):
+        _, xs = self.o.cg_mshift(MVM_HPHI_SQ, self.m, self.prec, np.array(r.pole[: r.order]), s.copy())
+        return r.a0 * s + sum(r.residue[k] * xs[k] for k in range(r.order))
+
+    def reset(self, eta):
+        self.pf = self._calc(self.rpf, eta)
+        return float(np.sum(eta * eta)) - self.o.clover_logdet(self.m)
+
+    def action(self):
+        return self.o.product_re(self.pf, self._calc(self.rfc, self.pf)) - self.o.clover_logdet(self.m)
+
+    def update(self, dt):
+        n = self.rfc.order
+        _, xs = self.o.cg_mshift(MVM_HPHI_SQ, self.m, self.prec, np.array(self.rfc.pole[:n]), self.pf.copy())
+        for k in range(n):
+            self.o.wilson_fermion_force_q(dt, self.m, self.rfc.residue[k], xs[k].copy())
+        self.o.clover_logdet_force(self.m, 1.0)
+
+
+def test_rhmc_trajectory_matches_oracle():
+    """Gauge + one-flavour RHMC monomial (x^(1/4) heatbath, x^(-1/2) action
+    and force, evlimits range): dH, plaquette and momenta against the
+    oracle restatement; the heatbath action |eta|^2 agrees with
+    <pf, r_fc(H^2) pf> to the rational precision."""
+    print("\n[case] rhmc trajectory", flush=True)
+    from yahmc_amd.hmc import RHMCMonomial
+    dims = (4, 4, 4, 8)
+    beta, mass, prec, length, nf, ng = 5.6, -0.2, 1e-24, 0.3, 2, 1
+    o = Oracle("su3_fund").init(dims, bc=BC_ANTIPERIODIC, csw=1.0)
+    rng = np.random.default_rng(31)
+    links = random_gauge(rng, o.V, 3)
+    P = rng.normal(0.0, np.sqrt(0.5), size=(4 * o.V, 8))
+    eta = np.zeros((o.V, 24))
+    eta[: o.VH] = rng.normal(0.0, np.sqrt(0.5), size=(o.VH, 24))
+
+    lat = yh.Lattice(dims, nc=3, repr="fundamental", bc="antiperiodic", csw=1.0)
+    lat.set_gauge(links)
+    rm = RHMCMonomial(mass, rap_prec=1e-10, inv_prec=prec)
+    hmc = HMC(lat, [GaugeMonomial(beta), rm], [("o2mn", ng), ("o2mn", nf)])
+    rec = hmc.trajectory(FixedRng(P, eta[: o.VH]), length=length, metropolis=False)
+    print(f"evlimits x (0.95, 1.05) {rm.limits}; orders pf {rm.rpf.order} fc {rm.rfc.order}; "
+          f"multishift iterations {rm.iterations}")
+
+    o.set_gauge(links)
+    o.represent()
+    o.set_momenta(P)
+    orh = OracleRHMC(o, mass, rm.rpf, rm.rfc, prec)
+    la = orh.reset(eta)
+    # heatbath: <pf, r_fc pf> = <eta, r_pf r_fc r_pf eta> = |eta|^2 up to the rational errors
+    s0 = orh.action()
+    assert abs(s0 - la) < 3e-10 * float(np.sum(eta * eta))
+    h0 = o.kinetic() + o.gauge_action(beta) + la
+
+    def momenta(dt, level):
+        if level == 0:
+            o.wilson_gauge_force(dt, beta)
+            return
+        o.begin_fermion_force()
+        orh.update(dt)
+        o.end_fermion_force(dt)
+
+    def integrate(length, level):
+        steps = ng if level == 0 else nf
+        dt = length / steps
+        for n in range(steps):
+            momenta(A * dt if n == 0 else 2 * A * dt, level)
+            fields(dt / 2, level)
+            momenta((1 - 2 * A) * dt, level)
+            fields(dt / 2, level)
+        momenta(A * dt, level)
+
+    def fields(dt, level):
+        if level == 0:
+            o.update_links(dt)
+        else:
+            integrate(dt, level - 1)
+            o.represent()
+
+    integrate(length, 1)
+    o.represent()
+    mom_o = o.get_momenta()
+    dh_o = o.kinetic() + o.gauge_action(beta) + orh.action() - h0
+    print(f"dH device {rec['dH']:.10e} oracle {dh_o:.10e}")
+    assert abs(rec["dH"] - dh_o) < 1e-7 * max(1.0, abs(dh_o))
+    assert abs(rec["plaquette"] - o.plaquette()) < 1e-10
+    assert np.abs(lat.get_momenta() - mom_o).max() < 1e-9

@@ -109,3 +109,49 @@ def test_spin_matrices_match_reference_forms():
     # g5_sigma(0,3): spinor.cpp:340-352
     want = np.array([im[0] - 1j * re[0], -im[1] + 1j * re[1], im[2] - 1j * re[2], -im[3] + 1j * re[3]])
     assert np.allclose(o.spin_matrix(1, 0, 3) @ psi, want)
+
+
+def test_rhmc_force_is_minus_gradient_of_rational_action():
+    """MonomialRHMC (monomials/rhmc.cpp:65-106): with the rational function
+    r(x) = a0 + sum_k res_k / (x + pole_k) the action
+    S = <pf, r(H^2) pf> - log det D_oo is a rational function of the links,
+    and the force sum_k force_q(res_k, (H^2 + pole_k)^-1 pf) plus
+    clover_logdet_force(mass, 1) must be exactly its gradient."""
+    from oracle import MVM_HPHI_SQ
+    yh = pytest.importorskip("yahmc_amd")
+    o = Oracle("su3_fund").init((4, 4, 2, 4), bc=BC_ANTIPERIODIC, csw=1.0)
+    rng = np.random.default_rng(23)
+    links = random_gauge(rng, o.V, 3)
+    o.set_gauge(links)
+    o.represent()
+    mass, prec = -0.2, 1e-28
+    r = yh.rational_find(-1, 2, 1e-8, 1e-3, 64.0)
+    poles = np.array(r.pole[: r.order])
+    pf = gaussian_spinor(rng, o.V, o.dr)
+    pf[o.VH:] = 0
+
+    o.set_momenta(np.zeros((4 * o.V, 8)))
+    o.begin_fermion_force()
+    _, xs = o.cg_mshift(MVM_HPHI_SQ, mass, prec, poles, pf.copy())
+    for k in range(r.order):
+        o.wilson_fermion_force_q(1.0, mass, r.residue[k], xs[k].copy())
+    o.clover_logdet_force(mass, 1.0)
+    o.end_fermion_force(1.0)
+    F = o.get_momenta()
+    Q = rng.normal(size=F.shape)
+    U = mat_to_complex(links, 3)
+    XQ = np.array([alg_to_mat(q, 3) for q in Q])
+
+    def S(tau):
+        o.set_gauge(complex_to_mat(np.array([expm(tau * x) @ u for x, u in zip(XQ, U)])))
+        o.represent()
+        _, ys = o.cg_mshift(MVM_HPHI_SQ, mass, prec, poles, pf.copy())
+        rp = r.a0 * pf + sum(r.residue[k] * ys[k] for k in range(r.order))
+        return o.product_re(pf, rp) - o.clover_logdet(mass)
+
+    h = 1e-4
+    dS = (8 * (S(h) - S(-h)) - (S(2 * h) - S(-2 * h))) / (12 * h)
+    want = -TF * np.sum(Q * F)
+    scale = TF * np.sum(np.abs(Q * F))
+    assert abs(dS - want) < 1e-8 * scale, (dS, want, scale)
+    assert abs(want) > 1e-3 * scale

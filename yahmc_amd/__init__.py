@@ -116,6 +116,8 @@ _SIGS = {
                               ctypes.POINTER(_VP), _PI], ctypes.c_int),
     "yh_rational_zolotarev": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                ctypes.POINTER(Rational)], ctypes.c_int),
+    "yh_rational_remez": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                           ctypes.POINTER(Rational)], ctypes.c_int),
     "yh_rational_find": ([ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                           ctypes.POINTER(Rational)], ctypes.c_int),
     "yh_rational_eval": ([ctypes.POINTER(Rational), ctypes.c_double], ctypes.c_double),
@@ -202,6 +204,14 @@ def rational_zolotarev(order, n, d, xmin, xmax):
     """Zolotarev approximation of x^(n/d), n/d = +-1/2 (host computation)."""
     r = Rational()
     _check(lib().yh_rational_zolotarev(order, n, d, xmin, xmax, ctypes.byref(r)))
+    return r
+
+
+def rational_remez(order, n, d, xmin, xmax):
+    """Minimax relative-error approximation of x^(n/d), 0 < |n/d| < 1, by
+    the Remez exchange (host computation)."""
+    r = Rational()
+    _check(lib().yh_rational_remez(order, n, d, xmin, xmax, ctypes.byref(r)))
     return r
 
 
