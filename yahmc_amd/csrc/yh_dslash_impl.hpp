@@ -81,6 +81,7 @@ __device__ __forceinline__ void hop_dir(Cx<R> (&acc)[4][D], const HopArgs& a, in
   // for FP16, which keep ordinary accesses
   load_link<PREC, RECON, D, R, PREC == PREC_F64>(u, lbase, lidx, a.g.VHpad);
   Cx<R> r[2][D];
+  const R sgn = flip ? (R)-1 : (R)1;
 #pragma unroll
   for (int i = 0; i < D; i++) {
     Cx<R> r0((R)0, (R)0), r1((R)0, (R)0);
@@ -94,12 +95,10 @@ __device__ __forceinline__ void hop_dir(Cx<R> (&acc)[4][D], const HopArgs& a, in
         cmac_conj(r1, u[j][i], h[1][j]);
       }
     }
-    if (flip) {
-      r0 = Cx<R>(-r0.re, -r0.im);
-      r1 = Cx<R>(-r1.re, -r1.im);
-    }
-    r[0][i] = r0;
-    r[1][i] = r1;
+    // the antiperiodic sign as a multiply: a branch here splits the kernel
+    // into blocks and serialises the neighbour loads of the t direction
+    r[0][i] = sgn * r0;
+    r[1][i] = sgn * r1;
   }
 #pragma unroll
   for (int i = 0; i < D; i++) {
@@ -125,6 +124,61 @@ __device__ __forceinline__ void hop_dir(Cx<R> (&acc)[4][D], const HopArgs& a, in
 template <int PREC>
 constexpr int hop_min_waves() {
   return PREC == PREC_F64 ? YH_MINW_F64 : PREC == PREC_F32 ? YH_MINW_F32 : YH_MINW_F16;
+}
+
+// scheduling groups: bit m - 1 of the mask ends a group after direction m
+// (t = 1, x = 2, y = 3; 0 = one group, 7 = one group per direction).  The
+// neighbour indices of a group are made to depend, via an opaque zero, on
+// the accumulated result of the previous group, so the compiler cannot
+// hoist the loads of all eight neighbours at once (they are loads of
+// read-only kernel data and would otherwise move freely) and the registers
+// of a finished group are free again.  Defaults from a sweep of all masks on
+// MI355X (SU(3), 32^4, GB/s by algorithmic bytes): FP64-18 t|xy|z 5763
+// (one group: 2670, spilling), FP64-12 t|x|y|z 5374 (4771), FP32-12 t|xyz
+// 5174 (5111), FP16-12 txy|z 4302 (3998).
+#ifndef YH_SCHED_F64_18
+#define YH_SCHED_F64_18 5
+#endif
+#ifndef YH_SCHED_F64_12
+#define YH_SCHED_F64_12 7
+#endif
+#ifndef YH_SCHED_F32
+#define YH_SCHED_F32 1
+#endif
+#ifndef YH_SCHED_F16
+#define YH_SCHED_F16 4
+#endif
+// representations with D >= 4: bit 3 also splits each direction between its
+// forward and backward hop (one neighbour spinor and link in flight)
+#ifndef YH_SCHED_WIDE
+#define YH_SCHED_WIDE 63
+#endif
+template <int PREC, int RECON, int D>
+constexpr int hop_sched() {
+  return D >= 4 ? YH_SCHED_WIDE
+                : PREC == PREC_F64 ? (RECON == RECON_18 ? YH_SCHED_F64_18 : YH_SCHED_F64_12)
+                                   : PREC == PREC_F32 ? YH_SCHED_F32 : YH_SCHED_F16;
+}
+// an opaque zero after every accumulator is final, if bit B of the mask is set
+template <int PREC, int RECON, int B, int D, typename R>
+__device__ __forceinline__ int hop_group_dep(Cx<R> (&acc)[4][D]) {
+  constexpr int s = hop_sched<PREC, RECON, D>();
+  if constexpr ((s >> B) & 1) {
+    // every accumulator passes through an (empty) volatile asm, so the whole
+    // group's arithmetic completes here and its loaded operands die
+#pragma unroll
+    for (int sp = 0; sp < 4; sp++)
+#pragma unroll
+      for (int i = 0; i < D; i++) {
+        __asm__ volatile("" : "+v"(acc[sp][i].re));
+        __asm__ volatile("" : "+v"(acc[sp][i].im));
+      }
+    int z;
+    __asm__ volatile("v_mov_b32 %0, 0" : "=v"(z));
+    return z;
+  } else {
+    return 0;
+  }
 }
 
 template <int D, int PREC, int RECON, int EPI>
@@ -153,29 +207,38 @@ __global__ void __launch_bounds__(256, hop_min_waves<PREC>()) k_hop(HopArgs a) {
       int tb = (t == 0) ? T - 1 : t - 1;
       int nf = half_index(tf, x, y, z, g), nb = half_index(tb, x, y, z, g);
       hop_dir<0, true, D, PREC, RECON, R>(acc, a, nf, k, ap && t == 0);
-      hop_dir<0, false, D, PREC, RECON, R>(acc, a, nb, nb, ap && tb == 0);
+      const int h = hop_group_dep<PREC, RECON, 3>(acc);
+      hop_dir<0, false, D, PREC, RECON, R>(acc, a, nb + h, nb + h, ap && tb == 0);
     }
+    const int dep1 = hop_group_dep<PREC, RECON, 0>(acc);
     {
       int xf = (x + 1 == g.L[1]) ? 0 : x + 1;
       int xb = (x == 0) ? g.L[1] - 1 : x - 1;
-      int nf = half_index(t, xf, y, z, g), nb = half_index(t, xb, y, z, g);
-      hop_dir<1, true, D, PREC, RECON, R>(acc, a, nf, k, false);
-      hop_dir<1, false, D, PREC, RECON, R>(acc, a, nb, nb, false);
+      int nf = half_index(t, xf, y, z, g) + dep1, nb = half_index(t, xb, y, z, g) + dep1;
+      hop_dir<1, true, D, PREC, RECON, R>(acc, a, nf, k + dep1, false);
+      const int h = hop_group_dep<PREC, RECON, 3>(acc);
+      hop_dir<1, false, D, PREC, RECON, R>(acc, a, nb + h, nb + h, false);
     }
+    const int dep2 = hop_group_dep<PREC, RECON, 1>(acc);
     {
       int yf = (y + 1 == g.L[2]) ? 0 : y + 1;
       int yb = (y == 0) ? g.L[2] - 1 : y - 1;
-      int nf = half_index(t, x, yf, z, g), nb = half_index(t, x, yb, z, g);
-      hop_dir<2, true, D, PREC, RECON, R>(acc, a, nf, k, false);
-      hop_dir<2, false, D, PREC, RECON, R>(acc, a, nb, nb, false);
+      int nf = half_index(t, x, yf, z, g) + dep2, nb = half_index(t, x, yb, z, g) + dep2;
+      hop_dir<2, true, D, PREC, RECON, R>(acc, a, nf, k + dep2, false);
+      const int h = hop_group_dep<PREC, RECON, 3>(acc);
+      hop_dir<2, false, D, PREC, RECON, R>(acc, a, nb + h, nb + h, false);
     }
+    const int dep3 = hop_group_dep<PREC, RECON, 2>(acc);
     {
       int zf = (z + 1 == g.L[3]) ? 0 : z + 1;
       int zb = (z == 0) ? g.L[3] - 1 : z - 1;
-      int nf = half_index(t, x, y, zf, g), nb = half_index(t, x, y, zb, g);
-      hop_dir<3, true, D, PREC, RECON, R>(acc, a, nf, k, false);
-      hop_dir<3, false, D, PREC, RECON, R>(acc, a, nb, nb, false);
+      int nf = half_index(t, x, y, zf, g) + dep3, nb = half_index(t, x, y, zb, g) + dep3;
+      hop_dir<3, true, D, PREC, RECON, R>(acc, a, nf, k + dep3, false);
+      const int h = hop_group_dep<PREC, RECON, 3>(acc);
+      hop_dir<3, false, D, PREC, RECON, R>(acc, a, nb + h, nb + h, false);
     }
+    // epilogue loads (clover block, own spinor) after the hop is final
+    const int ke = k + hop_group_dep<PREC, RECON, 4>(acc);
     const bool bz = bc_zero_slice(t, g);
     const R sc = bz ? (R)0 : (R)a.scale;
 #pragma unroll
@@ -189,28 +252,31 @@ __global__ void __launch_bounds__(256, hop_min_waves<PREC>()) k_hop(HopArgs a) {
       if (a.aux) store_spinor<PREC, D, R>(a.aux, a.aux_nrm, k, g.VHpad, acc);
       constexpr int N = 2 * D;
       Cx<R> o[4][D];
+      int kb = ke;
 #pragma unroll
       for (int blk2 = 0; blk2 < 2; blk2++) {
         R cl[N * N];
-        load_clover_block<PREC, D, R>(cl, a.clov, blk2, k, g.VHpad);
+        load_clover_block<PREC, D, R>(cl, a.clov, blk2, kb, g.VHpad);
         Cx<R> xin[N], yout[N];
 #pragma unroll
         for (int i = 0; i < D; i++) { xin[i] = acc[2 * blk2][i]; xin[D + i] = acc[2 * blk2 + 1][i]; }
         herm_mv<N, R>(yout, cl, xin);
 #pragma unroll
         for (int i = 0; i < D; i++) { o[2 * blk2][i] = yout[i]; o[2 * blk2 + 1][i] = yout[D + i]; }
+        kb = ke + hop_group_dep<PREC, RECON, 5>(o);
       }
       store_spinor<PREC, D, R, PREC == PREC_F64>(a.out, a.out_nrm, k, g.VHpad, o);
     } else {
       constexpr int N = 2 * D;
       Cx<R> xs[4][D];
-      load_spinor<PREC, D, R>(xs, a.self, a.self_nrm, k, g.VHpad);
+      load_spinor<PREC, D, R>(xs, a.self, a.self_nrm, ke, g.VHpad);
       const R dg = (R)(a.diag + bc_cf_shift(t, g));
       const bool zs = a.zero_self_bc && bz;
+      int kb = ke;
 #pragma unroll
       for (int blk2 = 0; blk2 < 2; blk2++) {
         R cl[N * N];
-        load_clover_block<PREC, D, R>(cl, a.clov, blk2, k, g.VHpad);
+        load_clover_block<PREC, D, R>(cl, a.clov, blk2, kb, g.VHpad);
         Cx<R> xin[N], yout[N];
 #pragma unroll
         for (int i = 0; i < D; i++) {
@@ -223,6 +289,7 @@ __global__ void __launch_bounds__(256, hop_min_waves<PREC>()) k_hop(HopArgs a) {
           acc[2 * blk2][i] = acc[2 * blk2][i] + yout[i] + dg * xin[i];
           acc[2 * blk2 + 1][i] = acc[2 * blk2 + 1][i] + yout[D + i] + dg * xin[D + i];
         }
+        kb = ke + hop_group_dep<PREC, RECON, 5>(acc);
       }
       if (a.gamma5) {
 #pragma unroll
