@@ -3,11 +3,13 @@ one MI355X (32^4, SU(3) fundamental clover, c_sw = 1, m = -0.75,
 antiperiodic time).
 
 A "step" is one hop D_eo over the V/2 = 524,288 even sites of the 32^4
-lattice with all inputs resident in HBM.  `value` is the achieved HBM
-bandwidth of that kernel under the compulsory-traffic model (each input
-spinor, each link and each output spinor moves once per hop), measured with
-HIP events on the stream the kernel runs on.  The line also carries the
-other precisions, the CG time-to-solution to an FP64 true residual of 1e-10
+lattice with all inputs resident in HBM.  The headline hop is the
+reference's own Dslash arithmetic: FP64 spinors and full 18-real links
+(library/dirac.cpp:227-704).  `value` is the achieved HBM bandwidth of that
+kernel under the compulsory-traffic model (each input spinor, each link and
+each output spinor moves once per hop), measured with HIP events on the
+stream the kernel runs on.  The line also carries the reduced-precision hops
+(FP32 and FP16 storage, 12-real links), the CG time-to-solution to an FP64 true residual of 1e-10
 (plain FP64 CG; mixed FP32 / FP16 reliable-update CG; flexible CG with the
 learned gauge-covariant preconditioner, trained on the device on (source,
 solution) pairs first), the roofline object and a CPU baseline (the oracle
@@ -46,8 +48,9 @@ BYTES_PER_SITE = {
 FLOP_PER_SITE = 1320
 # HBM-side bytes per launch of each hop variant, from two rocprofv3 --pmc
 # passes (FETCH_SIZE, WRITE_SIZE) scaled by the same counters on the copy
-# kernel; see profiles/hop_f32r12_32x4_traffic.md.  None = not measured.
-TRAFFIC_PER_LAUNCH = {"f32_r12": 348221239}
+# kernel; see profiles/hop_f64r18_32x4_traffic.md and hop_f32r12_32x4_traffic.md.
+# None = not measured.
+TRAFFIC_PER_LAUNCH = {"f64_r18": 946758821, "f32_r12": 348221239}
 
 
 def log(msg):
@@ -221,7 +224,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--prec", default="f32", choices=["f64", "f32", "f16"], help="precision of the headline hop")
+    ap.add_argument("--prec", default="f64", choices=["f64", "f32", "f16"], help="precision of the headline hop")
     ap.add_argument("--skip-cg", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--only-hop", action="store_true", help="time the headline hop only (for profiling)")
@@ -229,6 +232,8 @@ def main():
     ap.add_argument("--skip-learned", action="store_true")
     ap.add_argument("--train-steps", type=int, default=120)
     ap.add_argument("--no-autotune", action="store_true", help="keep the default XCD chunk instead of tuning it")
+    ap.add_argument("--xcd-chunk", type=int, default=None,
+                    help="fix the XCD chunk of every hop instead of tuning it (profiling runs)")
     ap.add_argument("--skip-hmc", action="store_true")
     ap.add_argument("--skip-rhmc", action="store_true")
     ap.add_argument("--only-rhmc", action="store_true", help="the batched multishift workload only")
@@ -269,7 +274,10 @@ def main():
         log(f"smoothing: {smoothing}")
 
     tuned = None
-    if not args.no_autotune:
+    if args.xcd_chunk is not None:
+        lat.tune(yh.TUNE_XCD_CHUNK, args.xcd_chunk)
+        tuned = {"fixed": args.xcd_chunk}
+    elif not args.no_autotune:
         t0 = time.perf_counter()
         tuned = {f"{k[0]}_r{k[1]}": v for k, v in lat.autotune(reps=20).items()}
         log(f"autotuned XCD chunks {tuned} ({time.perf_counter() - t0:.1f}s)")
