@@ -328,6 +328,16 @@ int yh_net_wgrad_host(yh_net* net, int layer, const double* g, const double* h, 
  * the FP64 true residual |b - H^2 x|^2 < prec |b|^2 */
 int yh_cg_net(yh_ctx* ctx, const yh_inv_par* par, yh_net* net, yh_field* dst, const yh_field* src, int* iters,
               int* updates);
+/* The learned solve with its fallback: flexible CG preconditioned by `net`,
+ * started from the MRE guess of `mre` (either may be NULL); if it stops
+ * without converging (cap par->max_iter, 20000 when 0, or a non-finite
+ * scalar) the plain mixed-precision CG (par->sloppy_prec, FP32 if unset)
+ * restarts from the MRE guess with the caller's cap.  *fellback = 1 when the
+ * fallback ran; iterations count both attempts.  The converged solution is
+ * stored in the MRE history.  No counterpart in the reference (its callers
+ * use auto_inverter + mre_guess, inverter.cpp:9-55, mre.cpp:54-85). */
+int yh_cg_learned(yh_ctx* ctx, const yh_inv_par* par, yh_net* net, yh_mre* mre, yh_field* dst, const yh_field* src,
+                  int* iters, int* updates, int* fellback);
 
 /* --- measurement support ---------------------------------------------------- */
 /* time `reps` back-to-back hop launches (Dphi_eo shape) of the given

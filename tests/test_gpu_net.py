@@ -179,3 +179,51 @@ def test_trained_network_reduces_iterations():
         print(f"{name}: iterations {its[name]}, true residual^2 {res:.2e}")
         assert res < 1e-20
     assert its["learned"] < its["plain"]
+
+
+def test_learned_solve_and_its_fallback():
+    """yh_cg_learned: the flexible CG with the network from the MRE guess, and
+    the fallback (plain mixed CG from the MRE guess) when it cannot converge.
+    An identity network (M = 1) converges without the fallback; an all-zero
+    network (M = 0) stops at its first step with a non-finite scalar and the
+    fallback solves; without a network the fallback is the solver.  Every
+    result meets the FP64 true residual, and the MRE history holds every
+    converged solution, so a nearby source starts from a good guess."""
+    print("\n[case] learned solve fallback", flush=True)
+    dims = (4, 4, 4, 8)
+    rng = np.random.default_rng(71)
+    lat = yh.Lattice(dims, nc=3, repr="fundamental", bc="antiperiodic", csw=1.0)
+    lat.set_gauge(random_gauge(rng, lat.V, 3))
+    mass, V = -0.2, lat.V
+    par = lat.inv_par(mass, 1e-20, "Hphi_sq")
+    mre = yh.MRE(lat, 3, "Hphi_sq", mass)
+
+    def residual(x, bh):
+        ax = lat.field("f64")
+        lat.Hphi_sq(mass, ax, lat.field("f64").upload(x.download()))
+        r = bh[: V // 2] - ax.download()[: V // 2]
+        return np.sum(r * r) / np.sum(bh[: V // 2] ** 2)
+
+    ident = yh.Net(lat, (1, 1))
+    W = np.zeros((9, 1, 1, 4, 4), dtype=complex)
+    W[0, 0, 0] = np.eye(4)
+    ident.set_weights([W])
+    zero = yh.Net(lat, (1, 1))
+    zero.set_weights([np.zeros((9, 1, 1, 4, 4), dtype=complex)])
+
+    bh = gaussian_spinor(rng, V, 3)
+    bh[V // 2:] = 0
+    results = []
+    for name, net, scale in (("identity", ident, 0.0), ("zero", zero, 1e-6), ("none", None, 1e-6)):
+        b2 = bh + scale * gaussian_spinor(rng, V, 3)
+        b2[V // 2:] = 0
+        x = lat.field("f64")
+        it, upd, fb = lat.cg_learned(par, net, mre, x, lat.field("f64").upload(b2))
+        res = residual(x, b2)
+        print(f"{name}: iterations {it}, updates {upd}, fell back {fb}, true residual^2 {res:.2e}")
+        assert res < 1e-20
+        results.append((it, fb))
+    assert results[0][1] is False and results[1][1] is True and results[2][1] is True
+    # the later sources are 1e-6 away from a stored solution's source: the MRE
+    # guess leaves a much shorter solve than the first one from zero
+    assert results[2][0] < 0.7 * results[0][0]

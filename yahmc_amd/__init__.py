@@ -156,6 +156,7 @@ _SIGS = {
     "yh_net_layer_host": ([_VP, ctypes.c_int, ctypes.c_int, _D, _D], ctypes.c_int),
     "yh_net_wgrad_host": ([_VP, ctypes.c_int, _D, _D, _D], ctypes.c_int),
     "yh_cg_net": ([_VP, ctypes.c_void_p, _VP, _VP, _VP, _PI, _PI], ctypes.c_int),
+    "yh_cg_learned": ([_VP, ctypes.c_void_p, _VP, _VP, _VP, _VP, _PI, _PI, _PI], ctypes.c_int),
     "yh_tune": ([_VP, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "yh_autotune": ([_VP, ctypes.c_int, _PI], ctypes.c_int),
     "yh_config_write": ([ctypes.c_char_p, ctypes.c_int, ctypes.c_void_p, _D], ctypes.c_int),
@@ -513,6 +514,17 @@ class Lattice:
         _check(lib().yh_cg_net(self.ctx, ctypes.byref(par), net.h if net is not None else None, d.h, s.h,
                                ctypes.byref(it), ctypes.byref(up)))
         return it.value, up.value
+
+    def cg_learned(self, par, net, mre, d, s):
+        """The learned solve with its fallback (yh_cg_learned): flexible CG
+        preconditioned by net from the MRE guess; plain mixed CG from the
+        MRE guess if that does not converge.  Returns (iterations, reliable
+        updates, fell_back)."""
+        it, up, fb = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+        _check(lib().yh_cg_learned(self.ctx, ctypes.byref(par), net.h if net is not None else None,
+                                   mre.h if mre is not None else None, d.h, s.h, ctypes.byref(it), ctypes.byref(up),
+                                   ctypes.byref(fb)))
+        return it.value, up.value, bool(fb.value)
 
     # ---- forces (include/forces.h) ------------------------------------
     @property

@@ -17,6 +17,10 @@
 #include <vector>
 #include "yh_internal.hpp"
 
+// iteration cap of the learned attempt when the caller gives none (the
+// fallback then runs with the caller's cap, 0 = none as in the reference)
+static const int LEARNED_MAX_ITER = 20000;
+
 using namespace yh;
 typedef std::complex<double> zd;
 
@@ -586,6 +590,48 @@ int yh_mre_guess(yh_mre* m, yh_field* dst, const yh_field* b) {
   for (int k = 0; k < n; k++)
     if ((rc = blas_caxpy(c, dst, xs[k].real(), xs[k].imag(), m->s[k], 0))) return rc;
   return sync(c);
+}
+
+
+// Learned-preconditioner solve with its fallback: flexible CG preconditioned
+// by the network, started from the MRE guess; if it stops without converging
+// (iteration cap or a non-finite scalar), the plain mixed-precision CG
+// restarts from the MRE guess.  The converged solution joins the MRE history.
+int yh_cg_learned(yh_ctx* cc, const yh_inv_par* par, yh_net* net, yh_mre* mre, yh_field* dptr, const yh_field* sptr,
+                  int* iters, int* updates, int* fellback) {
+  Ctx* c = cc;
+  if (!c || !par || !dptr || !sptr || !iters) { set_error("null argument"); return YH_ERR_ARG; }
+  if (mre && mre->c != c) { set_error("MRE history belongs to another context"); return YH_ERR_ARG; }
+  int rc, it = 0, upd = 0;
+  if (fellback) *fellback = 0;
+  if (mre && (rc = yh_mre_guess(mre, dptr, sptr))) return rc;
+  rc = YH_ERR_NOCONV;
+  if (net) {
+    yh_inv_par p = *par;
+    if (p.max_iter <= 0) p.max_iter = LEARNED_MAX_ITER;
+    rc = yh_cg_net(cc, &p, net, dptr, sptr, &it, &upd);
+    if (rc != YH_OK && rc != YH_ERR_NOCONV && rc != YH_ERR_NONFINITE) return rc;
+  }
+  if (rc != YH_OK) {
+    if (fellback) *fellback = 1;
+    const int it0 = it, upd0 = upd;
+    // restart from the guess: the abandoned iterate may be non-finite
+    if (mre && mre->max > 0) {
+      if ((rc = yh_mre_guess(mre, dptr, sptr))) return rc;
+    } else if ((rc = field_zero(c, dptr, YH_EVEN))) {
+      return rc;
+    }
+    yh_inv_par p = *par;
+    if (p.sloppy_prec != PREC_F32 && p.sloppy_prec != PREC_F16) p.sloppy_prec = PREC_F32;
+    rc = yh_cg_mixed(cc, &p, dptr, sptr, &it, &upd);
+    it += it0;
+    upd += upd0;
+  }
+  *iters = it;
+  if (updates) *updates = upd;
+  if (rc) return rc;
+  if (mre) rc = yh_mre_store(mre, dptr);
+  return rc;
 }
 
 }  // extern "C"
