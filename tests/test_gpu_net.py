@@ -227,3 +227,32 @@ def test_learned_solve_and_its_fallback():
     # the later sources are 1e-6 away from a stored solution's source: the MRE
     # guess leaves a much shorter solve than the first one from zero
     assert results[2][0] < 0.7 * results[0][0]
+
+
+def test_hmc_solves_feed_network_training():
+    """Training data from the HMC's own solves: the two-flavour monomial keeps
+    its last (source, solution) pairs, each solved with the links of its MD
+    step (the last one, the final action, with the final links: it satisfies
+    H^2 x = b to the solver precision), and a network trained on them on the
+    device lowers its loss."""
+    print("\n[case] hmc pairs -> training", flush=True)
+    from yahmc_amd.hmc import HMC, FermionMonomial, GaugeMonomial
+    dims = (4, 4, 4, 8)
+    rng = np.random.default_rng(73)
+    lat = yh.Lattice(dims, nc=3, repr="fundamental", bc="antiperiodic", csw=1.0)
+    lat.set_gauge(random_gauge(rng, lat.V, 3))
+    mass, V = -0.2, lat.V
+    fm = FermionMonomial(mass, prec=1e-24, keep_pairs=6)
+    HMC(lat, [GaugeMonomial(5.6), fm], [("o2mn", 1), ("o2mn", 2)]).trajectory(rng, length=0.2, metropolis=False)
+    assert len(fm.pairs) == 6
+    for b, x in fm.pairs[-1:]:
+        ax = lat.field("f64")
+        lat.Hphi_sq(mass, ax, x)
+        bh = b.download()[: V // 2]
+        r = bh - ax.download()[: V // 2]
+        assert np.sum(r * r) / np.sum(bh * bh) < 1e-20
+    net = yh.Net(lat, (1, 4, 1))
+    net.init_weights(rng)
+    hist = net.train([p[0] for p in fm.pairs], [p[1] for p in fm.pairs], steps=40, batch=3, lr=2e-3, rng=rng)
+    print(f"loss {np.mean(hist[:5]):.4f} -> {np.mean(hist[-5:]):.4f}")
+    assert np.mean(hist[-5:]) < np.mean(hist[:5])

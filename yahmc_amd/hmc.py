@@ -47,7 +47,7 @@ class GaugeMonomial:
 class FermionMonomial:
     """Two flavours of EO Wilson-clover fermions (MonomialHMC, hmc.cpp)."""
 
-    def __init__(self, mass, dm=0.0, prec=1e-24, level=1, sloppy="f32", max_iter=100000):
+    def __init__(self, mass, dm=0.0, prec=1e-24, level=1, sloppy="f32", max_iter=100000, keep_pairs=0):
         if level == 0:
             raise ValueError("fermion monomials cannot be on integrator level 0 (update.cpp)")
         self.omass = mass
@@ -59,6 +59,11 @@ class FermionMonomial:
         self.pf = None
         self.la = None
         self.iterations = []
+        # (source, solution) pairs of the monomial's own solves, the training
+        # data of the learned preconditioner (the last keep_pairs solves, each
+        # solved with the links of its MD step)
+        self.keep_pairs = keep_pairs
+        self.pairs = []
 
     def _fields(self, lat):
         if self.pf is None:
@@ -86,6 +91,14 @@ class FermionMonomial:
         par = lat.inv_par(self.smass, self.prec, "Hphi_sq", sloppy=self.sloppy, max_iter=self.max_iter)
         it, _ = lat.cg_mixed(par, self.y, self.g5pf)
         self.iterations.append(it)
+        if self.keep_pairs:
+            b, x = lat.field("f64"), lat.field("f64")
+            b.copy_from(self.g5pf)
+            x.copy_from(self.y)
+            self.pairs.append((b, x))
+            while len(self.pairs) > self.keep_pairs:
+                for f in self.pairs.pop(0):
+                    f.free()
         return self.y
 
     def action(self, lat):
