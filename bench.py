@@ -45,6 +45,14 @@ BYTES_PER_SITE = {
     ("f32", 1): 96 + 8 * 48 + 96,      # 576
     ("f16", 1): 52 + 8 * 24 + 52,      # 296 (48 B half spinor + 4 B scale)
 }
+# naive no-reuse model (every neighbour spinor and link read for every
+# output site): 8 spinors + 8 links + 1 output
+BYTES_PER_SITE_NAIVE = {
+    ("f64", 0): 8 * 192 + 8 * 144 + 192,   # 2880
+    ("f64", 1): 8 * 192 + 8 * 96 + 192,    # 2496
+    ("f32", 1): 8 * 96 + 8 * 48 + 96,      # 1248
+    ("f16", 1): 8 * 52 + 8 * 24 + 52,      # 660
+}
 FLOP_PER_SITE = 1320
 # HBM-side bytes per launch of each hop variant, from two rocprofv3 --pmc
 # passes (FETCH_SIZE, WRITE_SIZE) scaled by the same counters on the copy
@@ -83,9 +91,9 @@ def cpu_baseline(threads, budget_s=12.0):
                       f"{dt * 1e3:.2f} ms each; same compulsory-bytes model as the GPU value"}
 
 
-def true_residual(lat, fx, fb):
+def true_residual(lat, fx, fb, mass=MASS):
     ax = lat.field("f64")
-    lat.Hphi_sq(MASS, ax, fx)
+    lat.Hphi_sq(mass, ax, fx)
     lat.spinor_mulr_add_assign(ax, -1.0, fb)
     res = lat.spinor_sqnorm(ax) / lat.spinor_sqnorm(fb)
     ax.free()
@@ -143,6 +151,80 @@ def learned_solves(yh, lat, fb, prec, train_steps):
         net.free()
     for f in bs + xs:
         f.free()
+    return out
+
+
+def learned_protocol(yh, dims, ntrain=4, nheld=2, nsrc=4, steps=120, mass=-0.76):
+    """The learned-preconditioner evaluation protocol, reduced in size:
+    `ntrain` training and `nheld` held-out configurations (each a cold start
+    smoothed by 10 pure-gauge trajectories with its own seed), m = -0.76,
+    a 1-8-8-1 network trained on the device config by config on (H^2 v, v)
+    pairs of each training configuration, then on every held-out
+    configuration `nsrc` Gaussian sources solved one after another to an FP64
+    true residual of 1e-10 by (a) mixed FP32 CG with the MRE guess of the
+    previous solutions (mre_past = 5) and (b) the learned solve (flexible CG
+    with the network from its own MRE guess, mixed-CG fallback)."""
+    from yahmc_amd.md import smooth
+    from yahmc_amd.synthetic import gaussian_source
+    t_all = time.perf_counter()
+    lat = yh.Lattice(dims, nc=3, repr="fundamental", bc="antiperiodic", csw=CSW)
+    configs = []
+    for k in range(ntrain + nheld):
+        smooth(lat, beta=5.6, trajectories=10, length=1.0, steps=10, seed=32246 + 1000 * (k + 1))
+        configs.append((lat.get_gauge(), lat.plaquette()))
+    net = yh.Net(lat, (1, 8, 8, 1))
+    rng = np.random.default_rng(32500)
+    net.init_weights(rng)
+    hist = []
+    t0 = time.perf_counter()
+    for k in range(ntrain):
+        lat.set_gauge(configs[k][0])
+        xs = [lat.field("f64").upload(gaussian_source(dims, 3, seed=32600 + 100 * k + j)) for j in range(8)]
+        bs = []
+        for x in xs:
+            b = lat.field("f64")
+            lat.Hphi_sq(mass, b, x)
+            bs.append(b)
+        hist += net.train(bs, xs, steps=steps // ntrain, batch=4, lr=2e-3, rng=rng)
+        for f in bs + xs:
+            f.free()
+    lat.synchronize()
+    t_train = time.perf_counter() - t0
+    par = lat.inv_par(mass, 1e-20, "Hphi_sq", sloppy="f32")
+    res = {"mixed_cg_mre5": {"iterations": 0, "seconds": 0.0, "max_true_rel_residual": 0.0},
+           "learned_fcg_mre5": {"iterations": 0, "seconds": 0.0, "max_true_rel_residual": 0.0, "fallbacks": 0}}
+    for k in range(ntrain, ntrain + nheld):
+        lat.set_gauge(configs[k][0])
+        for name, use_net in (("mixed_cg_mre5", None), ("learned_fcg_mre5", net)):
+            mre = yh.MRE(lat, 5, "Hphi_sq", mass)
+            for j in range(nsrc):
+                fb = lat.field("f64").upload(gaussian_source(dims, 3, seed=32700 + 100 * k + j))
+                fx = lat.field("f64")
+                lat.synchronize()
+                t = time.perf_counter()
+                it, _, fell = lat.cg_learned(par, use_net, mre, fx, fb)
+                lat.synchronize()
+                r = res[name]
+                r["seconds"] += time.perf_counter() - t
+                r["iterations"] += it
+                r["max_true_rel_residual"] = max(r["max_true_rel_residual"], true_residual(lat, fx, fb, mass))
+                if use_net is not None:
+                    r["fallbacks"] += int(fell)
+                fb.free()
+                fx.free()
+            mre.free()
+    for r in res.values():
+        r["seconds"] = round(r["seconds"], 3)
+    out = {"lattice": list(dims), "mass": mass, "train_configs": ntrain, "held_out_configs": nheld,
+           "sources_per_config": nsrc, "plaquettes": [round(c[1], 5) for c in configs],
+           "network": [1, 8, 8, 1], "training": {"pairs_per_config": 8, "pair_kind": "(H^2 v, v), v Gaussian",
+                                                  "steps": len(hist), "seconds": round(t_train, 2),
+                                                  "loss_first10": round(float(np.mean(hist[:10])), 4),
+                                                  "loss_last10": round(float(np.mean(hist[-10:])), 4)},
+           "held_out": res, "seconds_total": round(time.perf_counter() - t_all, 1)}
+    net.free()
+    lat.close()
+    log(f"learned protocol: {out}")
     return out
 
 
@@ -238,6 +320,10 @@ def main():
     ap.add_argument("--skip-rhmc", action="store_true")
     ap.add_argument("--only-rhmc", action="store_true", help="the batched multishift workload only")
     ap.add_argument("--only-hmc", action="store_true", help="the sextet HMC trajectory only")
+    ap.add_argument("--skip-protocol", action="store_true", help="skip the held-out learned-preconditioner protocol")
+    ap.add_argument("--only-protocol", action="store_true", help="the learned-preconditioner protocol only")
+    ap.add_argument("--protocol-dims", default="32,16,16,16",
+                    help="lattice T,X,Y,Z of the learned-preconditioner protocol (the workload's is 64,32,32,32)")
     ap.add_argument("--only-learned", action="store_true",
                     help="smoothing, network training and the two flexible-CG solves only (for profiling)")
     args = ap.parse_args()
@@ -254,6 +340,9 @@ def main():
         return
     if args.only_rhmc:
         print(json.dumps(rhmc_multishift(yh)))
+        return
+    if args.only_protocol:
+        print(json.dumps(learned_protocol(yh, tuple(int(v) for v in args.protocol_dims.split(",")))))
         return
     t0 = time.perf_counter()
     links = random_gauge(DIMS, 3, seed=32244)
@@ -297,7 +386,8 @@ def main():
         hop[f"{prec}_r{18 if recon == 0 else 12}"] = {
             "us_per_hop": round(ms * 1e3, 2), "GB_s": round(b / (ms * 1e-3) / 1e9, 1),
             "frac_hbm": round(b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 3),
-            "GFLOP_s": round(FLOP_PER_SITE * lat.VH / (ms * 1e-3) / 1e9, 1), "bytes_per_site": BYTES_PER_SITE[(prec, recon)]}
+            "GFLOP_s": round(FLOP_PER_SITE * lat.VH / (ms * 1e-3) / 1e9, 1), "bytes_per_site": BYTES_PER_SITE[(prec, recon)],
+            "GB_s_naive_model": round(BYTES_PER_SITE_NAIVE[(prec, recon)] * lat.VH / (ms * 1e-3) / 1e9, 1)}
         log(f"hop {prec} recon{18 if recon == 0 else 12}: {hop[list(hop)[-1]]}")
         fs.free()
         fd.free()
@@ -333,13 +423,15 @@ def main():
                 if k in cg:
                     cg[k]["speedup_vs_fp64"] = round(cg["fp64"]["seconds"] / cg[k]["seconds"], 2)
 
-    hmc = rhmc = None
+    hmc = rhmc = protocol = None
     if not args.only_hop:
         lat.close()
         if not args.skip_hmc:
             hmc = hmc_trajectory(yh)
         if not args.skip_rhmc:
             rhmc = rhmc_multishift(yh)
+        if not args.skip_protocol and not args.skip_learned:
+            protocol = learned_protocol(yh, tuple(int(v) for v in args.protocol_dims.split(",")))
 
     cpu = None
     if not args.skip_cpu and not args.only_hop:
@@ -377,6 +469,7 @@ def main():
         "cg_time_to_1e-10": cg,
         "hmc_trajectory": hmc,
         "rhmc_multishift": rhmc,
+        "learned_protocol": protocol,
     }
     print(json.dumps(line))
 
