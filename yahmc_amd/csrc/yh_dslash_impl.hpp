@@ -181,13 +181,18 @@ __device__ __forceinline__ int hop_group_dep(Cx<R> (&acc)[4][D]) {
   }
 }
 
+// threads per workgroup of the hop (one output site each)
+#ifndef YH_HOP_BLOCK
+#define YH_HOP_BLOCK 256
+#endif
+
 template <int D, int PREC, int RECON, int EPI>
-__global__ void __launch_bounds__(256, hop_min_waves<PREC>()) k_hop(HopArgs a) {
+__global__ void __launch_bounds__(YH_HOP_BLOCK, hop_min_waves<PREC>()) k_hop(HopArgs a) {
   typedef typename SpinorStore<PREC, D>::real R;
   if (a.skip && *a.skip) return;
   const int nblk = gridDim.x;
   const int blk = xcd_block(nblk, a.g.xq);
-  const int k = blk * 256 + threadIdx.x;
+  const int k = blk * YH_HOP_BLOCK + threadIdx.x;
   const Geom& g = a.g;
   double nrm2 = 0.0;
   if (k < g.VH) {
@@ -320,7 +325,7 @@ __global__ void __launch_bounds__(256, hop_min_waves<PREC>()) k_hop(HopArgs a) {
   if constexpr (EPI == EPI_CLOV) {
     if (a.reduce) {
       double v[1] = {nrm2};
-      block_reduce_finish<1, 256>(v, a.partials, nblk, blockIdx.x, a.ticket, a.red_out);
+      block_reduce_finish<1, YH_HOP_BLOCK>(v, a.partials, nblk, blockIdx.x, a.ticket, a.red_out);
     }
   }
 }
@@ -364,7 +369,7 @@ __global__ void __launch_bounds__(256) k_clover(void* out, float* out_nrm, const
 // ---------------------------------------------------------------------------
 template <int D, int PREC, int RECON>
 static int hop_dispatch_epi(Ctx* c, const HopArgs& a, int epi, hipStream_t st) {
-  dim3 grid(c->nblk), block(256);
+  dim3 grid((c->g.VH + YH_HOP_BLOCK - 1) / YH_HOP_BLOCK), block(YH_HOP_BLOCK);
   switch (epi) {
     case EPI_PLAIN: hipLaunchKernelGGL((k_hop<D, PREC, RECON, EPI_PLAIN>), grid, block, 0, st, a); break;
     case EPI_CLOVINV: hipLaunchKernelGGL((k_hop<D, PREC, RECON, EPI_CLOVINV>), grid, block, 0, st, a); break;
