@@ -407,8 +407,114 @@ __device__ __forceinline__ void madd(cd (&A)[D][D], const cd (&B)[D][D], double 
     for (int j = 0; j < D; j++) A[i][j] = A[i][j] + s * B[i][j];
 }
 
+// A D x D matrix left in memory: element (i, j) is read where it is used, so
+// a product with one operand in memory keeps only the other operand and the
+// result in registers.  `dag` reads the conjugate transpose, `sgn` carries
+// the antiperiodic sign of a represented link.
+struct MSrc {
+  const double2* p;
+  size_t stride;
+  double sgn;
+  bool dag;
+};
+
+template <int D>
+__device__ __forceinline__ cd mget(const MSrc& m, int i, int j) {
+  const double2 v = m.p[(size_t)(m.dag ? j * D + i : i * D + j) * m.stride];
+  return m.dag ? cd(m.sgn * v.x, -m.sgn * v.y) : cd(m.sgn * v.x, m.sgn * v.y);
+}
+
+// represented link U_R(id, mu) (or its dagger) with the antiperiodic sign
+template <int D>
+__device__ __forceinline__ MSrc link_src(const double2* links, int id, int mu, bool dag, const Geom& g) {
+  int p = id >= g.VH, k = id - p * g.VH;
+  double sgn = 1.0;
+  if (g.bc == BC_ANTIPERIODIC && mu == 0) {
+    int c[4];
+    site_coords(k, p, g, c);
+    if (c[0] == 0) sgn = -1.0;
+  }
+  return MSrc{links + (size_t)(p * 4 + mu) * D * D * g.VHpad + k, (size_t)g.VHpad, sgn, dag};
+}
+
+// clover-force matrix of plane `num` at site id
+template <int D>
+__device__ __forceinline__ MSrc clf_src(const double2* clf, int num, int id, int V) {
+  return MSrc{clf + (size_t)num * D * D * V + id, (size_t)V, 1.0, false};
+}
+
+// the same source, but readable only after `dep` is final: every element of
+// dep passes through an empty volatile asm and the pointer through one more,
+// so the compiler can neither hoist the reads above the computation of dep
+// nor merge them with an earlier read of the same matrix (the reads are of
+// read-only kernel data and would otherwise all be issued at the start)
+template <int D>
+__device__ __forceinline__ MSrc after(MSrc m, cd (&dep)[D][D]) {
+#pragma unroll
+  for (int i = 0; i < D; i++)
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+      __asm__ volatile("" : "+v"(dep[i][j].re));
+      __asm__ volatile("" : "+v"(dep[i][j].im));
+    }
+  __asm__ volatile("" : "+v"(m.p));
+  return m;
+}
+
+template <int D>
+__device__ __forceinline__ void mload(cd (&A)[D][D], const MSrc& m) {
+#pragma unroll
+  for (int i = 0; i < D; i++)
+#pragma unroll
+    for (int j = 0; j < D; j++) A[i][j] = mget<D>(m, i, j);
+}
+
+// C (+)= s * M B, M in memory
+template <int D, bool ACC>
+__device__ __forceinline__ void mul_mb(cd (&C)[D][D], const MSrc& M, const cd (&B)[D][D], double s) {
+#pragma unroll
+  for (int i = 0; i < D; i++) {
+    cd row[D];
+#pragma unroll
+    for (int j = 0; j < D; j++) row[j] = cd(0, 0);
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+      const cd m = mget<D>(M, i, k);
+#pragma unroll
+      for (int j = 0; j < D; j++) cmac(row[j], m, B[k][j]);
+    }
+#pragma unroll
+    for (int j = 0; j < D; j++) C[i][j] = ACC ? C[i][j] + s * row[j] : s * row[j];
+  }
+}
+
+// C (+)= s * A M, M in memory
+template <int D, bool ACC>
+__device__ __forceinline__ void mul_am(cd (&C)[D][D], const cd (&A)[D][D], const MSrc& M, double s) {
+#pragma unroll
+  for (int j = 0; j < D; j++) {
+    cd col[D];
+#pragma unroll
+    for (int i = 0; i < D; i++) col[i] = cd(0, 0);
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+      const cd m = mget<D>(M, k, j);
+#pragma unroll
+      for (int i = 0; i < D; i++) cmac(col[i], A[i][k], m);
+    }
+#pragma unroll
+    for (int i = 0; i < D; i++) C[i][j] = ACC ? C[i][j] + s * col[i] : s * col[i];
+  }
+}
+
 // clover-leaf derivative, one thread per (site, mu) looping over nu != mu
-// (forces.cpp:46-102)
+// (forces.cpp:46-102).  With the links W0 = U(x-nu, mu)^+, W1 = U(x-nu, nu),
+// W2 = U(x+mu, nu), W3 = U(x+nu, mu)^+, W4 = U(x, nu)^+, W5 = U(x+mu-nu, nu)^+,
+// W6 = W0 W1, W7 = W2 W3, W8 = W7 W4 - W5 W6 and Z0..Z5 the clover-force
+// matrices of the plane at x, x+mu, x-nu, x+mu-nu, x+mu+nu, x+nu:
+//   F = U(x, mu) [W8 Z0 + Z1 W8 - W5 (W0 Z2 W1 + Z3 W6) + (W2 Z4 W3 + W7 Z5) W4].
+// The terms are ordered so that at most four matrices are live; every other
+// operand is read from memory (L2) where it is used.
 template <int NG, int D>
 __global__ void __launch_bounds__(128) k_force_clover_core(const double2* __restrict__ clf,
                                                            const double2* __restrict__ links, double* __restrict__ mom,
@@ -432,46 +538,40 @@ __global__ void __launch_bounds__(128) k_force_clover_core(const double2* __rest
     int num, sign;
     if (mu < nu) { num = nu * (nu - 1) / 2 + mu; sign = 1; }
     else { num = mu * (mu - 1) / 2 + nu; sign = -1; }
-    cd W0[D][D], W1[D][D], W2[D][D], W3[D][D], W4[D][D], W5[D][D], W6[D][D], W7[D][D], W8[D][D];
-    cd Z[D][D], T1[D][D], T2[D][D], Fm[D][D];
-    load_link_bc<D>(W0, links, o3, mu, g); dag<D>(W0);
-    load_link_bc<D>(W1, links, o3, nu, g);
-    load_link_bc<D>(W2, links, o1, nu, g);
-    load_link_bc<D>(W3, links, o2, mu, g); dag<D>(W3);
-    load_link_bc<D>(W4, links, id, nu, g); dag<D>(W4);
-    load_link_bc<D>(W5, links, o4, nu, g); dag<D>(W5);
-    mmul<D>(W6, W0, W1);
-    mmul<D>(W7, W2, W3);
-    mmul<D>(W8, W7, W4);
-    mmul<D>(T1, W5, W6);
-    madd<D>(W8, T1, -1.0);
-    // fmat = W8 Z0 + Z1 W8
-    load_clf<D>(Z, clf, num, id, g.V);
-    mmul<D>(Fm, W8, Z);
-    load_clf<D>(Z, clf, num, o1, g.V);
-    mmul<D>(T1, Z, W8);
-    madd<D>(Fm, T1, 1.0);
-    // fmat -= W5 (W0 Z2 W1 + Z3 W6)
-    load_clf<D>(Z, clf, num, o3, g.V);
-    mmul<D>(T1, W0, Z);
-    mmul<D>(T2, T1, W1);
-    load_clf<D>(Z, clf, num, o4, g.V);
-    mmul<D>(T1, Z, W6);
-    madd<D>(T2, T1, 1.0);
-    mmul<D>(T1, W5, T2);
-    madd<D>(Fm, T1, -1.0);
-    // fmat += (W2 Z4 W3 + W7 Z5) W4
-    load_clf<D>(Z, clf, num, o5, g.V);
-    mmul<D>(T1, W2, Z);
-    mmul<D>(T2, T1, W3);
-    load_clf<D>(Z, clf, num, o2, g.V);
-    mmul<D>(T1, W7, Z);
-    madd<D>(T2, T1, 1.0);
-    mmul<D>(T1, T2, W4);
-    madd<D>(Fm, T1, 1.0);
-    load_link_bc<D>(T1, links, id, mu, g);
-    mmul<D>(T2, T1, Fm);
-    project_add<NG, D>(mom, mu, id, g.V, T2, sign * coeff);
+    const MSrc w0 = link_src<D>(links, o3, mu, true, g), w1 = link_src<D>(links, o3, nu, false, g),
+               w2 = link_src<D>(links, o1, nu, false, g), w3 = link_src<D>(links, o2, mu, true, g),
+               w4 = link_src<D>(links, id, nu, true, g), w5 = link_src<D>(links, o4, nu, true, g);
+    const MSrc z0 = clf_src<D>(clf, num, id, g.V), z1 = clf_src<D>(clf, num, o1, g.V),
+               z2 = clf_src<D>(clf, num, o3, g.V), z3 = clf_src<D>(clf, num, o4, g.V),
+               z4 = clf_src<D>(clf, num, o5, g.V), z5 = clf_src<D>(clf, num, o2, g.V);
+    cd A[D][D], B[D][D], Fm[D][D], R[D][D];
+    // W8 = W2 W3 W4 - W5 W0 W1  (in A)
+    mload<D>(B, w3);
+    mul_mb<D, false>(R, after<D>(w2, B), B, 1.0);   // R = W7
+    mul_am<D, false>(A, R, after<D>(w4, R), 1.0);   // A = W7 W4
+    mload<D>(B, after<D>(w1, A));
+    mul_mb<D, false>(R, after<D>(w0, B), B, 1.0);   // R = W6
+    mul_mb<D, true>(A, after<D>(w5, R), R, -1.0);   // A = W8
+    // Fm = W8 Z0 + Z1 W8
+    mul_am<D, false>(Fm, A, after<D>(z0, A), 1.0);
+    mul_mb<D, true>(Fm, after<D>(z1, Fm), A, 1.0);
+    // Fm -= W5 (Z3 W6 + W0 Z2 W1)
+    mul_mb<D, false>(A, after<D>(z3, Fm), R, 1.0);  // A = Z3 W6
+    mload<D>(B, after<D>(z2, A));
+    mul_am<D, false>(R, B, after<D>(w1, B), 1.0);   // R = Z2 W1
+    mul_mb<D, true>(A, after<D>(w0, R), R, 1.0);    // A += W0 Z2 W1
+    mul_mb<D, true>(Fm, after<D>(w5, A), A, -1.0);
+    // Fm += (W7 Z5 + W2 Z4 W3) W4
+    mload<D>(B, after<D>(w3, Fm));
+    mul_mb<D, false>(R, after<D>(w2, B), B, 1.0);   // R = W7
+    mul_am<D, false>(A, R, after<D>(z5, R), 1.0);   // A = W7 Z5
+    mload<D>(B, after<D>(z4, A));
+    mul_am<D, false>(R, B, after<D>(w3, B), 1.0);   // R = Z4 W3
+    mul_mb<D, true>(A, after<D>(w2, R), R, 1.0);    // A += W2 Z4 W3
+    mul_am<D, true>(Fm, A, after<D>(w4, A), 1.0);   // Fm += A W4
+    // U(x, mu) Fm, projected onto the algebra
+    mul_mb<D, false>(A, after<D>(link_src<D>(links, id, mu, false, g), Fm), Fm, 1.0);
+    project_add<NG, D>(mom, mu, id, g.V, A, sign * coeff);
   }
 }
 
