@@ -262,6 +262,106 @@ __device__ __forceinline__ void project_add(double* mom, int mu, int id, int V, 
   }
 }
 
+// A D x D matrix left in memory: element (i, j) is read where it is used, so
+// a product with one operand in memory keeps only the other operand and the
+// result in registers.  `dag` reads the conjugate transpose, `sgn` carries
+// the antiperiodic sign of a represented link.
+struct MSrc {
+  const double2* p;
+  size_t stride;
+  double sgn;
+  bool dag;
+};
+
+template <int D>
+__device__ __forceinline__ cd mget(const MSrc& m, int i, int j) {
+  const double2 v = m.p[(size_t)(m.dag ? j * D + i : i * D + j) * m.stride];
+  return m.dag ? cd(m.sgn * v.x, -m.sgn * v.y) : cd(m.sgn * v.x, m.sgn * v.y);
+}
+
+// represented link U_R(id, mu) (or its dagger) with the antiperiodic sign
+template <int D>
+__device__ __forceinline__ MSrc link_src(const double2* links, int id, int mu, bool dag, const Geom& g) {
+  int p = id >= g.VH, k = id - p * g.VH;
+  double sgn = 1.0;
+  if (g.bc == BC_ANTIPERIODIC && mu == 0) {
+    int c[4];
+    site_coords(k, p, g, c);
+    if (c[0] == 0) sgn = -1.0;
+  }
+  return MSrc{links + (size_t)(p * 4 + mu) * D * D * g.VHpad + k, (size_t)g.VHpad, sgn, dag};
+}
+
+// clover-force matrix of plane `num` at site id
+template <int D>
+__device__ __forceinline__ MSrc clf_src(const double2* clf, int num, int id, int V) {
+  return MSrc{clf + (size_t)num * D * D * V + id, (size_t)V, 1.0, false};
+}
+
+// the same source, but readable only after `dep` is final: every element of
+// dep passes through an empty volatile asm and the pointer through one more,
+// so the compiler can neither hoist the reads above the computation of dep
+// nor merge them with an earlier read of the same matrix (the reads are of
+// read-only kernel data and would otherwise all be issued at the start)
+template <int D>
+__device__ __forceinline__ MSrc after(MSrc m, cd (&dep)[D][D]) {
+#pragma unroll
+  for (int i = 0; i < D; i++)
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+      __asm__ volatile("" : "+v"(dep[i][j].re));
+      __asm__ volatile("" : "+v"(dep[i][j].im));
+    }
+  __asm__ volatile("" : "+v"(m.p));
+  return m;
+}
+
+template <int D>
+__device__ __forceinline__ void mload(cd (&A)[D][D], const MSrc& m) {
+#pragma unroll
+  for (int i = 0; i < D; i++)
+#pragma unroll
+    for (int j = 0; j < D; j++) A[i][j] = mget<D>(m, i, j);
+}
+
+// C (+)= s * M B, M in memory
+template <int D, bool ACC>
+__device__ __forceinline__ void mul_mb(cd (&C)[D][D], const MSrc& M, const cd (&B)[D][D], double s) {
+#pragma unroll
+  for (int i = 0; i < D; i++) {
+    cd row[D];
+#pragma unroll
+    for (int j = 0; j < D; j++) row[j] = cd(0, 0);
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+      const cd m = mget<D>(M, i, k);
+#pragma unroll
+      for (int j = 0; j < D; j++) cmac(row[j], m, B[k][j]);
+    }
+#pragma unroll
+    for (int j = 0; j < D; j++) C[i][j] = ACC ? C[i][j] + s * row[j] : s * row[j];
+  }
+}
+
+// C (+)= s * A M, M in memory
+template <int D, bool ACC>
+__device__ __forceinline__ void mul_am(cd (&C)[D][D], const cd (&A)[D][D], const MSrc& M, double s) {
+#pragma unroll
+  for (int j = 0; j < D; j++) {
+    cd col[D];
+#pragma unroll
+    for (int i = 0; i < D; i++) col[i] = cd(0, 0);
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+      const cd m = mget<D>(M, k, j);
+#pragma unroll
+      for (int i = 0; i < D; i++) cmac(col[i], A[i][k], m);
+    }
+#pragma unroll
+    for (int i = 0; i < D; i++) C[i][j] = ACC ? C[i][j] + s * col[i] : s * col[i];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
@@ -286,14 +386,16 @@ __global__ void __launch_bounds__(256) k_force_hop(FF f, const double2* __restri
     spin_mat<D>(ps, c_g5omg + mu * 16, s);
     load_full_spinor<D>(o, f.y, id, g);          // rhs(x)
     outer_acc<D>(F, ps, o);
-    load_full_spinor<D>(s, f.y, fw, g);          // rhs(x + mu)
+    // the second pair of spinors is read after F is updated (fewer live registers)
+    const MSrc nf = after<D>(MSrc{nullptr, 0, 1.0, false}, F);
+    const int z = (int)(size_t)nf.p;
+    load_full_spinor<D>(s, f.y, fw + z, g);      // rhs(x + mu)
     spin_mat<D>(ps, c_g5omg + mu * 16, s);
     load_full_spinor<D>(o, f.x, id, g);          // lhs(x)
     outer_acc<D>(F, ps, o);
   }
-  cd U[D][D], R[D][D];
-  load_link_bc<D>(U, links, id, mu, g);
-  mmul<D>(R, U, F);
+  cd R[D][D];
+  mul_mb<D, false>(R, after<D>(link_src<D>(links, id, mu, false, g), F), F, 1.0);
   project_add<NG, D>(mom, mu, id, g.V, R, coeff);
 }
 
@@ -405,106 +507,6 @@ __device__ __forceinline__ void madd(cd (&A)[D][D], const cd (&B)[D][D], double 
   for (int i = 0; i < D; i++)
 #pragma unroll
     for (int j = 0; j < D; j++) A[i][j] = A[i][j] + s * B[i][j];
-}
-
-// A D x D matrix left in memory: element (i, j) is read where it is used, so
-// a product with one operand in memory keeps only the other operand and the
-// result in registers.  `dag` reads the conjugate transpose, `sgn` carries
-// the antiperiodic sign of a represented link.
-struct MSrc {
-  const double2* p;
-  size_t stride;
-  double sgn;
-  bool dag;
-};
-
-template <int D>
-__device__ __forceinline__ cd mget(const MSrc& m, int i, int j) {
-  const double2 v = m.p[(size_t)(m.dag ? j * D + i : i * D + j) * m.stride];
-  return m.dag ? cd(m.sgn * v.x, -m.sgn * v.y) : cd(m.sgn * v.x, m.sgn * v.y);
-}
-
-// represented link U_R(id, mu) (or its dagger) with the antiperiodic sign
-template <int D>
-__device__ __forceinline__ MSrc link_src(const double2* links, int id, int mu, bool dag, const Geom& g) {
-  int p = id >= g.VH, k = id - p * g.VH;
-  double sgn = 1.0;
-  if (g.bc == BC_ANTIPERIODIC && mu == 0) {
-    int c[4];
-    site_coords(k, p, g, c);
-    if (c[0] == 0) sgn = -1.0;
-  }
-  return MSrc{links + (size_t)(p * 4 + mu) * D * D * g.VHpad + k, (size_t)g.VHpad, sgn, dag};
-}
-
-// clover-force matrix of plane `num` at site id
-template <int D>
-__device__ __forceinline__ MSrc clf_src(const double2* clf, int num, int id, int V) {
-  return MSrc{clf + (size_t)num * D * D * V + id, (size_t)V, 1.0, false};
-}
-
-// the same source, but readable only after `dep` is final: every element of
-// dep passes through an empty volatile asm and the pointer through one more,
-// so the compiler can neither hoist the reads above the computation of dep
-// nor merge them with an earlier read of the same matrix (the reads are of
-// read-only kernel data and would otherwise all be issued at the start)
-template <int D>
-__device__ __forceinline__ MSrc after(MSrc m, cd (&dep)[D][D]) {
-#pragma unroll
-  for (int i = 0; i < D; i++)
-#pragma unroll
-    for (int j = 0; j < D; j++) {
-      __asm__ volatile("" : "+v"(dep[i][j].re));
-      __asm__ volatile("" : "+v"(dep[i][j].im));
-    }
-  __asm__ volatile("" : "+v"(m.p));
-  return m;
-}
-
-template <int D>
-__device__ __forceinline__ void mload(cd (&A)[D][D], const MSrc& m) {
-#pragma unroll
-  for (int i = 0; i < D; i++)
-#pragma unroll
-    for (int j = 0; j < D; j++) A[i][j] = mget<D>(m, i, j);
-}
-
-// C (+)= s * M B, M in memory
-template <int D, bool ACC>
-__device__ __forceinline__ void mul_mb(cd (&C)[D][D], const MSrc& M, const cd (&B)[D][D], double s) {
-#pragma unroll
-  for (int i = 0; i < D; i++) {
-    cd row[D];
-#pragma unroll
-    for (int j = 0; j < D; j++) row[j] = cd(0, 0);
-#pragma unroll
-    for (int k = 0; k < D; k++) {
-      const cd m = mget<D>(M, i, k);
-#pragma unroll
-      for (int j = 0; j < D; j++) cmac(row[j], m, B[k][j]);
-    }
-#pragma unroll
-    for (int j = 0; j < D; j++) C[i][j] = ACC ? C[i][j] + s * row[j] : s * row[j];
-  }
-}
-
-// C (+)= s * A M, M in memory
-template <int D, bool ACC>
-__device__ __forceinline__ void mul_am(cd (&C)[D][D], const cd (&A)[D][D], const MSrc& M, double s) {
-#pragma unroll
-  for (int j = 0; j < D; j++) {
-    cd col[D];
-#pragma unroll
-    for (int i = 0; i < D; i++) col[i] = cd(0, 0);
-#pragma unroll
-    for (int k = 0; k < D; k++) {
-      const cd m = mget<D>(M, k, j);
-#pragma unroll
-      for (int i = 0; i < D; i++) cmac(col[i], A[i][k], m);
-    }
-#pragma unroll
-    for (int i = 0; i < D; i++) C[i][j] = ACC ? C[i][j] + s * col[i] : s * col[i];
-  }
 }
 
 // clover-leaf derivative, one thread per (site, mu) looping over nu != mu
