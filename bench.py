@@ -44,6 +44,7 @@ BYTES_PER_SITE = {
     ("f64", 1): 192 + 8 * 96 + 192,    # 1152
     ("f32", 1): 96 + 8 * 48 + 96,      # 576
     ("f16", 1): 52 + 8 * 24 + 52,      # 296 (48 B half spinor + 4 B scale)
+    ("f32", 2): 96 + 8 * 24 + 96,      # 384 (FP32 spinors, FP16 12-real links)
 }
 # naive no-reuse model (every neighbour spinor and link read for every
 # output site): 8 spinors + 8 links + 1 output
@@ -52,6 +53,7 @@ BYTES_PER_SITE_NAIVE = {
     ("f64", 1): 8 * 192 + 8 * 96 + 192,    # 2496
     ("f32", 1): 8 * 96 + 8 * 48 + 96,      # 1248
     ("f16", 1): 8 * 52 + 8 * 24 + 52,      # 660
+    ("f32", 2): 8 * 96 + 8 * 24 + 96,      # 1056
 }
 FLOP_PER_SITE = 1320
 # HBM-side bytes per launch of each hop variant, from two rocprofv3 --pmc
@@ -98,6 +100,16 @@ def true_residual(lat, fx, fb, mass=MASS):
     res = lat.spinor_sqnorm(ax) / lat.spinor_sqnorm(fb)
     ax.free()
     return float(np.sqrt(res))
+
+
+def mixed_half_links(lat, fx, fb, prec):
+    """Mixed CG whose FP32 sloppy operator reads FP16 12-real links."""
+    import yahmc_amd as yh
+    lat.tune(yh.TUNE_HALF_LINKS, 1)
+    try:
+        return lat.cg_mixed(lat.inv_par(MASS, prec, "Hphi_sq", sloppy="f32", max_iter=20000), fx, fb)
+    finally:
+        lat.tune(yh.TUNE_HALF_LINKS, 0)
 
 
 def learned_solves(yh, lat, fb, prec, train_steps):
@@ -372,7 +384,7 @@ def main():
         log(f"autotuned XCD chunks {tuned} ({time.perf_counter() - t0:.1f}s)")
     hop = {}
     variants = [(args.prec, 0 if args.prec == "f64" else 1)] if args.only_hop else \
-        [("f64", 0), ("f64", 1), ("f32", 1), ("f16", 1)]
+        [("f64", 0), ("f64", 1), ("f32", 1), ("f32", 2), ("f16", 1)]
     if args.only_learned:
         fb = lat.field("f64").upload(gaussian_source(DIMS, 3, seed=32246))
         print(json.dumps(learned_solves(yh, lat, fb, 1e-20, args.train_steps)))
@@ -383,12 +395,12 @@ def main():
         lat.bench_hop(prec, recon, max(1, args.warmup), fd, fs)
         ms = lat.bench_hop(prec, recon, args.steps, fd, fs)
         b = BYTES_PER_SITE[(prec, recon)] * lat.VH
-        hop[f"{prec}_r{18 if recon == 0 else 12}"] = {
+        hop[f"{prec}_r{ {0: '18', 1: '12', 2: '12h'}[recon] }"] = {
             "us_per_hop": round(ms * 1e3, 2), "GB_s": round(b / (ms * 1e-3) / 1e9, 1),
             "frac_hbm": round(b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 3),
             "GFLOP_s": round(FLOP_PER_SITE * lat.VH / (ms * 1e-3) / 1e9, 1), "bytes_per_site": BYTES_PER_SITE[(prec, recon)],
             "GB_s_naive_model": round(BYTES_PER_SITE_NAIVE[(prec, recon)] * lat.VH / (ms * 1e-3) / 1e9, 1)}
-        log(f"hop {prec} recon{18 if recon == 0 else 12}: {hop[list(hop)[-1]]}")
+        log(f"hop {prec} recon{ {0: '18', 1: '12', 2: '12h'}[recon] }: {hop[list(hop)[-1]]}")
         fs.free()
         fd.free()
     key = f"{args.prec}_r{18 if args.prec == 'f64' else 12}"
@@ -404,7 +416,8 @@ def main():
         prec = 1e-20
         for name, fn in (("fp64", lambda fx: (lat.cg_inverter(lat.inv_par(MASS, prec, "Hphi_sq", max_iter=20000), fx, fb), 0)),
                          ("mixed_f32", lambda fx: lat.cg_mixed(lat.inv_par(MASS, prec, "Hphi_sq", sloppy="f32", max_iter=20000), fx, fb)),
-                         ("mixed_f16", lambda fx: lat.cg_mixed(lat.inv_par(MASS, prec, "Hphi_sq", sloppy="f16", max_iter=20000), fx, fb))):
+                         ("mixed_f16", lambda fx: lat.cg_mixed(lat.inv_par(MASS, prec, "Hphi_sq", sloppy="f16", max_iter=20000), fx, fb)),
+                         ("mixed_f32_h16links", lambda fx: mixed_half_links(lat, fx, fb, prec))):
             fx = lat.field("f64")
             lat.synchronize()
             t = time.perf_counter()
@@ -419,7 +432,7 @@ def main():
         if not args.skip_learned:
             cg.update(learned_solves(yh, lat, fb, prec, args.train_steps))
         if "fp64" in cg:
-            for k in ("mixed_f32", "mixed_f16", "fcg_plain_f32", "learned_f32", "learned_small_f32"):
+            for k in ("mixed_f32", "mixed_f16", "mixed_f32_h16links", "fcg_plain_f32", "learned_f32", "learned_small_f32"):
                 if k in cg:
                     cg[k]["speedup_vs_fp64"] = round(cg["fp64"]["seconds"] / cg[k]["seconds"], 2)
 

@@ -358,6 +358,10 @@ int yh_synchronize(yh_ctx* ctx);
 /* YH_TUNE_GRAPHS = the multishift solver captures its 16-iteration chunk
  * in a HIP graph once and replays it (1, default) or launches eagerly (0) */
 #define YH_TUNE_GRAPHS 3
+/* 1: FP32 operators of SU(3) fundamental fields read the FP16 12-real links
+ * (spinors and clover stay FP32): the sloppy operator of the mixed CG moves
+ * 384 instead of 576 bytes per hop site */
+#define YH_TUNE_HALF_LINKS 4
 int yh_tune(yh_ctx* ctx, int key, int value);
 /* autotuner (no counterpart in the reference): times the hop of every
  * precision / reconstruction this context holds for each XCD chunk size

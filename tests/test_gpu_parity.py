@@ -218,6 +218,31 @@ def test_bicgstab_and_mshift():
         assert rel(x[: o.VH], xo[: o.VH]) < 1e-11
 
 
+def test_half_links_under_fp32_spinors():
+    """YH_TUNE_HALF_LINKS: the FP32 operator reading FP16 12-real links agrees
+    with the oracle to FP16 link accuracy, and the mixed CG built on it still
+    reaches the FP64 true residual."""
+    o, lat, rng = setup("su3_fund", dims=(8, 8, 8, 8), bc=BC_ANTIPERIODIC, csw=1.0, seed=32244)
+    mass, prec = -0.75, 1e-20
+    lat.tune(yh.TUNE_HALF_LINKS, 1)
+    s = src_field(o, rng, even_only=True)
+    want = np.zeros_like(s)
+    o.Hphi(mass, want, s.copy())
+    fd = lat.field("f32")
+    lat.Hphi(mass, fd, lat.field("f32").upload(s))
+    err = rel(fd.download()[: o.VH], want[: o.VH])
+    assert 1e-6 < err < 5e-3  # the FP16 links are in use, to their accuracy
+    fb = lat.field("f64").upload(s)
+    fx = lat.field("f64")
+    it, upd = lat.cg_mixed(lat.inv_par(mass, prec, "Hphi_sq", sloppy="f32", max_iter=5000), fx, fb)
+    r = true_res(o, MVM_HPHI_SQ, mass, fx.download(), s)
+    print(f"half links: operator rel err {err:.2e}; mixed CG iterations={it} updates={upd} true residual^2={r:.3e}")
+    assert r < prec
+    lat.tune(yh.TUNE_HALF_LINKS, 0)
+    lat.Hphi(mass, fd, lat.field("f32").upload(s))
+    assert rel(fd.download()[: o.VH], want[: o.VH]) < 2e-6
+
+
 @pytest.mark.parametrize("sloppy", ["f32", "f16"])
 def test_cg_mixed_true_residual(sloppy):
     """Mixed-precision CG reaches the FP64 true residual 1e-10 (prec 1e-20),

@@ -19,7 +19,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "lib", "libyahmc.so")
 
 YH_OK = 0
-TUNE_XCD_CHUNK, TUNE_NET_PREFETCH, TUNE_GRAPHS = 1, 2, 3
+TUNE_XCD_CHUNK, TUNE_NET_PREFETCH, TUNE_GRAPHS, TUNE_HALF_LINKS = 1, 2, 3, 4
 ERRORS = {-1: "invalid argument", -2: "HIP error", -3: "bad state", -4: "not converged", -5: "non-finite",
           -6: "unsupported"}
 

@@ -172,7 +172,8 @@ static void free_scratch(Ctx* c) {
 int default_recon(Ctx* c, int prec) {
   // SU(3) fundamental sloppy operators use 12-real links; FP64 keeps the
   // full 18 reals so that operator parity with the reference is exact.
-  if (prec != PREC_F64 && c->nc == 3 && c->repr == YH_REPR_FUNDAMENTAL) return RECON_12;
+  if (prec != PREC_F64 && c->nc == 3 && c->repr == YH_REPR_FUNDAMENTAL)
+    return prec == PREC_F32 && c->half_links ? RECON_12H : RECON_12;
   return RECON_18;
 }
 
@@ -617,6 +618,10 @@ int yh_tune(yh_ctx* c, int key, int value) {
   }
   if (key == YH_TUNE_GRAPHS) {
     c->use_graphs = value != 0;
+    return YH_OK;
+  }
+  if (key == YH_TUNE_HALF_LINKS) {
+    c->half_links = value != 0;
     return YH_OK;
   }
   set_error("unknown tuning key");

@@ -21,7 +21,9 @@ extern template int clover_apply_d<6>(Ctx*, int, int, double, Field*, const Fiel
 int hop_launch(Ctx* c, const HopCall& h) {
   if (!c->gauge_set) { set_error("no gauge field"); return YH_ERR_STATE; }
   int p = h.out_parity;
-  void* links = c->links[h.prec][h.recon];
+  // RECON_12H reads the FP16 12-real link array under FP32 spinors
+  void* links = h.recon == RECON_12H ? (h.prec == PREC_F32 ? c->links[PREC_F16][RECON_12] : nullptr)
+                                     : c->links[h.prec][h.recon];
   if (!links) { set_error("links for this precision / reconstruction are not available"); return YH_ERR_UNSUPPORTED; }
   if (h.in->prec != h.prec || h.out->prec != h.prec) { set_error("field precision mismatch"); return YH_ERR_ARG; }
   HopArgs a;
@@ -35,7 +37,7 @@ int hop_launch(Ctx* c, const HopCall& h) {
   a.self_nrm = h.self ? h.self->nrm[p] : nullptr;
   size_t lvbytes;
   int lnv;
-  switch (h.prec * 2 + h.recon) {
+  switch (h.recon == RECON_12H ? 5 : h.prec * 2 + h.recon) {
     case 0: lvbytes = 16; lnv = c->dr * c->dr; break;
     case 1: lvbytes = 16; lnv = 6; break;
     case 2: lvbytes = 8; lnv = c->dr * c->dr; break;

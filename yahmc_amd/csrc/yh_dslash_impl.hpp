@@ -382,6 +382,13 @@ static int hop_dispatch_epi(Ctx* c, const HopArgs& a, int epi, hipStream_t st) {
 template <int D>
 int hop_dispatch_d(Ctx* c, const HopArgs& a, int prec, int recon, int epi) {
   hipStream_t st = c->stream;
+  if (recon == RECON_12H) {
+    if constexpr (D == 3) {
+      if (prec == PREC_F32) return hop_dispatch_epi<3, PREC_F32, RECON_12H>(c, a, epi, st);
+    }
+    set_error("FP16 12-real links under FP32 spinors need SU(3) fundamental");
+    return YH_ERR_UNSUPPORTED;
+  }
   if (recon == RECON_12) {
     if constexpr (D == 3) {
       if (prec == PREC_F64) return hop_dispatch_epi<3, PREC_F64, RECON_12>(c, a, epi, st);

@@ -471,13 +471,13 @@ __global__ void k_derive_links(const double2* __restrict__ src, void* __restrict
   } else if constexpr (PREC == PREC_F16 && RECON == RECON_18) {
     for (int v = 0; v < NV; v++) {
       double2 a = s[(size_t)v * g.VHpad + k];
-      d[(size_t)v * g.VHpad + k] = (unsigned)f2h((float)a.x) | ((unsigned)f2h((float)a.y) << 16);
+      d[(size_t)v * g.VHpad + k] = f2q((float)a.x) | (f2q((float)a.y) << 16);
     }
   } else {
     for (int v = 0; v < 3; v++) {
       double2 a = s[(size_t)(2 * v) * g.VHpad + k], b = s[(size_t)(2 * v + 1) * g.VHpad + k];
-      d[(size_t)v * g.VHpad + k] = make_uint2((unsigned)f2h((float)a.x) | ((unsigned)f2h((float)a.y) << 16),
-                                              (unsigned)f2h((float)b.x) | ((unsigned)f2h((float)b.y) << 16));
+      d[(size_t)v * g.VHpad + k] = make_uint2(f2q((float)a.x) | (f2q((float)a.y) << 16),
+                                              f2q((float)b.x) | (f2q((float)b.y) << 16));
     }
   }
 }

@@ -80,7 +80,8 @@ struct Ctx {
   // learned preconditioner: layer kernels prefetch the next path (tunable)
   int net_prefetch = 0;
   // per (precision, reconstruction) XCD chunk chosen by the autotuner (-1: g.xq)
-  int xq_tab[3][2] = {{-1, -1}, {-1, -1}, {-1, -1}};
+  int xq_tab[3][3] = {{-1, -1, -1}, {-1, -1, -1}, {-1, -1, -1}};
+  int half_links = 0;  // FP32 operators of SU(3) fundamental read FP16 12-real links
   // solver chunks are captured once into a HIP graph and replayed
   int use_graphs = 1;
 };
