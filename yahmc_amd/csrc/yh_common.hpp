@@ -221,10 +221,14 @@ __device__ __forceinline__ float h2f(unsigned short h) { return __half2float(__u
 // iterations with FP16 storage, 764 with fixed point, 681 with FP32 at 32^4.
 // Decoding puts the 16 bits under the exponent of 2^23, so
 // 2^-15 * float(2^23 + u) - 257 = q / 32768 exactly (one OR and one FMA).
-__device__ __forceinline__ unsigned f2q(float f) {
-  int q = __float2int_rn(f * 32768.0f);
-  q = q < -32768 ? -32768 : (q > 32767 ? 32767 : q);
-  return (unsigned)(q + 32768);
+// encoding: 2^23 + 32768 + f 32768, which the FMA rounds to an integer
+// (round to nearest even), has q + 32768 in its low 16 bits
+__device__ __forceinline__ unsigned q_enc_bits(float f) {
+  return __float_as_uint(fmaf(fminf(fmaxf(f, -1.0f), 32767.0f / 32768.0f), 32768.0f, 8421376.0f));
+}
+__device__ __forceinline__ unsigned f2q(float f) { return q_enc_bits(f) & 0xffffu; }
+__device__ __forceinline__ unsigned pack_q2(float lo, float hi) {
+  return (q_enc_bits(lo) & 0xffffu) | (q_enc_bits(hi) << 16);
 }
 __device__ __forceinline__ float q2f_lo(unsigned v) {
   return fmaf(__uint_as_float(0x4B000000u | (v & 0xffffu)), 1.0f / 32768.0f, -257.0f);
@@ -269,6 +273,25 @@ __device__ __forceinline__ void load_spinor(Cx<R> (&s)[4][D], const void* base, 
   }
 }
 
+// a 16-bit spinor without its site scale (returned): the hop applies the
+// scale once to the projected, transported half spinor
+template <int D, typename R>
+__device__ __forceinline__ R load_spinor_q(Cx<R> (&s)[4][D], const void* base, const float* nrm, int site,
+                                           int stride) {
+  const uint4* p = (const uint4*)base;
+#pragma unroll
+  for (int k = 0; k < D; k++) {
+    uint4 v = p[(size_t)k * stride + site];
+    unsigned w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      int c = 4 * k + q;
+      s[c / D][c % D] = Cx<R>((R)q2f_lo(w[q]), (R)q2f_hi(w[q]));
+    }
+  }
+  return (R)nrm[site];
+}
+
 // NT: write with non-temporal stores (a field no later access of the same
 // kernel reads)
 template <int PREC, int D, typename R, bool NT = false>
@@ -306,7 +329,7 @@ __device__ __forceinline__ void store_spinor(void* base, float* nrm, int site, i
 #pragma unroll
       for (int q = 0; q < 4; q++) {
         int c = 4 * k + q;
-        w[q] = f2q((float)s[c / D][c % D].re * inv) | (f2q((float)s[c / D][c % D].im * inv) << 16);
+        w[q] = pack_q2((float)s[c / D][c % D].re * inv, (float)s[c / D][c % D].im * inv);
       }
       const uint4 v = make_uint4(w[0], w[1], w[2], w[3]);
       if constexpr (NT) st_stream(&p[(size_t)k * stride + site], v);

@@ -66,7 +66,9 @@ __device__ __forceinline__ void hop_dir(Cx<R> (&acc)[4][D], const HopArgs& a, in
   constexpr int LNV = LinkStore<PREC, RECON, D>::NV;
   typedef typename LinkStore<PREC, RECON, D>::vec LV;
   Cx<R> s[4][D];
-  load_spinor<PREC, D, R>(s, a.in, a.in_nrm, nidx, a.g.VHpad);
+  R sc = (R)1;
+  if constexpr (PREC == PREC_F16) sc = load_spinor_q<D, R>(s, a.in, a.in_nrm, nidx, a.g.VHpad);
+  else load_spinor<PREC, D, R>(s, a.in, a.in_nrm, nidx, a.g.VHpad);
   Cx<R> h[2][D];
 #pragma unroll
   for (int i = 0; i < D; i++) {
@@ -81,7 +83,7 @@ __device__ __forceinline__ void hop_dir(Cx<R> (&acc)[4][D], const HopArgs& a, in
   // for FP16, which keep ordinary accesses
   load_link<PREC, RECON, D, R, PREC == PREC_F64>(u, lbase, lidx, a.g.VHpad);
   Cx<R> r[2][D];
-  const R sgn = flip ? (R)-1 : (R)1;
+  const R sgn = flip ? -sc : sc;  // antiperiodic sign and 16-bit site scale
 #pragma unroll
   for (int i = 0; i < D; i++) {
     Cx<R> r0((R)0, (R)0), r1((R)0, (R)0);
