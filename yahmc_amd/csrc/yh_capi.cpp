@@ -386,6 +386,10 @@ int yh_destroy(yh_ctx* c) {
     if (c->clov[p]) (void)hipFree(c->clov[p]);
     if (c->clinv[p]) (void)hipFree(c->clinv[p]);
   }
+  {
+    if (c->clov_nrm) (void)hipFree(c->clov_nrm);
+    if (c->clinv_nrm) (void)hipFree(c->clinv_nrm);
+  }
   for (int i = 4; i < 6; i++)
     if (c->tmp[i]) { (void)hipFree(c->tmp[i]->mem); delete static_cast<yh_field*>(c->tmp[i]); }
   if (c->fund) (void)hipFree(c->fund);

@@ -420,15 +420,29 @@ template <int D>
 struct CloverStore<PREC_F64, D> { typedef double2 vec; static constexpr int NV = (2 * D) * (2 * D); static constexpr int BYTES = 8 * 2 * (2 * D) * (2 * D); };
 template <int D>
 struct CloverStore<PREC_F32, D> { typedef float4 vec; static constexpr int NV = (2 * D) * (2 * D) / 2; static constexpr int BYTES = 4 * 2 * (2 * D) * (2 * D); };
+// 16-bit: fixed point (as the 16-bit spinors) with one float scale per block
+// and site, the block's largest |real|: 4 reals per uint2
 template <int D>
-struct CloverStore<PREC_F16, D> { typedef float4 vec; static constexpr int NV = (2 * D) * (2 * D) / 2; static constexpr int BYTES = 4 * 2 * (2 * D) * (2 * D); };
+struct CloverStore<PREC_F16, D> { typedef uint2 vec; static constexpr int NV = (2 * D) * (2 * D) / 2; static constexpr int BYTES = 2 * 2 * (2 * D) * (2 * D) + 8; };
 
 // load block b's packed n*n reals into a register array
+// (16-bit: cnrm holds the scales, [block][site])
 template <int PREC, int D, typename R>
 __device__ __forceinline__ void load_clover_block(R (&c)[(2 * D) * (2 * D)], const void* base, int blk, int site,
-                                                  int stride) {
+                                                  int stride, const float* cnrm = nullptr) {
   constexpr int n2 = (2 * D) * (2 * D);
-  if constexpr (PREC == PREC_F64) {
+  if constexpr (PREC == PREC_F16) {
+    const uint2* p = (const uint2*)base + (size_t)blk * (n2 / 4) * stride;
+    const R sc = (R)cnrm[(size_t)blk * stride + site];
+#pragma unroll
+    for (int k = 0; k < n2 / 4; k++) {
+      uint2 v = p[(size_t)k * stride + site];
+      c[4 * k] = sc * (R)q2f_lo(v.x);
+      c[4 * k + 1] = sc * (R)q2f_hi(v.x);
+      c[4 * k + 2] = sc * (R)q2f_lo(v.y);
+      c[4 * k + 3] = sc * (R)q2f_hi(v.y);
+    }
+  } else if constexpr (PREC == PREC_F64) {
     const double2* p = (const double2*)base + (size_t)blk * (n2 / 2) * stride;
 #pragma unroll
     for (int k = 0; k < n2 / 2; k++) {

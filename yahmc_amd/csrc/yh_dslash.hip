@@ -49,16 +49,24 @@ int hop_launch(Ctx* c, const HopCall& h) {
   a.lf = (const char*)links + (size_t)p * par_bytes;
   a.lb = (const char*)links + (size_t)(1 - p) * par_bytes;
   int N = 2 * c->dr;
-  int cprec = h.prec == PREC_F64 ? PREC_F64 : PREC_F32;
-  size_t cvb = cprec == PREC_F64 ? sizeof(double2) : sizeof(float4);
+  // the clover of an operator is stored at its spinor precision (16-bit:
+  // fixed point with block scales)
+  const int cprec = h.prec;
+  size_t cvb = cprec == PREC_F16 ? sizeof(uint2) : 16;
   size_t cnv = cprec == PREC_F64 ? (size_t)N * N : (size_t)N * N / 2;  // vec per site (2 blocks)
   a.clov = nullptr;
+  a.clov_nrm = nullptr;
   if (h.epi == EPI_CLOV) {
-    if (!c->clov[cprec]) { set_error("clover term missing"); return YH_ERR_STATE; }
+    if (!c->clov[cprec] || (cprec == PREC_F16 && !c->clov_nrm)) { set_error("clover term missing"); return YH_ERR_STATE; }
     a.clov = (const char*)c->clov[cprec] + (size_t)p * cnv * c->g.VHpad * cvb;
+    if (cprec == PREC_F16) a.clov_nrm = c->clov_nrm + (size_t)p * 2 * c->g.VHpad;
   } else if (h.epi == EPI_CLOVINV) {
-    if (!c->clinv[cprec] || p != 1) { set_error("clover inverse missing"); return YH_ERR_STATE; }
+    if (!c->clinv[cprec] || p != 1 || (cprec == PREC_F16 && !c->clinv_nrm)) {
+      set_error("clover inverse missing");
+      return YH_ERR_STATE;
+    }
     a.clov = c->clinv[cprec];
+    a.clov_nrm = c->clinv_nrm;
   }
   a.scale = h.scale;
   a.diag = h.diag;

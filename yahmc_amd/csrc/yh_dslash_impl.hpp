@@ -51,6 +51,7 @@ struct HopArgs {
   const void* lf;   // links of the output parity, [mu][vec][site]
   const void* lb;   // links of the input parity
   const void* clov; // packed clover (CLOV: term of the output parity; CLOVINV: inverse)
+  const float* clov_nrm;  // 16-bit clover: block scales [block][site] of the same sites
   double scale, diag;
   int parity, gamma5, zero_self_bc, reduce;
   double* partials;
@@ -263,7 +264,7 @@ __global__ void __launch_bounds__(YH_HOP_BLOCK, hop_min_waves<PREC>()) k_hop(Hop
 #pragma unroll
       for (int blk2 = 0; blk2 < 2; blk2++) {
         R cl[N * N];
-        load_clover_block<PREC, D, R>(cl, a.clov, blk2, kb, g.VHpad);
+        load_clover_block<PREC, D, R>(cl, a.clov, blk2, kb, g.VHpad, a.clov_nrm);
         Cx<R> xin[N], yout[N];
 #pragma unroll
         for (int i = 0; i < D; i++) { xin[i] = acc[2 * blk2][i]; xin[D + i] = acc[2 * blk2 + 1][i]; }
@@ -283,7 +284,7 @@ __global__ void __launch_bounds__(YH_HOP_BLOCK, hop_min_waves<PREC>()) k_hop(Hop
 #pragma unroll
       for (int blk2 = 0; blk2 < 2; blk2++) {
         R cl[N * N];
-        load_clover_block<PREC, D, R>(cl, a.clov, blk2, kb, g.VHpad);
+        load_clover_block<PREC, D, R>(cl, a.clov, blk2, kb, g.VHpad, a.clov_nrm);
         Cx<R> xin[N], yout[N];
 #pragma unroll
         for (int i = 0; i < D; i++) {
@@ -336,7 +337,8 @@ __global__ void __launch_bounds__(YH_HOP_BLOCK, hop_min_waves<PREC>()) k_hop(Hop
 // (library/dirac.cpp:25-225)
 template <int D, int PREC, bool INV>
 __global__ void __launch_bounds__(256) k_clover(void* out, float* out_nrm, const void* in, const float* in_nrm,
-                                                const void* clov, double diag, int parity, Geom g) {
+                                                const void* clov, const float* clov_nrm, double diag, int parity,
+                                                Geom g) {
   typedef typename SpinorStore<PREC, D>::real R;
   int k = blockIdx.x * 256 + threadIdx.x;
   if (k >= g.VH) return;
@@ -349,7 +351,7 @@ __global__ void __launch_bounds__(256) k_clover(void* out, float* out_nrm, const
 #pragma unroll
   for (int blk2 = 0; blk2 < 2; blk2++) {
     R cl[N * N];
-    load_clover_block<PREC, D, R>(cl, clov, blk2, k, g.VHpad);
+    load_clover_block<PREC, D, R>(cl, clov, blk2, k, g.VHpad, clov_nrm);
     Cx<R> xin[N], yout[N];
 #pragma unroll
     for (int i = 0; i < D; i++) { xin[i] = xs[2 * blk2][i]; xin[D + i] = xs[2 * blk2 + 1][i]; }
@@ -410,21 +412,27 @@ int hop_dispatch_d(Ctx* c, const HopArgs& a, int prec, int recon, int epi) {
 template <int D>
 int clover_apply_d(Ctx* c, int prec, int parity, double diag, Field* out, const Field* in, int inverse) {
   int N = 2 * c->dr;
-  int cprec = prec == PREC_F64 ? PREC_F64 : PREC_F32;
-  size_t cvb = cprec == PREC_F64 ? sizeof(double2) : sizeof(float4);
+  const int cprec = prec;
+  size_t cvb = cprec == PREC_F16 ? sizeof(uint2) : 16;
   size_t cnv = cprec == PREC_F64 ? (size_t)N * N : (size_t)N * N / 2;
   const void* cl;
-  if (inverse) cl = c->clinv[cprec];
-  else cl = (const char*)c->clov[cprec] + (size_t)parity * cnv * c->g.VHpad * cvb;
-  if (!cl) { set_error("clover data missing"); return YH_ERR_STATE; }
+  const float* cn = nullptr;
+  if (inverse) {
+    cl = c->clinv[cprec];
+    cn = c->clinv_nrm;
+  } else {
+    cl = c->clov[cprec] ? (const char*)c->clov[cprec] + (size_t)parity * cnv * c->g.VHpad * cvb : nullptr;
+    cn = c->clov_nrm ? c->clov_nrm + (size_t)parity * 2 * c->g.VHpad : nullptr;
+  }
+  if (!cl || (cprec == PREC_F16 && !cn)) { set_error("clover data missing"); return YH_ERR_STATE; }
   dim3 grid((c->g.VH + 255) / 256);
   void* o = out->half[parity];
   float* on = out->nrm[parity];
   const void* i = in->half[parity];
   const float* inn = in->nrm[parity];
 #define YH_CL(P)                                                                                               \
-  if (inverse) hipLaunchKernelGGL((k_clover<D, P, true>), grid, dim3(256), 0, c->stream, o, on, i, inn, cl, diag, parity, c->g); \
-  else hipLaunchKernelGGL((k_clover<D, P, false>), grid, dim3(256), 0, c->stream, o, on, i, inn, cl, diag, parity, c->g);
+  if (inverse) hipLaunchKernelGGL((k_clover<D, P, true>), grid, dim3(256), 0, c->stream, o, on, i, inn, cl, cn, diag, parity, c->g); \
+  else hipLaunchKernelGGL((k_clover<D, P, false>), grid, dim3(256), 0, c->stream, o, on, i, inn, cl, cn, diag, parity, c->g);
   if (prec == PREC_F64) { YH_CL(PREC_F64) }
   else if (prec == PREC_F32) { YH_CL(PREC_F32) }
   else {

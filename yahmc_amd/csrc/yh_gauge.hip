@@ -492,6 +492,31 @@ __global__ void k_clover_to_f32(const double2* __restrict__ src, float4* __restr
   dst[(size_t)v * VHpad + k] = make_float4((float)a.x, (float)a.y, (float)b.x, (float)b.y);
 }
 
+// packed clover F64 -> 16-bit fixed point: per site and block the largest
+// |real| becomes the block scale, 4 reals per uint2 (same real order)
+__global__ void k_clover_to_q16(const double2* __restrict__ src, uint2* __restrict__ dst, float* __restrict__ nrm,
+                                int n2, int VHpad, int VH) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= VH) return;
+  for (int blk = 0; blk < 2; blk++) {
+    const double2* s = src + (size_t)blk * (n2 / 2) * VHpad + k;
+    double mx = 0.0;
+    for (int v = 0; v < n2 / 2; v++) {
+      double2 a = s[(size_t)v * VHpad];
+      mx = fmax(mx, fmax(fabs(a.x), fabs(a.y)));
+    }
+    const float sc = (float)mx;
+    const double inv = mx > 0.0 ? 1.0 / (double)sc : 0.0;
+    nrm[(size_t)blk * VHpad + k] = sc;
+    uint2* d = dst + (size_t)blk * (n2 / 4) * VHpad + k;
+    for (int v = 0; v < n2 / 4; v++) {
+      double2 a = s[(size_t)(2 * v) * VHpad], b = s[(size_t)(2 * v + 1) * VHpad];
+      d[(size_t)v * VHpad] = make_uint2(pack_q2((float)(a.x * inv), (float)(a.y * inv)),
+                                        pack_q2((float)(b.x * inv), (float)(b.y * inv)));
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // host wrappers
 // ---------------------------------------------------------------------------
@@ -596,6 +621,18 @@ int clover_build(Ctx* c) {
                        c->g.VH);
   }
   YH_HIP(hipGetLastError());
+  // 16-bit copy for the 16-bit operators (built for D_R <= 4)
+  if (c->dr <= 4) {
+    size_t b16 = (size_t)2 * (N * N / 2) * c->g.VHpad * sizeof(uint2);
+    if (!c->clov[PREC_F16]) YH_HIP(hipMalloc(&c->clov[PREC_F16], b16));
+    if (!c->clov_nrm) YH_HIP(hipMalloc(&c->clov_nrm, sizeof(float) * 4 * c->g.VHpad));
+    for (int p = 0; p < 2; p++)
+      hipLaunchKernelGGL(k_clover_to_q16, dim3(grid1(c->g.VH)), dim3(256), 0, c->stream,
+                         out + (size_t)p * (N * N) * c->g.VHpad,
+                         (uint2*)c->clov[PREC_F16] + (size_t)p * (N * N / 2) * c->g.VHpad,
+                         c->clov_nrm + (size_t)p * 2 * c->g.VHpad, N * N, c->g.VHpad, c->g.VH);
+    YH_HIP(hipGetLastError());
+  }
   return YH_OK;
 }
 
@@ -621,6 +658,14 @@ int clover_invert(Ctx* c, double sigma) {
   hipLaunchKernelGGL(k_clover_to_f32, grid, dim3(256), 0, c->stream, inv, (float4*)c->clinv[PREC_F32], N * N / 2,
                      c->g.VHpad, c->g.VH);
   YH_HIP(hipGetLastError());
+  if (c->dr <= 4) {
+    size_t b16 = (size_t)(N * N / 2) * c->g.VHpad * sizeof(uint2);
+    if (!c->clinv[PREC_F16]) YH_HIP(hipMalloc(&c->clinv[PREC_F16], b16));
+    if (!c->clinv_nrm) YH_HIP(hipMalloc(&c->clinv_nrm, sizeof(float) * 2 * c->g.VHpad));
+    hipLaunchKernelGGL(k_clover_to_q16, dim3(grid1(c->g.VH)), dim3(256), 0, c->stream, inv,
+                       (uint2*)c->clinv[PREC_F16], c->clinv_nrm, N * N, c->g.VHpad, c->g.VH);
+    YH_HIP(hipGetLastError());
+  }
   c->cl_sigma = sigma;
   return YH_OK;
 }

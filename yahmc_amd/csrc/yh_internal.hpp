@@ -63,6 +63,8 @@ struct Ctx {
   void* clov[3] = {nullptr, nullptr, nullptr};
   // packed inverse of the odd-site diagonal at sigma = cl_sigma
   void* clinv[3] = {nullptr, nullptr, nullptr};
+  float* clov_nrm = nullptr;   // 16-bit clover block scales [parity][block][site]
+  float* clinv_nrm = nullptr;  // 16-bit clover inverse block scales [block][site]
   double cl_sigma = 0xF00F;
   double* logdet_site = nullptr;  // per odd site log det (for the sum)
   bool gauge_set = false;
