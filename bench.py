@@ -465,7 +465,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"f64": "f64", "f32": "f32", "f16": "f16-storage/f32-arith"}[args.prec],
+        "dtype": {"f64": "f64", "f32": "f32", "f16": "int16-fixed-point-storage/f32-arith"}[args.prec],
         "data": "synthetic (seeded SU(3) field smoothed by pure-gauge trajectories, Gaussian sources)"
                 if smoothing else "synthetic (seeded random SU(3) links, Gaussian source)",
         "config": {"workload": f"hop D_eo, {args.prec} recon-{18 if args.prec == 'f64' else 12}, 32^4 SU(3) fundamental "

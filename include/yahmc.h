@@ -51,6 +51,9 @@ extern "C" {
 /* storage precisions of device spinor fields */
 #define YH_PREC_F64 0
 #define YH_PREC_F32 1
+/* 16-bit storage: signed fixed point (value / 32768) of the components divided
+ * by a per-site float scale for spinors and clover blocks; plain fixed point
+ * for links (unitary, elements of modulus <= 1); arithmetic in FP32 */
 #define YH_PREC_F16 2
 
 /* parity masks (include/global.h:10-13) */
@@ -173,7 +176,7 @@ int yh_mre_destroy(yh_mre* mre);
 int yh_mre_reset(yh_mre* mre);
 int yh_mre_store(yh_mre* mre, const yh_field* x);
 int yh_mre_guess(yh_mre* mre, yh_field* dst, const yh_field* src);
-/* project-new: FP64 reliable-update CG with an FP32 / FP16 inner operator;
+/* project-new: FP64 reliable-update CG with an FP32 / 16-bit inner operator;
  * reaches the same FP64 true residual as yh_cg.  iters = sloppy iterations,
  * updates = FP64 reliable updates */
 int yh_cg_mixed(yh_ctx* ctx, const yh_inv_par* par, yh_field* dst, const yh_field* src, int* iters, int* updates);
@@ -358,7 +361,7 @@ int yh_synchronize(yh_ctx* ctx);
 /* YH_TUNE_GRAPHS = the multishift solver captures its 16-iteration chunk
  * in a HIP graph once and replays it (1, default) or launches eagerly (0) */
 #define YH_TUNE_GRAPHS 3
-/* 1: FP32 operators of SU(3) fundamental fields read the FP16 12-real links
+/* 1: FP32 operators of SU(3) fundamental fields read the 16-bit 12-real links
  * (spinors and clover stay FP32): the sloppy operator of the mixed CG moves
  * 384 instead of 576 bytes per hop site */
 #define YH_TUNE_HALF_LINKS 4
