@@ -180,17 +180,17 @@ def learned_protocol(yh, dims, ntrain=4, nheld=2, nsrc=4, steps=120, mass=-0.76)
     from yahmc_amd.synthetic import gaussian_source
     t_all = time.perf_counter()
     lat = yh.Lattice(dims, nc=3, repr="fundamental", bc="antiperiodic", csw=CSW)
-    configs = []
+    cfgs = []
     for k in range(ntrain + nheld):
         smooth(lat, beta=5.6, trajectories=10, length=1.0, steps=10, seed=32246 + 1000 * (k + 1))
-        configs.append((lat.get_gauge(), lat.plaquette()))
+        cfgs.append((lat.get_gauge(), lat.plaquette()))
     net = yh.Net(lat, (1, 8, 8, 1))
     rng = np.random.default_rng(32500)
     net.init_weights(rng)
     hist = []
     t0 = time.perf_counter()
     for k in range(ntrain):
-        lat.set_gauge(configs[k][0])
+        lat.set_gauge(cfgs[k][0])
         xs = [lat.field("f64").upload(gaussian_source(dims, 3, seed=32600 + 100 * k + j)) for j in range(8)]
         bs = []
         for x in xs:
@@ -206,7 +206,7 @@ def learned_protocol(yh, dims, ntrain=4, nheld=2, nsrc=4, steps=120, mass=-0.76)
     res = {"mixed_cg_mre5": {"iterations": 0, "seconds": 0.0, "max_true_rel_residual": 0.0},
            "learned_fcg_mre5": {"iterations": 0, "seconds": 0.0, "max_true_rel_residual": 0.0, "fallbacks": 0}}
     for k in range(ntrain, ntrain + nheld):
-        lat.set_gauge(configs[k][0])
+        lat.set_gauge(cfgs[k][0])
         for name, use_net in (("mixed_cg_mre5", None), ("learned_fcg_mre5", net)):
             mre = yh.MRE(lat, 5, "Hphi_sq", mass)
             for j in range(nsrc):
@@ -228,7 +228,7 @@ def learned_protocol(yh, dims, ntrain=4, nheld=2, nsrc=4, steps=120, mass=-0.76)
     for r in res.values():
         r["seconds"] = round(r["seconds"], 3)
     out = {"lattice": list(dims), "mass": mass, "train_configs": ntrain, "held_out_configs": nheld,
-           "sources_per_config": nsrc, "plaquettes": [round(c[1], 5) for c in configs],
+           "sources_per_config": nsrc, "plaquettes": [round(c[1], 5) for c in cfgs],
            "network": [1, 8, 8, 1], "training": {"pairs_per_config": 8, "pair_kind": "(H^2 v, v), v Gaussian",
                                                   "steps": len(hist), "seconds": round(t_train, 2),
                                                   "loss_first10": round(float(np.mean(hist[:10])), 4),
@@ -340,7 +340,7 @@ def main():
                     help="smoothing, network training and the two flexible-CG solves only (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
+        print(json.dumps({"error": "this benchmark runs on one GPU; multi-GPU runs are not supported",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 

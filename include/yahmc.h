@@ -13,8 +13,8 @@
  *                  library/geometry.cpp:756-878)
  *   gauge field    4V suNg{re[NC*NC], im[NC*NC]} doubles, index 4*id+mu
  *                  (include/gaugefield.h:6-18)
- * Device fields live in HBM in the library's own layout (see DESIGN.md of
- * the project) and are referred to by opaque handles.
+ * Device fields live in HBM in the library's own layout (structure of
+ * arrays, site-minor) and are referred to by opaque handles.
  *
  * All calls are synchronous with respect to the host unless stated, run on
  * the context's HIP stream, and return YH_OK (0) or a negative YH_ERR_*.
