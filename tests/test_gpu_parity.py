@@ -131,7 +131,8 @@ def test_config1_8x4_operators():
     assert rel(fd.download()[: o.VH], want[: o.VH]) < 1e-13
 
 
-@pytest.mark.parametrize("prec,tol", [("f32", 2e-6), ("f16", 5e-3)])
+# 16-bit: fixed-point spinors, links and clover resolve 1.5e-5 of each scale
+@pytest.mark.parametrize("prec,tol", [("f32", 2e-6), ("f16", 2e-4)])
 def test_low_precision_operators(prec, tol):
     o, lat, rng = setup("su3_fund", dims=(4, 4, 4, 8), bc=BC_ANTIPERIODIC)
     mass = -0.75
