@@ -240,11 +240,15 @@ def learned_protocol(yh, dims, ntrain=4, nheld=2, nsrc=4, steps=120, mass=-0.76)
     return out
 
 
-def hmc_trajectory(yh):
+SLOPPY_NAME = {"f32": "FP32", "f16": "16-bit fixed point"}
+
+
+def hmc_trajectory(yh, sloppy="f16"):
     """One HMC trajectory on 24^3 x 32 with two flavours of sextet (two-index
     symmetric SU(3)) clover fermions: tau = 1, o2mn with the Wilson gauge
     monomial on level 0 (1 step) and the fermion monomial on level 1
-    (10 steps), mixed-precision CG to an FP64 true residual of 1e-12, FP64
+    (10 steps), mixed-precision CG (16-bit fixed-point sloppy operator by
+    default) to an FP64 true residual of 1e-12, FP64
     link update and Hamiltonian.  Seeded random SU(3) links (hot start);
     beta = 5.6 and m = -0.75 (values the workload leaves open)."""
     from yahmc_amd.hmc import HMC, FermionMonomial, GaugeMonomial
@@ -253,7 +257,7 @@ def hmc_trajectory(yh):
     lat = yh.Lattice(dims, nc=3, repr="symmetric", bc="antiperiodic", csw=1.0)
     lat.set_gauge(random_gauge(dims, 3, seed=32244))
     plaq0 = lat.plaquette()
-    fm = FermionMonomial(-0.75, prec=1e-24, level=1, sloppy="f32")
+    fm = FermionMonomial(-0.75, prec=1e-24, level=1, sloppy=sloppy)
     hmc = HMC(lat, [GaugeMonomial(5.6), fm], [("o2mn", 1), ("o2mn", 10)])
     lat.synchronize()
     t0 = time.perf_counter()
@@ -265,7 +269,7 @@ def hmc_trajectory(yh):
            "seconds": round(dt, 3), "dH": rec["dH"], "accepted": rec["accepted"],
            "plaquette_before": round(plaq0, 6), "plaquette_after": round(rec["plaquette"], 6),
            "cg_solves": len(fm.iterations), "cg_iterations_total": int(sum(fm.iterations)),
-           "cg": "mixed FP32 reliable-update CG, FP64 true residual 1e-12"}
+           "cg": f"mixed reliable-update CG, sloppy {SLOPPY_NAME[sloppy]}, FP64 true residual 1e-12"}
     log(f"hmc trajectory: {out}")
     lat.close()
     return out
@@ -332,6 +336,7 @@ def main():
     ap.add_argument("--skip-rhmc", action="store_true")
     ap.add_argument("--only-rhmc", action="store_true", help="the batched multishift workload only")
     ap.add_argument("--only-hmc", action="store_true", help="the sextet HMC trajectory only")
+    ap.add_argument("--hmc-sloppy", default="f16", choices=["f32", "f16"], help="sloppy precision of the HMC solves")
     ap.add_argument("--skip-protocol", action="store_true", help="skip the held-out learned-preconditioner protocol")
     ap.add_argument("--only-protocol", action="store_true", help="the learned-preconditioner protocol only")
     ap.add_argument("--protocol-dims", default="32,16,16,16",
@@ -348,7 +353,7 @@ def main():
     from yahmc_amd.synthetic import gaussian_source, random_gauge
 
     if args.only_hmc:
-        print(json.dumps(hmc_trajectory(yh)))
+        print(json.dumps(hmc_trajectory(yh, args.hmc_sloppy)))
         return
     if args.only_rhmc:
         print(json.dumps(rhmc_multishift(yh)))
@@ -440,7 +445,7 @@ def main():
     if not args.only_hop:
         lat.close()
         if not args.skip_hmc:
-            hmc = hmc_trajectory(yh)
+            hmc = hmc_trajectory(yh, args.hmc_sloppy)
         if not args.skip_rhmc:
             rhmc = rhmc_multishift(yh)
         if not args.skip_protocol and not args.skip_learned:

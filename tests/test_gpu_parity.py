@@ -3,7 +3,7 @@ same seeded gauge field and sources.
 
 Tolerances: FP64 operator outputs within 1e-13 relative (max |diff| / max
 |ref|); representation and clover term within 1e-13; FP32 operators within
-2e-6; FP16 within 5e-3; solvers to the reference's own squared relative
+2e-6; 16-bit fixed point within 2e-4; solvers to the reference's own squared relative
 residual (1e-28 for CG / BiCGstab / multishift, checks/inverters.cpp:7).
 """
 import numpy as np
@@ -147,6 +147,38 @@ def test_low_precision_operators(prec, tol):
     o.Dphi_eo(want, s)
     lat.Dphi_eo(fd, lat.field(prec).upload(s))
     assert rel(fd.download()[: o.VH], want[: o.VH]) < tol
+
+
+@pytest.mark.parametrize("model", [m for m in MODELS if m != "su3_fund"])
+def test_f16_operators_all_representations(model):
+    """16-bit operators of every representation (6-dimensional ones included)
+    against the oracle's FP64 operator, open and antiperiodic time."""
+    o, lat, rng = setup(model, dims=(4, 4, 4, 6), bc=BC_OPEN if model == "su3_asym" else BC_ANTIPERIODIC)
+    mass = -0.6
+    s = src_field(o, rng, even_only=True)
+    want = np.zeros_like(s)
+    o.Hphi(mass, want, s.copy())
+    fd = lat.field("f16")
+    lat.Hphi(mass, fd, lat.field("f16").upload(s))
+    assert rel(fd.download()[: o.VH], want[: o.VH]) < 2e-4
+    want = np.zeros_like(s)
+    o.Dphi_eo(want, s)
+    lat.Dphi_eo(fd, lat.field("f16").upload(s))
+    assert rel(fd.download()[: o.VH], want[: o.VH]) < 2e-4
+
+
+def test_cg_mixed_f16_sextet_true_residual():
+    """Mixed CG with a 16-bit sloppy operator on SU(3) two-index symmetric
+    fermions reaches the FP64 true residual (prec 1e-20)."""
+    o, lat, rng = setup("su3_sym", dims=(6, 6, 6, 6), bc=BC_ANTIPERIODIC, csw=1.0, seed=9901)
+    mass, prec = -0.4, 1e-20
+    b = src_field(o, rng, even_only=True)
+    fx = lat.field("f64")
+    it, upd = lat.cg_mixed(lat.inv_par(mass, prec, "Hphi_sq", sloppy="f16", reliable_every=50, max_iter=5000),
+                           fx, lat.field("f64").upload(b))
+    r = true_res(o, MVM_HPHI_SQ, mass, fx.download(), b)
+    print(f"sextet mixed f16: iterations={it} updates={upd} true residual^2={r:.3e}")
+    assert r < prec
 
 
 def test_linalg():

@@ -676,7 +676,6 @@ int yh_autotune(yh_ctx* c, int reps, int* chosen) {
     for (int recon = 0; recon < 2; recon++) {
       if (chosen) chosen[prec * 2 + recon] = -1;
       if (!c->links[prec][recon]) continue;
-      if (prec == PREC_F16 && c->dr > 4) continue;
       int rc;
       Field *s = nullptr, *d = nullptr;
       if ((rc = field_alloc(c, prec, &s)) || (rc = field_alloc(c, prec, &d))) {

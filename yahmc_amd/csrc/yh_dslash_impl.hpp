@@ -404,9 +404,7 @@ int hop_dispatch_d(Ctx* c, const HopArgs& a, int prec, int recon, int epi) {
   }
   if (prec == PREC_F64) return hop_dispatch_epi<D, PREC_F64, RECON_18>(c, a, epi, st);
   if (prec == PREC_F32) return hop_dispatch_epi<D, PREC_F32, RECON_18>(c, a, epi, st);
-  if constexpr (D <= 4) return hop_dispatch_epi<D, PREC_F16, RECON_18>(c, a, epi, st);
-  set_error("FP16 fields are built for representation dimension <= 4");
-  return YH_ERR_UNSUPPORTED;
+  return hop_dispatch_epi<D, PREC_F16, RECON_18>(c, a, epi, st);
 }
 
 template <int D>
@@ -435,10 +433,7 @@ int clover_apply_d(Ctx* c, int prec, int parity, double diag, Field* out, const 
   else hipLaunchKernelGGL((k_clover<D, P, false>), grid, dim3(256), 0, c->stream, o, on, i, inn, cl, cn, diag, parity, c->g);
   if (prec == PREC_F64) { YH_CL(PREC_F64) }
   else if (prec == PREC_F32) { YH_CL(PREC_F32) }
-  else {
-    if constexpr (D <= 4) { YH_CL(PREC_F16) }
-    else { set_error("FP16 fields are built for representation dimension <= 4"); return YH_ERR_UNSUPPORTED; }
-  }
+  else { YH_CL(PREC_F16) }
 #undef YH_CL
   YH_HIP(hipGetLastError());
   return YH_OK;

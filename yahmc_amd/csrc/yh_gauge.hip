@@ -621,8 +621,8 @@ int clover_build(Ctx* c) {
                        c->g.VH);
   }
   YH_HIP(hipGetLastError());
-  // 16-bit copy for the 16-bit operators (built for D_R <= 4)
-  if (c->dr <= 4) {
+  // 16-bit copy for the 16-bit operators
+  {
     size_t b16 = (size_t)2 * (N * N / 2) * c->g.VHpad * sizeof(uint2);
     if (!c->clov[PREC_F16]) YH_HIP(hipMalloc(&c->clov[PREC_F16], b16));
     if (!c->clov_nrm) YH_HIP(hipMalloc(&c->clov_nrm, sizeof(float) * 4 * c->g.VHpad));
@@ -658,7 +658,7 @@ int clover_invert(Ctx* c, double sigma) {
   hipLaunchKernelGGL(k_clover_to_f32, grid, dim3(256), 0, c->stream, inv, (float4*)c->clinv[PREC_F32], N * N / 2,
                      c->g.VHpad, c->g.VH);
   YH_HIP(hipGetLastError());
-  if (c->dr <= 4) {
+  {
     size_t b16 = (size_t)(N * N / 2) * c->g.VHpad * sizeof(uint2);
     if (!c->clinv[PREC_F16]) YH_HIP(hipMalloc(&c->clinv[PREC_F16], b16));
     if (!c->clinv_nrm) YH_HIP(hipMalloc(&c->clinv_nrm, sizeof(float) * 2 * c->g.VHpad));
