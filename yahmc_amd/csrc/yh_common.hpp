@@ -213,29 +213,31 @@ struct SpinorStore<PREC_F16, D> {
 __device__ __forceinline__ float h2f(unsigned short h) { return __half2float(__ushort_as_half(h)); }
 
 // 16-bit links and spinors are fixed point: element = q / 32768 with q a
-// signed 16-bit integer stored offset by 32768.  Every element of a unitary
+// signed (two's complement) 16-bit integer.  Every element of a unitary
 // matrix in any representation has modulus <= 1 (a spinor is divided by its
 // largest component first), and fixed point resolves 1.5e-5 over the whole
 // range where FP16 rounds elements near 1 to 2.4e-4.  That error is the
 // sloppy operator's, and the mixed CG's iteration count follows it: 983
 // iterations with FP16 storage, 764 with fixed point, 681 with FP32 at 32^4.
-// Decoding puts the 16 bits under the exponent of 2^23, so
-// 2^-15 * float(2^23 + u) - 257 = q / 32768 exactly (one OR and one FMA).
-// encoding: 2^23 + 32768 + f 32768, which the FMA rounds to an integer
-// (round to nearest even), has q + 32768 in its low 16 bits
+// Decoding is one sign-extending convert per word (q2r_lo / q2r_hi compile
+// to v_cvt_f32_i32 with a WORD_0 / WORD_1 source select); the 2^-15 is a
+// separate multiply or is folded into a later scale.
+// encoding: 1.5 * 2^23 + 32768 f, which the FMA rounds to an integer
+// (round to nearest even), holds 2^22 + q in its mantissa, so its low 16
+// bits are q in two's complement
 __device__ __forceinline__ unsigned q_enc_bits(float f) {
-  return __float_as_uint(fmaf(fminf(fmaxf(f, -1.0f), 32767.0f / 32768.0f), 32768.0f, 8421376.0f));
+  return __float_as_uint(fmaf(fminf(fmaxf(f, -1.0f), 32767.0f / 32768.0f), 32768.0f, 12582912.0f));
 }
 __device__ __forceinline__ unsigned f2q(float f) { return q_enc_bits(f) & 0xffffu; }
 __device__ __forceinline__ unsigned pack_q2(float lo, float hi) {
   return (q_enc_bits(lo) & 0xffffu) | (q_enc_bits(hi) << 16);
 }
-__device__ __forceinline__ float q2f_lo(unsigned v) {
-  return fmaf(__uint_as_float(0x4B000000u | (v & 0xffffu)), 1.0f / 32768.0f, -257.0f);
-}
-__device__ __forceinline__ float q2f_hi(unsigned v) {
-  return fmaf(__uint_as_float(0x4B000000u | (v >> 16)), 1.0f / 32768.0f, -257.0f);
-}
+// q itself, as a float
+__device__ __forceinline__ float q2r_lo(unsigned v) { return (float)(int)(short)(v & 0xffffu); }
+__device__ __forceinline__ float q2r_hi(unsigned v) { return (float)((int)v >> 16); }
+constexpr float Q_UNIT = 1.0f / 32768.0f;
+__device__ __forceinline__ float q2f_lo(unsigned v) { return q2r_lo(v) * Q_UNIT; }
+__device__ __forceinline__ float q2f_hi(unsigned v) { return q2r_hi(v) * Q_UNIT; }
 __device__ __forceinline__ unsigned short f2h(float f) { return __half_as_ushort(__float2half_rn(f)); }
 
 // load the full spinor of one site into registers s[spin][colour]
@@ -259,7 +261,7 @@ __device__ __forceinline__ void load_spinor(Cx<R> (&s)[4][D], const void* base, 
     }
   } else {
     const uint4* p = (const uint4*)base;
-    float sc = nrm[site];
+    const float sc = nrm[site] * Q_UNIT;
 #pragma unroll
     for (int k = 0; k < D; k++) {
       uint4 v = p[(size_t)k * stride + site];
@@ -267,14 +269,15 @@ __device__ __forceinline__ void load_spinor(Cx<R> (&s)[4][D], const void* base, 
 #pragma unroll
       for (int q = 0; q < 4; q++) {
         int c = 4 * k + q;
-        s[c / D][c % D] = Cx<R>((R)(q2f_lo(w[q]) * sc), (R)(q2f_hi(w[q]) * sc));
+        s[c / D][c % D] = Cx<R>((R)(q2r_lo(w[q]) * sc), (R)(q2r_hi(w[q]) * sc));
       }
     }
   }
 }
 
-// a 16-bit spinor without its site scale (returned): the hop applies the
-// scale once to the projected, transported half spinor
+// a 16-bit spinor as the integers q, without 2^-15 and its site scale (their
+// product is returned): the hop applies the scale once to the projected,
+// transported half spinor
 template <int D, typename R>
 __device__ __forceinline__ R load_spinor_q(Cx<R> (&s)[4][D], const void* base, const float* nrm, int site,
                                            int stride) {
@@ -286,10 +289,10 @@ __device__ __forceinline__ R load_spinor_q(Cx<R> (&s)[4][D], const void* base, c
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       int c = 4 * k + q;
-      s[c / D][c % D] = Cx<R>((R)q2f_lo(w[q]), (R)q2f_hi(w[q]));
+      s[c / D][c % D] = Cx<R>((R)q2r_lo(w[q]), (R)q2r_hi(w[q]));
     }
   }
-  return (R)nrm[site];
+  return (R)(nrm[site] * Q_UNIT);
 }
 
 // NT: write with non-temporal stores (a field no later access of the same
@@ -368,7 +371,10 @@ __device__ __forceinline__ void su3_third_row(Cx<R> (&u)[3][3]) {
 }
 
 // NT: non-temporal loads (each link is read once per hop)
-template <int PREC, int RECON, int D, typename R, bool NT = false>
+// RAW (16-bit links): the matrix times 32768 (the integers q; a reconstructed
+// third row is brought to the same scale), for callers that fold the 2^-15
+// into a later scale
+template <int PREC, int RECON, int D, typename R, bool NT = false, bool RAW = false>
 __device__ __forceinline__ void load_link(Cx<R> (&u)[D][D], const void* base, int site, int stride) {
   typedef typename LinkStore<PREC, RECON, D>::vec V;
   const V* p = (const V*)base;
@@ -381,6 +387,7 @@ __device__ __forceinline__ void load_link(Cx<R> (&u)[D][D], const void* base, in
     for (int k = 0; k < D * D; k++) {
       V v = ld((size_t)k * stride + site);
       if constexpr (PREC == PREC_F64 || PREC == PREC_F32) u[k / D][k % D] = Cx<R>((R)v.x, (R)v.y);
+      else if constexpr (RAW) u[k / D][k % D] = Cx<R>((R)q2r_lo(v), (R)q2r_hi(v));
       else u[k / D][k % D] = Cx<R>((R)q2f_lo(v), (R)q2f_hi(v));
     }
   } else {
@@ -403,11 +410,20 @@ __device__ __forceinline__ void load_link(Cx<R> (&u)[D][D], const void* base, in
 #pragma unroll
       for (int k = 0; k < 3; k++) {
         uint2 v = ld((size_t)k * stride + site);
-        w[(2 * k) / 3][(2 * k) % 3] = Cx<R>((R)q2f_lo(v.x), (R)q2f_hi(v.x));
-        w[(2 * k + 1) / 3][(2 * k + 1) % 3] = Cx<R>((R)q2f_lo(v.y), (R)q2f_hi(v.y));
+        if constexpr (RAW) {
+          w[(2 * k) / 3][(2 * k) % 3] = Cx<R>((R)q2r_lo(v.x), (R)q2r_hi(v.x));
+          w[(2 * k + 1) / 3][(2 * k + 1) % 3] = Cx<R>((R)q2r_lo(v.y), (R)q2r_hi(v.y));
+        } else {
+          w[(2 * k) / 3][(2 * k) % 3] = Cx<R>((R)q2f_lo(v.x), (R)q2f_hi(v.x));
+          w[(2 * k + 1) / 3][(2 * k + 1) % 3] = Cx<R>((R)q2f_lo(v.y), (R)q2f_hi(v.y));
+        }
       }
     }
     su3_third_row(w);
+    if constexpr (RAW) {
+#pragma unroll
+      for (int j = 0; j < 3; j++) w[2][j] = Cx<R>(w[2][j].re * (R)Q_UNIT, w[2][j].im * (R)Q_UNIT);
+    }
   }
 }
 
@@ -433,14 +449,14 @@ __device__ __forceinline__ void load_clover_block(R (&c)[(2 * D) * (2 * D)], con
   constexpr int n2 = (2 * D) * (2 * D);
   if constexpr (PREC == PREC_F16) {
     const uint2* p = (const uint2*)base + (size_t)blk * (n2 / 4) * stride;
-    const R sc = (R)cnrm[(size_t)blk * stride + site];
+    const R sc = (R)(cnrm[(size_t)blk * stride + site] * Q_UNIT);
 #pragma unroll
     for (int k = 0; k < n2 / 4; k++) {
       uint2 v = p[(size_t)k * stride + site];
-      c[4 * k] = sc * (R)q2f_lo(v.x);
-      c[4 * k + 1] = sc * (R)q2f_hi(v.x);
-      c[4 * k + 2] = sc * (R)q2f_lo(v.y);
-      c[4 * k + 3] = sc * (R)q2f_hi(v.y);
+      c[4 * k] = sc * (R)q2r_lo(v.x);
+      c[4 * k + 1] = sc * (R)q2r_hi(v.x);
+      c[4 * k + 2] = sc * (R)q2r_lo(v.y);
+      c[4 * k + 3] = sc * (R)q2r_hi(v.y);
     }
   } else if constexpr (PREC == PREC_F64) {
     const double2* p = (const double2*)base + (size_t)blk * (n2 / 2) * stride;

@@ -82,9 +82,10 @@ __device__ __forceinline__ void hop_dir(Cx<R> (&acc)[4][D], const HopArgs& a, in
   // loads and write their output with non-temporal stores: measured faster
   // for FP64 (326 -> 301 us, 18 reals, 32^4), neutral for FP32 and slower
   // for FP16, which keep ordinary accesses
-  load_link<PREC, RECON, D, R, PREC == PREC_F64>(u, lbase, lidx, a.g.VHpad);
+  load_link<PREC, RECON, D, R, PREC == PREC_F64, PREC == PREC_F16>(u, lbase, lidx, a.g.VHpad);
+  if constexpr (PREC == PREC_F16) sc *= (R)Q_UNIT;  // the links were loaded as 32768 U
   Cx<R> r[2][D];
-  const R sgn = flip ? -sc : sc;  // antiperiodic sign and 16-bit site scale
+  const R sgn = flip ? -sc : sc;  // antiperiodic sign and 16-bit scales
 #pragma unroll
   for (int i = 0; i < D; i++) {
     Cx<R> r0((R)0, (R)0), r1((R)0, (R)0);
