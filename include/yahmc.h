@@ -88,7 +88,9 @@ typedef struct {
   int max_iter;       /* 0 = no cap (the reference has none) */
   int mvm;            /* YH_MVM_* */
   int sloppy_prec;    /* mixed solvers: YH_PREC_F32 or YH_PREC_F16 */
-  int reliable_every; /* mixed CG: FP64 reliable update every N iterations */
+  int reliable_every; /* mixed / flexible CG: FP64 reliable update at the latest
+                        every N iterations (0 = 200 for the mixed CG, 50 for
+                        it with YH_TUNE_HALF_LINKS and for the flexible CG) */
   double delta;       /* mixed CG: reliable update when |r| drops by delta */
   int log;
 } yh_inv_par;

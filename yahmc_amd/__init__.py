@@ -409,7 +409,7 @@ class Lattice:
 
     # ---- solvers (include/inverter.h) --------------------------------
     @staticmethod
-    def inv_par(mass, prec, mvm="Hphi_sq", max_iter=0, sloppy="f32", reliable_every=50, delta=0.1, log=0):
+    def inv_par(mass, prec, mvm="Hphi_sq", max_iter=0, sloppy="f32", reliable_every=0, delta=0.1, log=0):
         p = InvPar()
         p.mass = mass
         p.prec = prec
