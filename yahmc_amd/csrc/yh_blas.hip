@@ -127,27 +127,29 @@ __global__ void __launch_bounds__(256) k_cg_xr(void* x, float* xn, void* r, floa
   YH_SITE_LOOP_BEGIN
   double v[1] = {0.0};
   if (act) {
-    Cx<R> X[4][D], Pv[4][D];
+    // all four operands are loaded before the first store: the fields are
+    // distinct, but the compiler cannot know that, and a load after a store
+    // would wait for it
+    Cx<R> X[4][D], Pv[4][D], Q[4][D], Rv[4][D];
     load_spinor<PP, D, R>(Pv, p, pn, k, g.VHpad);
     load_spinor<P, D, R>(X, x, xn, k, g.VHpad);
-#pragma unroll
-    for (int s = 0; s < 4; s++)
-#pragma unroll
-      for (int i = 0; i < D; i++) X[s][i] = Cx<R>(X[s][i].re + (R)alpha * Pv[s][i].re, X[s][i].im + (R)alpha * Pv[s][i].im);
-    store_spinor<P, D, R>(x, xn, k, g.VHpad, X);
-    Cx<R> Q[4][D];
     load_spinor<PP, D, R>(Q, q, qn, k, g.VHpad);
-    load_spinor<P, D, R>(X, r, rn, k, g.VHpad);
+    load_spinor<P, D, R>(Rv, r, rn, k, g.VHpad);
 #pragma unroll
     for (int s = 0; s < 4; s++)
 #pragma unroll
-      for (int i = 0; i < D; i++) X[s][i] = Cx<R>(X[s][i].re - (R)alpha * Q[s][i].re, X[s][i].im - (R)alpha * Q[s][i].im);
-    store_spinor<P, D, R>(r, rn, k, g.VHpad, X);
-    if constexpr (P == PREC_F16) load_spinor<P, D, R>(X, r, rn, k, g.VHpad);
+      for (int i = 0; i < D; i++) {
+        X[s][i] = Cx<R>(X[s][i].re + (R)alpha * Pv[s][i].re, X[s][i].im + (R)alpha * Pv[s][i].im);
+        Rv[s][i] = Cx<R>(Rv[s][i].re - (R)alpha * Q[s][i].re, Rv[s][i].im - (R)alpha * Q[s][i].im);
+      }
+    store_spinor<P, D, R>(x, xn, k, g.VHpad, X);
+    store_spinor<P, D, R>(r, rn, k, g.VHpad, Rv);
+    // a 16-bit residual is reduced as stored (quantised)
+    if constexpr (P == PREC_F16) load_spinor<P, D, R>(Rv, r, rn, k, g.VHpad);
 #pragma unroll
     for (int s = 0; s < 4; s++)
 #pragma unroll
-      for (int i = 0; i < D; i++) v[0] += (double)X[s][i].re * X[s][i].re + (double)X[s][i].im * X[s][i].im;
+      for (int i = 0; i < D; i++) v[0] += (double)Rv[s][i].re * Rv[s][i].re + (double)Rv[s][i].im * Rv[s][i].im;
   }
   double res[1];
   if (block_reduce_finish<1, 256>(v, partials, gridDim.x, blockIdx.x, ticket, res)) {
