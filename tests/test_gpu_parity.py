@@ -167,6 +167,27 @@ def test_f16_operators_all_representations(model):
     assert rel(fd.download()[: o.VH], want[: o.VH]) < 2e-4
 
 
+def test_f16_storage_round_trip():
+    """16-bit fixed point: each component is stored as round(32768 c / m) in
+    two's complement, m the site's largest |component|, so the round trip is
+    within 2^-16 m (2^-15 m at +m, which clamps to 32767) and exact at -m and 0."""
+    o, lat, rng = setup("su3_fund", dims=(4, 4, 4, 4))
+    s = src_field(o, rng)
+    s[3] = 0.0                       # all-zero site
+    s[5, 7] = -np.abs(s[5]).max() * 1.5  # the site's extreme is negative: stored as -32768
+    s[9, 2] = np.abs(s[9]).max() * 2.0   # positive extreme: clamps to 32767/32768
+    s[11] *= 1e-20                   # tiny site scale
+    back = lat.field("f16").upload(s).download()
+    m = np.abs(s).max(axis=1, keepdims=True)
+    # a site's largest component clamps when it is positive (q = 32767)
+    bound = np.where(s == m, m * (2.0 ** -15 + 2e-7), m * (2.0 ** -16 + 2e-7))
+    err = np.abs(back - s)
+    assert np.all(err <= bound), np.max(err / np.maximum(m, 1e-300))
+    assert err[9, 2] > m[9, 0] * 2.0 ** -16  # the clamp is visible
+    assert np.all(back[3] == 0.0)
+    assert back[5, 7] == np.float32(s[5, 7])
+
+
 def test_cg_mixed_f16_sextet_true_residual():
     """Mixed CG with a 16-bit sloppy operator on SU(3) two-index symmetric
     fermions reaches the FP64 true residual (prec 1e-20)."""
