@@ -1488,37 +1488,82 @@ double or_kinetic(void)
 	return v * (TF / 2.0);
 }
 
-static gmat reunitarize_g(gmat u)
+static double row_norm(const cplx *r)
 {
-	/* suNg::reunitarize: Gram-Schmidt rows; NC = 3 third row conj(r0 x r1) */
-	cplx r[NC][NC];
-	memcpy(r, u.m, sizeof(r));
-	double n0 = 0;
-	for (int j = 0; j < NC; j++) n0 += creal(r[0][j] * conj(r[0][j]));
-	n0 = sqrt(n0);
-	for (int j = 0; j < NC; j++) r[0][j] /= n0;
-	if (NC == 2) {
-		r[1][0] = -conj(r[0][1]);
-		r[1][1] = conj(r[0][0]);
-	} else {
-		for (int i = 1; i < NC; i++) {
-			if (NC == 3 && i == 2) {
-				r[2][0] = conj(r[0][1] * r[1][2] - r[0][2] * r[1][1]);
-				r[2][1] = conj(r[0][2] * r[1][0] - r[0][0] * r[1][2]);
-				r[2][2] = conj(r[0][0] * r[1][1] - r[0][1] * r[1][0]);
-				break;
-			}
-			for (int k = 0; k < i; k++) {
-				cplx p = 0;
-				for (int j = 0; j < NC; j++) p += conj(r[k][j]) * r[i][j];
-				for (int j = 0; j < NC; j++) r[i][j] -= p * r[k][j];
-			}
-			double n = 0;
-			for (int j = 0; j < NC; j++) n += creal(r[i][j] * conj(r[i][j]));
-			n = sqrt(n);
-			for (int j = 0; j < NC; j++) r[i][j] /= n;
+	double n = 0;
+	for (int j = 0; j < NC; j++) n += creal(r[j] * conj(r[j]));
+	return sqrt(n);
+}
+
+/* determinant by Gaussian elimination with partial pivoting (the reference
+ * uses an unpivoted LU, suNg.cpp:296-330; the value is the same) */
+static cplx det_g(cplx a[NC][NC])
+{
+	cplx m[NC][NC];
+	memcpy(m, a, sizeof(m));
+	cplx det = 1;
+	for (int c = 0; c < NC; c++) {
+		int piv = c;
+		for (int i = c + 1; i < NC; i++)
+			if (cabs(m[i][c]) > cabs(m[piv][c])) piv = i;
+		if (piv != c) {
+			for (int j = 0; j < NC; j++) { cplx t = m[c][j]; m[c][j] = m[piv][j]; m[piv][j] = t; }
+			det = -det;
+		}
+		if (m[c][c] == 0) return 0;
+		det *= m[c][c];
+		for (int i = c + 1; i < NC; i++) {
+			cplx f = m[i][c] / m[c][c];
+			for (int j = c; j < NC; j++) m[i][j] -= f * m[c][j];
 		}
 	}
+	return det;
+}
+
+/* suNg::reunitarize (suNg.cpp:171-359).  A zero row returns the input
+ * unchanged (the reference returns early there, with the rows before it
+ * already processed in the general branch).
+ *   NC = 2: normalise row 0, row 1 = (-conj r01, conj r00)      (171-196)
+ *   NC = 3: normalise row 0, orthogonalise and normalise row 1,
+ *           row 2 = conj(r0 x r1)                               (197-252)
+ *   other:  modified Gram-Schmidt over all rows, then the last row
+ *           times exp(-i arg det) so that det = 1               (253-358) */
+static gmat reunitarize_g(gmat u)
+{
+	cplx r[NC][NC];
+	memcpy(r, u.m, sizeof(r));
+#if NC == 2 || NC == 3
+	double n0 = row_norm(r[0]);
+	if (n0 == 0) return u;
+	for (int j = 0; j < NC; j++) r[0][j] /= n0;
+#if NC == 2
+	r[1][0] = -conj(r[0][1]);
+	r[1][1] = conj(r[0][0]);
+#else
+	cplx p = 0;
+	for (int j = 0; j < NC; j++) p += conj(r[0][j]) * r[1][j];
+	for (int j = 0; j < NC; j++) r[1][j] -= p * r[0][j];
+	double n1 = row_norm(r[1]);
+	for (int j = 0; j < NC; j++) r[1][j] /= n1;
+	r[2][0] = conj(r[0][1] * r[1][2] - r[0][2] * r[1][1]);
+	r[2][1] = conj(r[0][2] * r[1][0] - r[0][0] * r[1][2]);
+	r[2][2] = conj(r[0][0] * r[1][1] - r[0][1] * r[1][0]);
+#endif
+#else
+	for (int i = 0; i < NC; i++) {
+		double n = row_norm(r[i]);
+		if (n == 0) return u;
+		for (int j = 0; j < NC; j++) r[i][j] /= n;
+		for (int k = i + 1; k < NC; k++) {
+			cplx p = 0;
+			for (int j = 0; j < NC; j++) p += conj(r[i][j]) * r[k][j];
+			for (int j = 0; j < NC; j++) r[k][j] -= p * r[i][j];
+		}
+	}
+	cplx d = det_g(r);
+	cplx ph = conj(d) / cabs(d);
+	for (int j = 0; j < NC; j++) r[NC - 1][j] *= ph;
+#endif
 	gmat o;
 	memcpy(o.m, r, sizeof(r));
 	return o;

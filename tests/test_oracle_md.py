@@ -6,7 +6,9 @@ integrator.cpp:42-63, suNa.cpp:113-134):
     S_g = -beta * plaq * 6V (finite differences along random directions);
   - the link update equals scipy's matrix exponential;
   - a 2nd-order minimal-norm trajectory conserves H up to O(dt^2) and is
-    reversible."""
+    reversible.
+Every check runs for N = 2, 3 and 4 (the models su2_fund, su3_fund and
+su4_asym; the gauge MD uses only the fundamental links)."""
 import numpy as np
 import pytest
 from scipy.linalg import expm
@@ -18,16 +20,21 @@ from test_oracle_forces import alg_to_mat
 TF = 2.0
 
 
-def make(seed=4, dims=(4, 4, 4, 4)):
-    o = Oracle("su3_fund").init(dims, bc=BC_PERIODIC, csw=0.0)
+MODELS = pytest.mark.parametrize("model,nc", [("su2_fund", 2), ("su3_fund", 3), ("su4_asym", 4)])
+
+
+def make(model, nc, seed=4, dims=(4, 4, 4, 4)):
+    o = Oracle(model).init(dims, bc=BC_PERIODIC, csw=0.0)
+    assert o.nc == nc
     rng = np.random.default_rng(seed)
-    o.set_gauge(random_gauge(rng, o.V, 3))
+    o.set_gauge(random_gauge(rng, o.V, nc))
     return o, rng
 
 
-def test_plaquette():
-    o, rng = make()
-    U = mat_to_complex(o.get_gauge(), 3).reshape(o.V, 4, 3, 3)
+@MODELS
+def test_plaquette(model, nc):
+    o, rng = make(model, nc)
+    U = mat_to_complex(o.get_gauge(), nc).reshape(o.V, 4, nc, nc)
     _, fw, _ = eo_geometry(o.dims)
     s = 0.0
     for x in range(o.V):
@@ -35,21 +42,22 @@ def test_plaquette():
             for nu in range(mu):
                 P = U[x, mu] @ U[fw[x, mu], nu] @ U[fw[x, nu], mu].conj().T @ U[x, nu].conj().T
                 s += np.trace(P).real
-    assert abs(o.plaquette() - s / (3 * 6 * o.V)) < 1e-14
-    o.set_gauge(unit_gauge(o.V, 3))
+    assert abs(o.plaquette() - s / (nc * 6 * o.V)) < 1e-14
+    o.set_gauge(unit_gauge(o.V, nc))
     assert abs(o.plaquette() - 1.0) < 1e-15
 
 
-def test_gauge_force_is_minus_gradient():
-    o, rng = make(seed=8)
+@MODELS
+def test_gauge_force_is_minus_gradient(model, nc):
+    o, rng = make(model, nc, seed=8)
     beta = 5.6
     links = o.get_gauge()
-    o.set_momenta(np.zeros((4 * o.V, 8)))
+    o.set_momenta(np.zeros((4 * o.V, nc * nc - 1)))
     o.wilson_gauge_force(1.0, beta)
     F = o.get_momenta()
     Q = rng.normal(size=F.shape)
-    U = mat_to_complex(links, 3)
-    XQ = np.array([alg_to_mat(q, 3) for q in Q])
+    U = mat_to_complex(links, nc)
+    XQ = np.array([alg_to_mat(q, nc) for q in Q])
 
     def S(tau):
         o.set_gauge(complex_to_mat(np.array([expm(tau * x) @ u for x, u in zip(XQ, U)])))
@@ -62,15 +70,17 @@ def test_gauge_force_is_minus_gradient():
     assert abs(dS - want) < 1e-8 * scale
 
 
-def test_update_links_is_matrix_exponential():
-    o, rng = make(seed=2)
-    P = rng.normal(size=(4 * o.V, 8))
+@MODELS
+def test_update_links_is_matrix_exponential(model, nc):
+    o, rng = make(model, nc, seed=2)
+    P = rng.normal(size=(4 * o.V, nc * nc - 1))
     o.set_momenta(P)
-    U0 = mat_to_complex(o.get_gauge(), 3)
+    U0 = mat_to_complex(o.get_gauge(), nc)
     o.update_links(0.13)
-    U1 = mat_to_complex(o.get_gauge(), 3)
+    U1 = mat_to_complex(o.get_gauge(), nc)
     for k in range(0, 4 * o.V, 37):
-        assert np.abs(U1[k] - expm(0.13 * alg_to_mat(P[k], 3)) @ U0[k]).max() < 1e-13
+        assert np.abs(U1[k] - expm(0.13 * alg_to_mat(P[k], nc)) @ U0[k]).max() < 1e-13
+        assert abs(np.linalg.det(U1[k]) - 1.0) < 1e-13
 
 
 def o2mn(o, beta, length, steps):
@@ -89,11 +99,12 @@ def hamiltonian(o, beta):
     return o.kinetic() + o.gauge_action(beta)
 
 
-def test_o2mn_energy_conservation_scales_as_dt2_and_is_reversible():
-    o, rng = make(seed=3)
+@MODELS
+def test_o2mn_energy_conservation_scales_as_dt2_and_is_reversible(model, nc):
+    o, rng = make(model, nc, seed=3)
     beta = 5.6
     U0 = o.get_gauge()
-    P0 = rng.normal(size=(4 * o.V, 8))
+    P0 = rng.normal(size=(4 * o.V, nc * nc - 1))
     dH = []
     for steps in (4, 8):
         o.set_gauge(U0)
