@@ -367,6 +367,10 @@ int yh_synchronize(yh_ctx* ctx);
  * (spinors and clover stay FP32): the sloppy operator of the mixed CG moves
  * 384 instead of 576 bytes per hop site */
 #define YH_TUNE_HALF_LINKS 4
+/* 1: FP64 operators of SU(3) fundamental fields read 12-real links (the third
+ * row rebuilt as conj(r0 x r1)): 1,152 instead of 1,536 bytes per hop site;
+ * 0 (default): all 18 reals */
+#define YH_TUNE_RECON12_F64 5
 int yh_tune(yh_ctx* ctx, int key, int value);
 /* autotuner (no counterpart in the reference): times the hop of every
  * precision / reconstruction this context holds for each XCD chunk size

@@ -171,9 +171,11 @@ static void free_scratch(Ctx* c) {
 
 int default_recon(Ctx* c, int prec) {
   // SU(3) fundamental sloppy operators use 12-real links; FP64 keeps the
-  // full 18 reals so that operator parity with the reference is exact.
-  if (prec != PREC_F64 && c->nc == 3 && c->repr == YH_REPR_FUNDAMENTAL)
-    return prec == PREC_F32 && c->half_links ? RECON_12H : RECON_12;
+  // full 18 reals unless YH_TUNE_RECON12_F64 asks for 12.
+  if (c->nc == 3 && c->repr == YH_REPR_FUNDAMENTAL) {
+    if (prec != PREC_F64) return prec == PREC_F32 && c->half_links ? RECON_12H : RECON_12;
+    if (c->recon12_f64) return RECON_12;
+  }
   return RECON_18;
 }
 
@@ -216,6 +218,12 @@ int apply_op(Ctx* c, int prec, int recon, int mvm, double mass, Field* out, Fiel
 // ---------------------------------------------------------------------------
 // scalars
 // ---------------------------------------------------------------------------
+int upload_table(Ctx* c, double2* dst, const double2* host, size_t n) {
+  YH_HIP(hipMemcpyAsync(dst, host, sizeof(double2) * n, hipMemcpyHostToDevice, c->stream));
+  YH_HIP(hipStreamSynchronize(c->stream));
+  return YH_OK;
+}
+
 int read_scalars(Ctx* c) {
   YH_HIP(hipMemcpyAsync(c->sc_host, c->sc, sizeof(SolverScalars), hipMemcpyDeviceToHost, c->stream));
   YH_HIP(hipStreamSynchronize(c->stream));
@@ -278,7 +286,9 @@ int cg_device(Ctx* c, int prec, int recon, const yh_inv_par* par, Field* x, cons
   int done = 0;
   const bool sq = par->mvm == YH_MVM_HPHI_SQ;
   while (true) {
-    for (int i = 0; i < chunk; i++) {
+    // the last chunk stops at the iteration cap
+    const int n = par->max_iter > 0 ? std::min(chunk, par->max_iter - done) : chunk;
+    for (int i = 0; i < n; i++) {
       if ((rc = apply_op(c, prec, recon, par->mvm, par->mass, qq, p, nullptr, sq ? 0 : -1, &c->sc->converged))) return rc;
       if (!sq) {
         if ((rc = blas_dot_re(c, p, qq, 0, 0))) return rc;  // pap into red[0]
@@ -286,8 +296,10 @@ int cg_device(Ctx* c, int prec, int recon, const yh_inv_par* par, Field* x, cons
       if ((rc = cg_update_xr(c, x, r, p, qq, 0, prec))) return rc;
       if ((rc = cg_update_p(c, p, r, 0, prec))) return rc;
     }
-    done += chunk;
     if ((rc = read_scalars(c))) return rc;
+    // every iteration that ran applied the operator once
+    c->mvm_counter += (long)(h->iters - done) * hops_per_apply(par->mvm);
+    done = h->iters;
     if (!(h->rr == h->rr)) { set_error("cg: non-finite residual"); return YH_ERR_NONFINITE; }
     if (h->converged) break;
     if (par->max_iter > 0 && done >= par->max_iter) {
@@ -361,11 +373,15 @@ int yh_create(const yh_lattice* lat, yh_ctx** out) {
   if (e == hipSuccess) e = hipMemset(c->sc, 0, sizeof(SolverScalars));
   if (e == hipSuccess) e = hipHostMalloc((void**)&c->sc_host, sizeof(SolverScalars), hipHostMallocDefault);
   if (e == hipSuccess) e = hipMalloc(&c->fund, sizeof(double) * 4 * (size_t)V * 2 * lat->nc * lat->nc);
+  if (e == hipSuccess) e = hipMalloc(&c->tab, sizeof(double2) * (TAB_BASIS + TAB_ITR + TAB_ITF));
   if (e != hipSuccess) {
     set_error(std::string("context allocation: ") + hipGetErrorString(e));
     yh_destroy(c);
     return YH_ERR_HIP;
   }
+  c->tab_basis = c->tab;
+  c->tab_itr = c->tab + TAB_BASIS;
+  c->tab_itf = c->tab + TAB_BASIS + TAB_ITR;
   int rc = gauge_init_basis(c);
   if (!rc) rc = force_init_constants(c);
   if (!rc) rc = md_init_constants(c);
@@ -393,6 +409,7 @@ int yh_destroy(yh_ctx* c) {
   for (int i = 4; i < 6; i++)
     if (c->tmp[i]) { (void)hipFree(c->tmp[i]->mem); delete static_cast<yh_field*>(c->tmp[i]); }
   if (c->fund) (void)hipFree(c->fund);
+  if (c->tab) (void)hipFree(c->tab);
   if (c->logdet_site) (void)hipFree(c->logdet_site);
   if (c->mom) (void)hipFree(c->mom);
   if (c->clf) (void)hipFree(c->clf);
@@ -606,7 +623,7 @@ static int par_check(yh_ctx* c, const yh_inv_par* par, const yh_field* d, const 
 int yh_cg(yh_ctx* c, const yh_inv_par* par, yh_field* dst, const yh_field* src, int* iters) {
   int rc = par_check(c, par, dst, src, iters);
   if (rc) return rc;
-  return cg_device(c, PREC_F64, RECON_18, par, dst, src, iters, nullptr, nullptr);
+  return cg_device(c, PREC_F64, default_recon(c, PREC_F64), par, dst, src, iters, nullptr, nullptr);
 }
 
 int yh_tune(yh_ctx* c, int key, int value) {
@@ -626,6 +643,14 @@ int yh_tune(yh_ctx* c, int key, int value) {
   }
   if (key == YH_TUNE_HALF_LINKS) {
     c->half_links = value != 0;
+    return YH_OK;
+  }
+  if (key == YH_TUNE_RECON12_F64) {
+    if (value && !(c->nc == 3 && c->repr == YH_REPR_FUNDAMENTAL)) {
+      set_error("12-real links need SU(3) fundamental");
+      return YH_ERR_UNSUPPORTED;
+    }
+    c->recon12_f64 = value != 0;
     return YH_OK;
   }
   set_error("unknown tuning key");

@@ -80,7 +80,11 @@ int hop_launch(Ctx* c, const HopCall& h) {
   a.skip = h.skip_flag;
   a.g = c->g;
   if (c->xq_tab[h.prec][h.recon] >= 0) a.g.xq = c->xq_tab[h.prec][h.recon];
-  c->mvm_counter++;
+  // the reference counts every dirac_kernel call (dirac.cpp:233); a launch
+  // with a skip flag may return at once (solver chunks queued past
+  // convergence), so its solver counts the hops that ran from the device
+  // iteration counter instead
+  if (!h.skip_flag) c->mvm_counter++;
   switch (c->dr) {
     case 2: return hop_dispatch_d<2>(c, a, h.prec, h.recon, h.epi);
     case 3: return hop_dispatch_d<3>(c, a, h.prec, h.recon, h.epi);

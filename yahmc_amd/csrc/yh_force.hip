@@ -12,11 +12,13 @@
 // index), clover force matrices clf[(plane * D * D + e) * V + id] (double2),
 // planes in the reference order 01, 02, 12, 03, 13, 23.
 #include <vector>
+#include <mutex>
 #include "yh_internal.hpp"
 
 namespace yh {
 
-__constant__ double2 c_itr[15 * 36];     // iT^R_a, a < NG, D x D row-major
+// represented generators iT^R_a, a < NG, D x D row-major: the context's
+// table tab_itr[(a * D + i) * D + k], passed to the kernels
 __constant__ double2 c_g5omg[4 * 16];    // gamma5 (1 - gamma_mu)
 __constant__ double2 c_g5sig[6 * 16];    // gamma5 (1/2)[gamma_mu, gamma_nu], reference plane order
 
@@ -122,9 +124,17 @@ static void gamma_mats(cd g[4][4][4]) {
 int force_init_constants(Ctx* c) {
   std::vector<cd> itr;
   rep_generators(c->nc, c->repr, c->dr, itr);
-  std::vector<double2> h(15 * 36, make_double2(0, 0));
+  std::vector<double2> h(TAB_ITR, make_double2(0, 0));
+  if (itr.size() > h.size()) { set_error("generator table too small"); return YH_ERR_UNSUPPORTED; }
   for (size_t i = 0; i < itr.size(); i++) h[i] = make_double2(itr[i].re, itr[i].im);
-  YH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_itr), h.data(), sizeof(double2) * h.size()));
+  int rc = upload_table(c, c->tab_itr, h.data(), h.size());
+  if (rc) return rc;
+  // gamma5 (1 - gamma_mu) and gamma5 sigma_mu_nu do not depend on the model:
+  // uploaded once per device
+  static std::mutex mtx;
+  static unsigned done_mask = 0;
+  std::lock_guard<std::mutex> lk(mtx);
+  if (c->device < 32 && (done_mask >> c->device) & 1u) return YH_OK;
   cd g[4][4][4];
   gamma_mats(g);
   const double g5[4] = {1, 1, -1, -1};
@@ -147,6 +157,8 @@ int force_init_constants(Ctx* c) {
   }
   YH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_g5omg), om.data(), sizeof(double2) * om.size()));
   YH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_g5sig), sg.data(), sizeof(double2) * sg.size()));
+  YH_HIP(hipDeviceSynchronize());
+  if (c->device < 32) done_mask |= 1u << c->device;
   return YH_OK;
 }
 
@@ -247,7 +259,8 @@ __device__ __forceinline__ void mmul(cd (&R)[D][D], const cd (&A)[D][D], const c
 
 // mom_a += -(coeff / TF) Re Tr(iT^R_a F)
 template <int NG, int D>
-__device__ __forceinline__ void project_add(double* mom, int mu, int id, int V, const cd (&F)[D][D], double coeff) {
+__device__ __forceinline__ void project_add(double* mom, int mu, int id, int V, const cd (&F)[D][D], double coeff,
+                                            const double2* __restrict__ itr) {
 #pragma unroll
   for (int a = 0; a < NG; a++) {
     double re = 0.0;
@@ -255,7 +268,7 @@ __device__ __forceinline__ void project_add(double* mom, int mu, int id, int V, 
     for (int i = 0; i < D; i++)
 #pragma unroll
       for (int k = 0; k < D; k++) {
-        double2 t = c_itr[(a * D + i) * D + k];
+        double2 t = itr[(a * D + i) * D + k];
         re += t.x * F[k][i].re - t.y * F[k][i].im;
       }
     mom[(size_t)(mu * NG + a) * V + id] += -(coeff * 0.5) * re;
@@ -368,7 +381,7 @@ __device__ __forceinline__ void mul_am(cd (&C)[D][D], const cd (&A)[D][D], const
 // hopping force, one thread per (site, mu)  (forces.cpp:204-223)
 template <int NG, int D>
 __global__ void __launch_bounds__(256) k_force_hop(FF f, const double2* __restrict__ links, double* __restrict__ mom,
-                                                   double coeff, Geom g) {
+                                                   double coeff, Geom g, const double2* __restrict__ itr) {
   int t = blockIdx.x * 256 + threadIdx.x;
   if (t >= 4 * g.V) return;
   int id = t >> 2, mu = t & 3;
@@ -396,7 +409,7 @@ __global__ void __launch_bounds__(256) k_force_hop(FF f, const double2* __restri
   }
   cd R[D][D];
   mul_mb<D, false>(R, after<D>(link_src<D>(links, id, mu, false, g), F), F, 1.0);
-  project_add<NG, D>(mom, mu, id, g.V, R, coeff);
+  project_add<NG, D>(mom, mu, id, g.V, R, coeff, itr);
 }
 
 // clover force matrices, one thread per site  (forces.cpp:130-168)
@@ -520,7 +533,7 @@ __device__ __forceinline__ void madd(cd (&A)[D][D], const cd (&B)[D][D], double 
 template <int NG, int D>
 __global__ void __launch_bounds__(128) k_force_clover_core(const double2* __restrict__ clf,
                                                            const double2* __restrict__ links, double* __restrict__ mom,
-                                                           double coeff, Geom g) {
+                                                           double coeff, Geom g, const double2* __restrict__ itr) {
   int t = blockIdx.x * 128 + threadIdx.x;
   if (t >= 4 * g.V) return;
   int id = t >> 2, mu = t & 3;
@@ -573,7 +586,7 @@ __global__ void __launch_bounds__(128) k_force_clover_core(const double2* __rest
     mul_am<D, true>(Fm, A, after<D>(w4, A), 1.0);   // Fm += A W4
     // U(x, mu) Fm, projected onto the algebra
     mul_mb<D, false>(A, after<D>(link_src<D>(links, id, mu, false, g), Fm), Fm, 1.0);
-    project_add<NG, D>(mom, mu, id, g.V, A, sign * coeff);
+    project_add<NG, D>(mom, mu, id, g.V, A, sign * coeff, itr);
   }
 }
 
@@ -637,7 +650,7 @@ int force_hop(Ctx* c, double coeff, const Field* lhs, const Field* rhs) {
   f.y[0] = rhs->half[0]; f.y[1] = rhs->half[1];
   const double2* links = (const double2*)c->links[PREC_F64][RECON_18];
   dim3 grid((4 * c->g.V + 255) / 256);
-  YH_FORCE_MODELS(hipLaunchKernelGGL((k_force_hop<NG_, D_>), grid, dim3(256), 0, c->stream, f, links, c->mom, coeff, c->g));
+  YH_FORCE_MODELS(hipLaunchKernelGGL((k_force_hop<NG_, D_>), grid, dim3(256), 0, c->stream, f, links, c->mom, coeff, c->g, c->tab_itr));
   YH_HIP(hipGetLastError());
   return YH_OK;
 }
@@ -684,7 +697,7 @@ int force_clover_core(Ctx* c, double dt) {
   const double2* links = (const double2*)c->links[PREC_F64][RECON_18];
   dim3 grid((4 * c->g.V + 127) / 128);
   YH_FORCE_MODELS(hipLaunchKernelGGL((k_force_clover_core<NG_, D_>), grid, dim3(128), 0, c->stream, (const double2*)c->clf,
-                                     links, c->mom, coeff, c->g));
+                                     links, c->mom, coeff, c->g, c->tab_itr));
   YH_HIP(hipGetLastError());
   return YH_OK;
 }

@@ -16,8 +16,8 @@
 
 namespace yh {
 
-// representation basis b_n (DR <= 15, NC <= 4) and whether it is used
-__constant__ double2 c_basis[15 * 16];
+// representation basis b_n (DR <= 15, NC <= 4): the context's table
+// tab_basis[n * 16 + row * NC + col]
 
 static const double kSqrtHalf = 0.70710678118654752440;
 
@@ -68,8 +68,7 @@ int gauge_init_basis(Ctx* c) {
       for (int k = 0; k < nc * nc; k++) b[n * 16 + k] = make_double2(kSqrtHalf * g[k].y, -kSqrtHalf * g[k].x);
     }
   }
-  YH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_basis), b.data(), sizeof(double2) * b.size()));
-  return YH_OK;
+  return upload_table(c, c->tab_basis, b.data(), b.size());
 }
 
 typedef Cx<double> cd;
@@ -124,7 +123,7 @@ __device__ __forceinline__ void mdm(cd (&r)[NC][NC], const cd (&a)[NC][NC], cons
 
 // R(U) for the fermion representation
 template <int NC, int REPR, int D>
-__device__ __forceinline__ void represent(cd (&R)[D][D], const cd (&U)[NC][NC]) {
+__device__ __forceinline__ void represent(cd (&R)[D][D], const cd (&U)[NC][NC], const double2* __restrict__ basis) {
   if constexpr (REPR == YH_REPR_FUNDAMENTAL) {
 #pragma unroll
     for (int i = 0; i < NC; i++)
@@ -141,7 +140,7 @@ __device__ __forceinline__ void represent(cd (&R)[D][D], const cd (&U)[NC][NC]) 
           cd s(0, 0);
 #pragma unroll
           for (int q = 0; q < NC; q++) {
-            double2 bb = c_basis[m * 16 + q * NC + d];
+            double2 bb = basis[m * 16 + q * NC + d];
             cmac(s, U[a][q], cd(bb.x, bb.y));
           }
           W[a][d] = s;
@@ -165,10 +164,10 @@ __device__ __forceinline__ void represent(cd (&R)[D][D], const cd (&U)[NC][NC]) 
 #pragma unroll
           for (int b = 0; b < NC; b++) {
             if constexpr (REPR == YH_REPR_ADJOINT) {
-              double2 bb = c_basis[n * 16 + b * NC + a];  // Tr(b_n X) = sum (b_n)_{ba} X_{ab}
+              double2 bb = basis[n * 16 + b * NC + a];  // Tr(b_n X) = sum (b_n)_{ba} X_{ab}
               cmac(s, cd(bb.x, bb.y), X[a][b]);
             } else {
-              double2 bb = c_basis[n * 16 + a * NC + b];
+              double2 bb = basis[n * 16 + a * NC + b];
               cmac_conj(s, cd(bb.x, bb.y), X[a][b]);
             }
           }
@@ -180,13 +179,14 @@ __device__ __forceinline__ void represent(cd (&R)[D][D], const cd (&U)[NC][NC]) 
 
 // fund links (reference AoS) -> F64 recon-18 SoA links (no BC sign)
 template <int NC, int REPR, int D>
-__global__ void k_represent(const double* __restrict__ fund, double2* __restrict__ out, Geom g) {
+__global__ void k_represent(const double* __restrict__ fund, double2* __restrict__ out, Geom g,
+                            const double2* __restrict__ basis) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= 4 * g.V) return;
   int id = t >> 2, mu = t & 3;
   cd U[NC][NC], R[D][D];
   load_fund<NC>(U, fund, id, mu);
-  represent<NC, REPR, D>(R, U);
+  represent<NC, REPR, D>(R, U, basis);
   int p = id >= g.VH;
   int k = id - p * g.VH;
 #pragma unroll
@@ -205,7 +205,8 @@ __device__ __forceinline__ int shift(const int c[4], int mu, int d, const Geom& 
 // sum of the four plaquette leaves at x in the (mu, nu) plane, represented
 // (library/wilsonloop.cpp:5-31)
 template <int NC, int REPR, int D>
-__device__ void leaf_sum(cd (&Q)[D][D], const double* fund, int id, const int c[4], int mu, int nu, const Geom& g) {
+__device__ void leaf_sum(cd (&Q)[D][D], const double* fund, int id, const int c[4], int mu, int nu, const Geom& g,
+                         const double2* __restrict__ basis) {
   cd A[NC][NC], B[NC][NC], T1[NC][NC], T2[NC][NC], L[NC][NC], R[D][D];
 #pragma unroll
   for (int i = 0; i < D; i++)
@@ -228,7 +229,7 @@ __device__ void leaf_sum(cd (&Q)[D][D], const double* fund, int id, const int c[
   load_fund<NC>(B, fund, xn, mu);
   mm<NC>(T2, A, B);
   mmd<NC>(L, T1, T2);
-  represent<NC, REPR, D>(R, L);
+  represent<NC, REPR, D>(R, L, basis);
 #pragma unroll
   for (int i = 0; i < D; i++)
 #pragma unroll
@@ -241,7 +242,7 @@ __device__ void leaf_sum(cd (&Q)[D][D], const double* fund, int id, const int c[
   mmd<NC>(T2, A, T1);
   load_fund<NC>(A, fund, bm, mu);
   mm<NC>(L, T2, A);
-  represent<NC, REPR, D>(R, L);
+  represent<NC, REPR, D>(R, L, basis);
 #pragma unroll
   for (int i = 0; i < D; i++)
 #pragma unroll
@@ -254,7 +255,7 @@ __device__ void leaf_sum(cd (&Q)[D][D], const double* fund, int id, const int c[
   mdm<NC>(T2, T1, A);
   load_fund<NC>(A, fund, bn, nu);
   mm<NC>(L, T2, A);
-  represent<NC, REPR, D>(R, L);
+  represent<NC, REPR, D>(R, L, basis);
 #pragma unroll
   for (int i = 0; i < D; i++)
 #pragma unroll
@@ -267,7 +268,7 @@ __device__ void leaf_sum(cd (&Q)[D][D], const double* fund, int id, const int c[
   mm<NC>(T2, T1, A);
   load_fund<NC>(A, fund, id, mu);
   mmd<NC>(L, T2, A);
-  represent<NC, REPR, D>(R, L);
+  represent<NC, REPR, D>(R, L, basis);
 #pragma unroll
   for (int i = 0; i < D; i++)
 #pragma unroll
@@ -315,7 +316,8 @@ __device__ __forceinline__ void load_packed(cd (&M)[N][N], const double2* base, 
 // (library/clover.cpp:120-168; the dense form is -(c_sw/16) sum_{mu<nu}
 //  (1/2)[g_mu, g_nu] (x) (Q_mu_nu - Q_mu_nu^+))
 template <int NC, int REPR, int D>
-__global__ void k_clover_build(const double* __restrict__ fund, double2* __restrict__ out, Geom g, double csw) {
+__global__ void k_clover_build(const double* __restrict__ fund, double2* __restrict__ out, Geom g, double csw,
+                               const double2* __restrict__ basis) {
   int id = blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= g.V) return;
   constexpr int N = 2 * D;
@@ -335,32 +337,32 @@ __global__ void k_clover_build(const double* __restrict__ fund, double2* __restr
   }
   // plane order of clover.cpp:135-140: 01, 02, 03, 12, 13, 23
   // at: from Q03; bt: Q12; ct: Q01, Q02; dt: Q13, Q23 (clover.cpp:149-156)
-  leaf_sum<NC, REPR, D>(Q, fund, id, c, 0, 3, g);
+  leaf_sum<NC, REPR, D>(Q, fund, id, c, 0, 3, g, basis);
 #pragma unroll
   for (int i = 0; i < D; i++)
 #pragma unroll
     for (int j = 0; j < D; j++) at[i][j] = cd(Q[i][j].im + Q[j][i].im, Q[j][i].re - Q[i][j].re);
-  leaf_sum<NC, REPR, D>(Q, fund, id, c, 1, 2, g);
+  leaf_sum<NC, REPR, D>(Q, fund, id, c, 1, 2, g, basis);
 #pragma unroll
   for (int i = 0; i < D; i++)
 #pragma unroll
     for (int j = 0; j < D; j++) bt[i][j] = cd(Q[i][j].im + Q[j][i].im, Q[i][j].re - Q[j][i].re);
-  leaf_sum<NC, REPR, D>(Q, fund, id, c, 0, 1, g);
+  leaf_sum<NC, REPR, D>(Q, fund, id, c, 0, 1, g, basis);
 #pragma unroll
   for (int i = 0; i < D; i++)
 #pragma unroll
     for (int j = 0; j < D; j++) ct[i][j] = cd(Q[i][j].im + Q[j][i].im, Q[j][i].re - Q[i][j].re);
-  leaf_sum<NC, REPR, D>(Q, fund, id, c, 0, 2, g);
+  leaf_sum<NC, REPR, D>(Q, fund, id, c, 0, 2, g, basis);
 #pragma unroll
   for (int i = 0; i < D; i++)
 #pragma unroll
     for (int j = 0; j < D; j++) ct[i][j] = ct[i][j] + cd(-Q[i][j].re + Q[j][i].re, -Q[i][j].im - Q[j][i].im);
-  leaf_sum<NC, REPR, D>(Q, fund, id, c, 1, 3, g);
+  leaf_sum<NC, REPR, D>(Q, fund, id, c, 1, 3, g, basis);
 #pragma unroll
   for (int i = 0; i < D; i++)
 #pragma unroll
     for (int j = 0; j < D; j++) dt[i][j] = cd(Q[i][j].re - Q[j][i].re, Q[i][j].im + Q[j][i].im);
-  leaf_sum<NC, REPR, D>(Q, fund, id, c, 2, 3, g);
+  leaf_sum<NC, REPR, D>(Q, fund, id, c, 2, 3, g, basis);
 #pragma unroll
   for (int i = 0; i < D; i++)
 #pragma unroll
@@ -543,7 +545,7 @@ int gauge_represent(Ctx* c) {
   }
   double2* out = (double2*)c->links[PREC_F64][RECON_18];
   YH_ALL_MODELS(hipLaunchKernelGGL((k_represent<NC_, REPR_, D_>), dim3(grid1(4L * c->g.V)), dim3(256), 0, c->stream,
-                                   c->fund, out, c->g);
+                                   c->fund, out, c->g, c->tab_basis);
                 YH_HIP(hipGetLastError()));
   set_error("unsupported (nc, repr) model");
   return YH_ERR_UNSUPPORTED;
@@ -605,7 +607,7 @@ int clover_build(Ctx* c) {
   int rc = YH_OK;
   auto launch = [&]() -> int {
     YH_ALL_MODELS(hipLaunchKernelGGL((k_clover_build<NC_, REPR_, D_>), dim3(grid_bs(c->g.V, 128)), dim3(128), 0, c->stream,
-                                     c->fund, out, c->g, c->csw);
+                                     c->fund, out, c->g, c->csw, c->tab_basis);
                   YH_HIP(hipGetLastError()));
     set_error("unsupported (nc, repr) model");
     return YH_ERR_UNSUPPORTED;

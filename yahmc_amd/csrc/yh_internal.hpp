@@ -84,9 +84,21 @@ struct Ctx {
   // per (precision, reconstruction) XCD chunk chosen by the autotuner (-1: g.xq)
   int xq_tab[3][3] = {{-1, -1, -1}, {-1, -1, -1}, {-1, -1, -1}};
   int half_links = 0;  // FP32 operators of SU(3) fundamental read FP16 12-real links
+  int recon12_f64 = 0;  // FP64 operators of SU(3) fundamental read 12-real links
   // solver chunks are captured once into a HIP graph and replayed
   int use_graphs = 1;
+  // model tables of this context's (nc, representation), in device memory
+  // and passed to the kernels as arguments (two live contexts of different
+  // models each keep their own): representation basis b_n, represented
+  // generators iT^R_a and fundamental generators iT_a
+  double2* tab = nullptr;
+  double2* tab_basis = nullptr;  // TAB_BASIS entries
+  double2* tab_itr = nullptr;    // TAB_ITR entries
+  double2* tab_itf = nullptr;    // TAB_ITF entries
 };
+enum { TAB_BASIS = 15 * 16, TAB_ITR = 15 * 64, TAB_ITF = 15 * 16 };
+// copies `n` table entries to device memory on the context stream and waits
+int upload_table(Ctx* c, double2* dst, const double2* host, size_t n);
 
 // ---- fields ---------------------------------------------------------------
 int field_alloc(Ctx* c, int prec, Field** out);
@@ -172,6 +184,8 @@ Field* tmp_field(Ctx* c, int prec, int slot, int* rc);
 int default_recon(Ctx* c, int prec);
 int apply_op(Ctx* c, int prec, int recon, int mvm, double mass, Field* out, Field* in, Field* side, int reduce_slot,
              const int* skip, double* reduce_ptr = nullptr);
+// hop launches of one apply_op (Hphi_sq: 4, Dphi / Hphi: 2)
+inline int hops_per_apply(int mvm) { return mvm == YH_MVM_HPHI_SQ ? 4 : 2; }
 int mshift_device(Ctx* c, const yh_inv_par* par, int nrhs, int num, const double* shift, Field** x, const Field** b,
                   int* iters);
 int read_scalars(Ctx* c);

@@ -134,51 +134,118 @@ __device__ __forceinline__ M<NC> alg(const double* mom, int V, int mu, int id) {
   return r;
 }
 
-// row-by-row unitarisation (library/suNg.cpp:171-359)
+// determinant by Gaussian elimination with partial pivoting (the row swaps
+// are selects, so every index stays static); the reference uses an
+// unpivoted LU (suNg.cpp:296-330) for the same value
+template <int NC>
+__device__ __forceinline__ cd det_gauss(M<NC> m) {
+  cd det(1, 0);
+#pragma unroll
+  for (int c = 0; c < NC; c++) {
+#pragma unroll
+    for (int i = c + 1; i < NC; i++) {
+      const double ai = m.m[i][c].re * m.m[i][c].re + m.m[i][c].im * m.m[i][c].im;
+      const double ac = m.m[c][c].re * m.m[c][c].re + m.m[c][c].im * m.m[c][c].im;
+      const bool sw = ai > ac;
+#pragma unroll
+      for (int j = 0; j < NC; j++) {
+        const cd a = m.m[c][j], b = m.m[i][j];
+        m.m[c][j] = sw ? b : a;
+        m.m[i][j] = sw ? a : b;
+      }
+      det = sw ? cd(-det.re, -det.im) : det;
+    }
+    const cd p = m.m[c][c];
+    det = det * p;
+    const double n = 1.0 / (p.re * p.re + p.im * p.im);
+    const cd inv(p.re * n, -p.im * n);
+#pragma unroll
+    for (int i = c + 1; i < NC; i++) {
+      const cd f = m.m[i][c] * inv;
+#pragma unroll
+      for (int j = c; j < NC; j++) m.m[i][j] = m.m[i][j] - f * m.m[c][j];
+    }
+  }
+  return det;
+}
+
+// suNg::reunitarize (library/suNg.cpp:171-359); a zero row returns the
+// input unchanged (the reference returns early there, with the rows before
+// it already processed in the general branch)
+//   NC = 2: normalise row 0, row 1 = (-conj r01, conj r00)
+//   NC = 3: normalise row 0, orthogonalise and normalise row 1,
+//           row 2 = conj(r0 x r1)
+//   other:  modified Gram-Schmidt over all rows, then the last row times
+//           exp(-i arg det) so that det = 1
 template <int NC>
 __device__ __forceinline__ void reunit(M<NC>& u) {
-  double n0 = 0;
+  if constexpr (NC == 2 || NC == 3) {
+    double n0 = 0;
 #pragma unroll
-  for (int j = 0; j < NC; j++) n0 += u.m[0][j].re * u.m[0][j].re + u.m[0][j].im * u.m[0][j].im;
-  n0 = 1.0 / sqrt(n0);
+    for (int j = 0; j < NC; j++) n0 += u.m[0][j].re * u.m[0][j].re + u.m[0][j].im * u.m[0][j].im;
+    if (n0 == 0.0) return;
+    n0 = 1.0 / sqrt(n0);
 #pragma unroll
-  for (int j = 0; j < NC; j++) u.m[0][j] = n0 * u.m[0][j];
-  if constexpr (NC == 2) {
-    u.m[1][0] = cd(-u.m[0][1].re, u.m[0][1].im);
-    u.m[1][1] = conj(u.m[0][0]);
+    for (int j = 0; j < NC; j++) u.m[0][j] = n0 * u.m[0][j];
+    if constexpr (NC == 2) {
+      u.m[1][0] = cd(-u.m[0][1].re, u.m[0][1].im);
+      u.m[1][1] = conj(u.m[0][0]);
+    } else {
+      cd p(0, 0);
+#pragma unroll
+      for (int j = 0; j < NC; j++) cmac_conj(p, u.m[0][j], u.m[1][j]);
+#pragma unroll
+      for (int j = 0; j < NC; j++) u.m[1][j] = u.m[1][j] - p * u.m[0][j];
+      double n1 = 0;
+#pragma unroll
+      for (int j = 0; j < NC; j++) n1 += u.m[1][j].re * u.m[1][j].re + u.m[1][j].im * u.m[1][j].im;
+      n1 = 1.0 / sqrt(n1);
+#pragma unroll
+      for (int j = 0; j < NC; j++) u.m[1][j] = n1 * u.m[1][j];
+      u.m[2][0] = conj(u.m[0][1] * u.m[1][2] - u.m[0][2] * u.m[1][1]);
+      u.m[2][1] = conj(u.m[0][2] * u.m[1][0] - u.m[0][0] * u.m[1][2]);
+      u.m[2][2] = conj(u.m[0][0] * u.m[1][1] - u.m[0][1] * u.m[1][0]);
+    }
   } else {
+    const M<NC> orig = u;
+    bool zero = false;
 #pragma unroll
-    for (int i = 1; i < NC; i++) {
-      if (NC == 3 && i == 2) {
-        u.m[2][0] = conj(u.m[0][1] * u.m[1][2] - u.m[0][2] * u.m[1][1]);
-        u.m[2][1] = conj(u.m[0][2] * u.m[1][0] - u.m[0][0] * u.m[1][2]);
-        u.m[2][2] = conj(u.m[0][0] * u.m[1][1] - u.m[0][1] * u.m[1][0]);
-        break;
-      }
-#pragma unroll
-      for (int k = 0; k < i; k++) {
-        cd p(0, 0);
-#pragma unroll
-        for (int j = 0; j < NC; j++) cmac_conj(p, u.m[k][j], u.m[i][j]);
-#pragma unroll
-        for (int j = 0; j < NC; j++) u.m[i][j] = u.m[i][j] - p * u.m[k][j];
-      }
+    for (int i = 0; i < NC; i++) {
       double n = 0;
 #pragma unroll
       for (int j = 0; j < NC; j++) n += u.m[i][j].re * u.m[i][j].re + u.m[i][j].im * u.m[i][j].im;
-      n = 1.0 / sqrt(n);
+      zero = zero || n == 0.0;
+      n = n == 0.0 ? 1.0 : 1.0 / sqrt(n);
 #pragma unroll
       for (int j = 0; j < NC; j++) u.m[i][j] = n * u.m[i][j];
+#pragma unroll
+      for (int k = i + 1; k < NC; k++) {
+        cd p(0, 0);
+#pragma unroll
+        for (int j = 0; j < NC; j++) cmac_conj(p, u.m[i][j], u.m[k][j]);
+#pragma unroll
+        for (int j = 0; j < NC; j++) u.m[k][j] = u.m[k][j] - p * u.m[i][j];
+      }
     }
+    if (zero) {
+      u = orig;
+      return;
+    }
+    const cd d = det_gauss<NC>(u);
+    const double inv = 1.0 / sqrt(d.re * d.re + d.im * d.im);
+    const cd ph(d.re * inv, -d.im * inv);
+#pragma unroll
+    for (int j = 0; j < NC; j++) u.m[NC - 1][j] = u.m[NC - 1][j] * ph;
   }
 }
 
-__constant__ double2 c_itf[15 * 16];  // fundamental generators iT_a, NC x NC
+// fundamental generators iT_a, NC x NC: the context's table
+// tab_itf[a * 16 + i * NC + k], passed to the kernels
 
 // Wilson gauge force: mom -= algebra_project(U staple, dt beta / NC)
 template <int NC>
 __global__ void __launch_bounds__(128) k_gauge_force(const double* __restrict__ fund, double* __restrict__ mom,
-                                                     double coeff, Geom g) {
+                                                     double coeff, Geom g, const double2* __restrict__ itf) {
   int t = blockIdx.x * 128 + threadIdx.x;
   if (t >= 4 * g.V) return;
   int id = t >> 2, mu = t & 3;
@@ -215,7 +282,7 @@ __global__ void __launch_bounds__(128) k_gauge_force(const double* __restrict__ 
     for (int i = 0; i < NC; i++)
 #pragma unroll
       for (int k = 0; k < NC; k++) {
-        double2 tg = c_itf[a * 16 + i * NC + k];
+        double2 tg = itf[a * 16 + i * NC + k];
         re += tg.x * F.m[k][i].re - tg.y * F.m[k][i].im;
       }
     mom[(size_t)(mu * NG + a) * g.V + id] -= -(coeff * 0.5) * re;
@@ -319,8 +386,7 @@ int md_init_constants(Ctx* c) {
         }
       }
   }
-  YH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_itf), h.data(), sizeof(double2) * h.size()));
-  return YH_OK;
+  return upload_table(c, c->tab_itf, h.data(), h.size());
 }
 
 #define YH_NC_SWITCH(...)                                     \
@@ -336,7 +402,7 @@ int md_gauge_force(Ctx* c, double dt, double beta) {
   if ((rc = force_alloc(c))) return rc;
   double coeff = dt * beta / c->nc;
   dim3 grid((4 * c->g.V + 127) / 128);
-  YH_NC_SWITCH(hipLaunchKernelGGL((k_gauge_force<NC_>), grid, dim3(128), 0, c->stream, c->fund, c->mom, coeff, c->g));
+  YH_NC_SWITCH(hipLaunchKernelGGL((k_gauge_force<NC_>), grid, dim3(128), 0, c->stream, c->fund, c->mom, coeff, c->g, c->tab_itf));
   YH_HIP(hipGetLastError());
   return YH_OK;
 }

@@ -11,6 +11,7 @@
 //   or the wall formula sum_{a,A,x} g1[a] g2[A] S_A(g1col[a], x) conj(S_{g2col[A]}(a, x)),
 //   real part; then summed per time slice in a fixed order (FP64).
 #include <vector>
+#include <mutex>
 #include "yh_internal.hpp"
 
 namespace yh {
@@ -137,7 +138,18 @@ int meson_init_tables() {
         h.col[c][w][a] = pairs[c][w]->col[a];
         h.val[c][w][a] = make_double2(pairs[c][w]->re[a], pairs[c][w]->im[a]);
       }
+  // the tables do not depend on the lattice: uploaded once per device (the
+  // current one), so a new context never rewrites memory that another
+  // context's kernels may be reading
+  static std::mutex mtx;
+  static unsigned done_mask = 0;
+  std::lock_guard<std::mutex> lk(mtx);
+  int dev = 0;
+  YH_HIP(hipGetDevice(&dev));
+  if (dev < 32 && (done_mask >> dev) & 1u) return YH_OK;
   YH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_mes), &h, sizeof(h)));
+  YH_HIP(hipDeviceSynchronize());
+  if (dev < 32) done_mask |= 1u << dev;
   return YH_OK;
 }
 

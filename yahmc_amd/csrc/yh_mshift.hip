@@ -278,18 +278,16 @@ int mshift_device(Ctx* c, const yh_inv_par* par, int nrhs, int num, const double
   // The chunk records no memset node.  If capture fails the solver stays eager.
   hipGraphExec_t exec = nullptr;
   bool graphs = c->use_graphs != 0;
-  long mvm_per_chunk = 0;
+  // iterations of each right-hand side already counted in mvm_counter
+  std::vector<int> counted(nrhs, 0);
   while (!rc) {
     if (done_iters > 0 && graphs && !exec) {
       hipGraph_t gr = nullptr;
-      long mvm0 = c->mvm_counter;
       hipError_t ge = hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed);
       int crc = ge == hipSuccess ? enqueue_chunk() : YH_ERR_HIP;
       if (ge == hipSuccess) ge = hipStreamEndCapture(c->stream, &gr);
       if (ge == hipSuccess && crc == YH_OK) ge = hipGraphInstantiate(&exec, gr, nullptr, nullptr, 0);
       if (gr) (void)hipGraphDestroy(gr);
-      mvm_per_chunk = c->mvm_counter - mvm0;
-      c->mvm_counter = mvm0;
       if (ge != hipSuccess || crc != YH_OK || !exec) {
         (void)hipGetLastError();
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -300,7 +298,6 @@ int mshift_device(Ctx* c, const yh_inv_par* par, int nrhs, int num, const double
     if (exec) {
       he = hipGraphLaunch(exec, c->stream);
       if (he != hipSuccess) { set_error(hipGetErrorString(he)); rc = YH_ERR_HIP; break; }
-      c->mvm_counter += mvm_per_chunk;
     } else if ((rc = enqueue_chunk())) {
       break;
     }
@@ -312,6 +309,10 @@ int mshift_device(Ctx* c, const yh_inv_par* par, int nrhs, int num, const double
     for (int j = 0; j < nrhs; j++) {
       if (!(hs[j].rr == hs[j].rr)) { set_error("cg_mshift: non-finite residual"); rc = YH_ERR_NONFINITE; }
       all = all && hs[j].done;
+      // the operator ran once per iteration of this right-hand side (the
+      // launches queued after it converged return at once)
+      c->mvm_counter += (long)(hs[j].iters - counted[j]) * hops_per_apply(YH_MVM_HPHI_SQ);
+      counted[j] = hs[j].iters;
     }
     if (rc || all) break;
     if (par->max_iter > 0 && done_iters >= par->max_iter) { set_error("cg_mshift: max_iter reached"); rc = YH_ERR_NOCONV; }

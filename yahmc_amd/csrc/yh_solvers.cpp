@@ -289,21 +289,30 @@ int yh_cg_mixed(yh_ctx* cc, const yh_inv_par* par, yh_field* dptr, const yh_fiel
   h->norm_b = norm_b;
   if ((rc = write_scalars(c))) return rc;
   int total = 0, at_update = 0;
+  // reliable updates in a row that do not lower the true residual: the
+  // sloppy recurrence has stopped helping, and the solve ends with NOCONV
+  // (it would otherwise loop for ever when max_iter is 0)
+  const int max_stalls = 10;
+  int stalls = 0;
+  double best_rr = rr;
   const int chunk = 16;
   while (true) {
     int n = std::min(chunk, std::max(1, K - (total - at_update)));
+    if (par->max_iter > 0) n = std::min(n, std::max(1, par->max_iter - total));
     for (int i = 0; i < n; i++) {
       if ((rc = apply_op(c, S, recons, YH_MVM_HPHI_SQ, par->mass, q, p, nullptr, 0, &c->sc->converged))) return rc;
       if ((rc = cg_update_xr(c, xs, rs, p, q, 0, PREC_F32))) return rc;
       if ((rc = cg_update_p(c, p, rs, 0, PREC_F32))) return rc;
     }
     if ((rc = read_scalars(c))) return rc;
+    c->mvm_counter += (long)(h->iters - total) * hops_per_apply(YH_MVM_HPHI_SQ);
     total = h->iters;
     if (par->log)
       fprintf(stderr, "[INVERTER][DEBUG] cg_mixed: iters=%d rr=%.6e pap=%.6e alpha=%.6e beta=%.6e stop=%d reason=%d\n",
               h->iters, h->rr, h->red[0], h->alpha, h->beta, h->converged, h->reason);
     if (!(h->rr == h->rr)) { set_error("cg_mixed: non-finite residual"); return YH_ERR_NONFINITE; }
-    if (!h->converged && total - at_update < K) continue;
+    const bool capped = par->max_iter > 0 && total >= par->max_iter;
+    if (!h->converged && total - at_update < K && !capped) continue;
     if (h->converged) {
       // the iteration that raised the stop flag skipped its p = r + beta p
       // step; complete it with the iterated residual before r is replaced
@@ -317,10 +326,16 @@ int yh_cg_mixed(yh_ctx* cc, const yh_inv_par* par, yh_field* dptr, const yh_fiel
     nupd++;
     if (!(rr == rr)) { set_error("cg_mixed: non-finite true residual"); return YH_ERR_NONFINITE; }
     if (rr < target) break;
-    if (par->max_iter > 0 && total >= par->max_iter) {
+    if (rr < best_rr) {
+      best_rr = rr;
+      stalls = 0;
+    } else {
+      stalls++;
+    }
+    if ((par->max_iter > 0 && total >= par->max_iter) || stalls >= max_stalls) {
       *iters = total;
       if (updates) *updates = nupd;
-      set_error("cg_mixed: max_iter reached");
+      set_error(stalls >= max_stalls ? "cg_mixed: true residual stalled" : "cg_mixed: max_iter reached");
       return YH_ERR_NOCONV;
     }
     if ((rc = field_convert(c, rs, 0, r64, 0)) || (rc = field_zero(c, xs, YH_EVEN))) return rc;
