@@ -139,6 +139,7 @@ __global__ void __launch_bounds__(256) k_ms_k(MsState* st, const MsFields* __res
   __shared__ int last;
   __syncthreads();
   if (threadIdx.x == 0) {
+    stores_complete();
     unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     last = (t == gridDim.x - 1);
   }

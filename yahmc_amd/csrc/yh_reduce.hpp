@@ -11,10 +11,23 @@
 // the L2), the wave waits for the store to complete (s_waitcnt), and only
 // then takes its ticket with a relaxed agent-scope atomic; the last
 // workgroup reads the partials with agent-scope atomic loads.
+//
+// The ordering of the partial stores before the ticket rests on the
+// hardware: s_waitcnt vmcnt(0) returns only once the write-through stores
+// are acknowledged.  The language does not order relaxed atomics to
+// different addresses, so compiler-only fences on both sides of the wait
+// keep the compiler from moving a store past it or the ticket before it.
 #pragma once
 #include <hip/hip_runtime.h>
 
 namespace yh {
+
+// every store issued so far has completed before anything after this
+__device__ __forceinline__ void stores_complete() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -44,7 +57,7 @@ __device__ bool block_reduce_finish(double (&v)[NR], double* partials, int nblk,
       for (int q = 0; q < NT / 64; q++) s += sm[r][q];
       __hip_atomic_store(&partials[(size_t)r * nblk + blk], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    __builtin_amdgcn_s_waitcnt(0);  // the partial stores have completed
+    stores_complete();
     unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     is_last = (t == (unsigned)(nblk - 1));
   }
