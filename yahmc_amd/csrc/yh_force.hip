@@ -522,71 +522,216 @@ __device__ __forceinline__ void madd(cd (&A)[D][D], const cd (&B)[D][D], double 
     for (int j = 0; j < D; j++) A[i][j] = A[i][j] + s * B[i][j];
 }
 
-// clover-leaf derivative, one thread per (site, mu) looping over nu != mu
-// (forces.cpp:46-102).  With the links W0 = U(x-nu, mu)^+, W1 = U(x-nu, nu),
-// W2 = U(x+mu, nu), W3 = U(x+nu, mu)^+, W4 = U(x, nu)^+, W5 = U(x+mu-nu, nu)^+,
-// W6 = W0 W1, W7 = W2 W3, W8 = W7 W4 - W5 W6 and Z0..Z5 the clover-force
-// matrices of the plane at x, x+mu, x-nu, x+mu-nu, x+mu+nu, x+nu:
-//   F = U(x, mu) [W8 Z0 + Z1 W8 - W5 (W0 Z2 W1 + Z3 W6) + (W2 Z4 W3 + W7 Z5) W4].
-// The terms are ordered so that at most four matrices are live; every other
-// operand is read from memory (L2) where it is used.
+// clover-leaf derivative (forces.cpp:33-106).  With the links
+// W0 = U(x-nu, mu)^+, W1 = U(x-nu, nu), W2 = U(x+mu, nu), W3 = U(x+nu, mu)^+,
+// W4 = U(x, nu)^+, W5 = U(x+mu-nu, nu)^+, W6 = W0 W1, W7 = W2 W3,
+// W8 = W7 W4 - W5 W6 and Z0..Z5 the clover-force matrices of the plane at
+// x, x+mu, x-nu, x+mu-nu, x+mu+nu, x+nu:
+//   F = U(x, mu) [W8 Z0 + Z1 W8 - W5 (W0 Z2 W1 + Z3 W6) + (W2 Z4 W3 + W7 Z5) W4]
+// and P(x, mu) += proj(F, sign c_sw dt / 8).
+//
+// Work split: one launch per nu (the three nu != mu in ascending order, so
+// every momentum receives its three terms in the reference's order and no
+// two lanes write the same momentum); within a launch a group of D lanes
+// handles one (x, mu), lane i computing row i of F.  Row i is a chain of
+// row-vector by matrix products starting from u = row i of U(x, mu):
+//   a = u W2, b = a W3, c = b W4, d = u W5, e = d W0, f = e W1, h = u Z1
+//   F_i = (a Z4 W3 + b Z5) W4 + (c - f) Z0 - (e Z2 + d Z3 W0) W1
+//         + h W2 W3 W4 - h W5 W0 W1
+// (22 products, no D x D intermediate).  For each product the group stages
+// the right-hand matrix in LDS (lane i fetches row i, or column i for a
+// dagger, one step ahead), and every lane of the group reads the whole
+// matrix from there.  The projection is summed over the group's lanes in
+// fixed order.  A lane holds about seven row vectors (D complex each).
+template <int D>
+struct Op {
+  const double2* p;  // element (r, c) of the stored matrix at p[(r * D + c) * stride]
+  double sgn;
+  bool dag;          // the operand is the conjugate transpose of the stored matrix
+};
+
+template <int D>
+__device__ __forceinline__ Op<D> link_op(const double2* links, int id, int mu, bool dag, const Geom& g) {
+  const MSrc m = link_src<D>(links, id, mu, dag, g);
+  return Op<D>{m.p, m.sgn, dag};
+}
+
+template <int D>
+__device__ __forceinline__ Op<D> clf_op(const double2* clf, int num, int id, int V) {
+  return Op<D>{clf + (size_t)num * D * D * V + id, 1.0, false};
+}
+
+// row i of the operand (sign and conjugation applied); `stride` is VHpad for
+// links and V for clover-force matrices.  The index passes through an empty
+// volatile asm first: the loads are of read-only kernel data, and without it
+// the compiler issues the loads of every step at the start of the kernel
+template <int D>
+__device__ __forceinline__ void fetch_row(double2 (&r)[D], const Op<D>& o, int i, size_t stride) {
+  int z = 0;
+  __asm__ volatile("" : "+v"(z));
+  const double2* p = o.p + z;
+#pragma unroll
+  for (int j = 0; j < D; j++) {
+    const double2 v = p[(size_t)(o.dag ? j * D + i : i * D + j) * stride];
+    r[j] = o.dag ? make_double2(o.sgn * v.x, -o.sgn * v.y) : make_double2(o.sgn * v.x, o.sgn * v.y);
+  }
+}
+
+template <int D>
+__device__ __forceinline__ void put_row(double2* buf, const double2 (&r)[D], int i, bool on) {
+  if (on) {
+#pragma unroll
+    for (int j = 0; j < D; j++) buf[i * D + j] = r[j];
+  }
+}
+
+// an opaque zero that the compiler can only compute after every component
+// of v is final
+template <int D>
+__device__ __forceinline__ int after_row(cd (&v)[D]) {
+#pragma unroll
+  for (int k = 0; k < D; k++) {
+    __asm__ volatile("" : "+v"(v[k].re));
+    __asm__ volatile("" : "+v"(v[k].im));
+  }
+  int z;
+  __asm__ volatile("v_mov_b32 %0, 0" : "=v"(z));
+  return z;
+}
+
+// out (+)= s * (r M), M row-major in LDS.  M is read two rows at a time, and
+// each pair's reads take an index that depends (through an empty volatile
+// asm) on the previous pair's products: the workgroup is one wave, so the
+// compiler drops its barriers and would otherwise issue the LDS reads of
+// every step at once and spill them
+template <int D, bool ACC>
+__device__ __forceinline__ void vm(cd (&out)[D], cd (&r)[D], const double2* M, double s) {
+  cd t[D];
+#pragma unroll
+  for (int j = 0; j < D; j++) t[j] = cd(0, 0);
+  int z = after_row<D>(r);
+#pragma unroll
+  for (int k = 0; k < D; k++) {
+    if (k > 0 && (k & 1) == 0) z = after_row<D>(t);
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+      const double2 m = M[z + k * D + j];
+      cmac(t[j], r[k], cd(m.x, m.y));
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < D; j++) out[j] = ACC ? out[j] + s * t[j] : s * t[j];
+}
+
 template <int NG, int D>
-__global__ void __launch_bounds__(128) k_force_clover_core(const double2* __restrict__ clf,
-                                                           const double2* __restrict__ links, double* __restrict__ mom,
-                                                           double coeff, Geom g, const double2* __restrict__ itr) {
-  int t = blockIdx.x * 128 + threadIdx.x;
-  if (t >= 4 * g.V) return;
-  int id = t >> 2, mu = t & 3;
+__global__ void __launch_bounds__(64) k_force_clover_rows(const double2* __restrict__ clf,
+                                                          const double2* __restrict__ links, double* __restrict__ mom,
+                                                          double coeff, Geom g, const double2* __restrict__ itr,
+                                                          int nu_index) {
+  constexpr int G = 64 / D;  // groups per wave
+  __shared__ double2 sbuf[G][2][D * D];
+  __shared__ double spart[64][NG];
+  const int lane = threadIdx.x;
+  const int gi = min(lane / D, G - 1);
+  const int i = lane % D;
+  const int group = blockIdx.x * G + gi;
+  const bool on = lane < G * D && group < 4 * g.V;
+  // tail lanes compute on a valid site and write nothing
+  const int gq = group < 4 * g.V ? group : 4 * g.V - 1;
+  const int mu = gq / g.V, id = gq - mu * g.V;
+  const int nu = nu_index + (nu_index >= mu ? 1 : 0);
   int c[4];
   fcoords(id, g, c);
-  int o1 = fshift(c, mu, 1, g);
-  for (int nu = 0; nu < 4; nu++) {
-    if (nu == mu) continue;
-    int o2 = fshift(c, nu, 1, g);
-    int o3 = fshift(c, nu, -1, g);
-    int c3[4];
-    fcoords(o3, g, c3);
-    int o4 = fshift(c3, mu, 1, g);
-    int c2[4];
-    fcoords(o2, g, c2);
-    int o5 = fshift(c2, mu, 1, g);
-    int num, sign;
-    if (mu < nu) { num = nu * (nu - 1) / 2 + mu; sign = 1; }
-    else { num = mu * (mu - 1) / 2 + nu; sign = -1; }
-    const MSrc w0 = link_src<D>(links, o3, mu, true, g), w1 = link_src<D>(links, o3, nu, false, g),
-               w2 = link_src<D>(links, o1, nu, false, g), w3 = link_src<D>(links, o2, mu, true, g),
-               w4 = link_src<D>(links, id, nu, true, g), w5 = link_src<D>(links, o4, nu, true, g);
-    const MSrc z0 = clf_src<D>(clf, num, id, g.V), z1 = clf_src<D>(clf, num, o1, g.V),
-               z2 = clf_src<D>(clf, num, o3, g.V), z3 = clf_src<D>(clf, num, o4, g.V),
-               z4 = clf_src<D>(clf, num, o5, g.V), z5 = clf_src<D>(clf, num, o2, g.V);
-    cd A[D][D], B[D][D], Fm[D][D], R[D][D];
-    // W8 = W2 W3 W4 - W5 W0 W1  (in A)
-    mload<D>(B, w3);
-    mul_mb<D, false>(R, after<D>(w2, B), B, 1.0);   // R = W7
-    mul_am<D, false>(A, R, after<D>(w4, R), 1.0);   // A = W7 W4
-    mload<D>(B, after<D>(w1, A));
-    mul_mb<D, false>(R, after<D>(w0, B), B, 1.0);   // R = W6
-    mul_mb<D, true>(A, after<D>(w5, R), R, -1.0);   // A = W8
-    // Fm = W8 Z0 + Z1 W8
-    mul_am<D, false>(Fm, A, after<D>(z0, A), 1.0);
-    mul_mb<D, true>(Fm, after<D>(z1, Fm), A, 1.0);
-    // Fm -= W5 (Z3 W6 + W0 Z2 W1)
-    mul_mb<D, false>(A, after<D>(z3, Fm), R, 1.0);  // A = Z3 W6
-    mload<D>(B, after<D>(z2, A));
-    mul_am<D, false>(R, B, after<D>(w1, B), 1.0);   // R = Z2 W1
-    mul_mb<D, true>(A, after<D>(w0, R), R, 1.0);    // A += W0 Z2 W1
-    mul_mb<D, true>(Fm, after<D>(w5, A), A, -1.0);
-    // Fm += (W7 Z5 + W2 Z4 W3) W4
-    mload<D>(B, after<D>(w3, Fm));
-    mul_mb<D, false>(R, after<D>(w2, B), B, 1.0);   // R = W7
-    mul_am<D, false>(A, R, after<D>(z5, R), 1.0);   // A = W7 Z5
-    mload<D>(B, after<D>(z4, A));
-    mul_am<D, false>(R, B, after<D>(w3, B), 1.0);   // R = Z4 W3
-    mul_mb<D, true>(A, after<D>(w2, R), R, 1.0);    // A += W2 Z4 W3
-    mul_am<D, true>(Fm, A, after<D>(w4, A), 1.0);   // Fm += A W4
-    // U(x, mu) Fm, projected onto the algebra
-    mul_mb<D, false>(A, after<D>(link_src<D>(links, id, mu, false, g), Fm), Fm, 1.0);
-    project_add<NG, D>(mom, mu, id, g.V, A, sign * coeff, itr);
+  const int o1 = fshift(c, mu, 1, g);  // x + mu
+  const int o2 = fshift(c, nu, 1, g);  // x + nu
+  const int o3 = fshift(c, nu, -1, g); // x - nu
+  int c3[4], c2[4];
+  fcoords(o3, g, c3);
+  fcoords(o2, g, c2);
+  const int o4 = fshift(c3, mu, 1, g); // x + mu - nu
+  const int o5 = fshift(c2, mu, 1, g); // x + mu + nu
+  int num, sign;
+  if (mu < nu) { num = nu * (nu - 1) / 2 + mu; sign = 1; }
+  else { num = mu * (mu - 1) / 2 + nu; sign = -1; }
+  const Op<D> W0 = link_op<D>(links, o3, mu, true, g), W1 = link_op<D>(links, o3, nu, false, g),
+              W2 = link_op<D>(links, o1, nu, false, g), W3 = link_op<D>(links, o2, mu, true, g),
+              W4 = link_op<D>(links, id, nu, true, g), W5 = link_op<D>(links, o4, nu, true, g);
+  const Op<D> Z0 = clf_op<D>(clf, num, id, g.V), Z1 = clf_op<D>(clf, num, o1, g.V),
+              Z2 = clf_op<D>(clf, num, o3, g.V), Z3 = clf_op<D>(clf, num, o4, g.V),
+              Z4 = clf_op<D>(clf, num, o5, g.V), Z5 = clf_op<D>(clf, num, o2, g.V);
+  const size_t SL = g.VHpad, SZ = g.V;  // strides of links and of clover-force matrices
+  // u = row i of U(x, mu), loaded again where it is used rather than kept live
+  const Op<D> U = link_op<D>(links, id, mu, false, g);
+  cd u[D];
+  auto load_u = [&]() {
+    double2 r[D];
+    fetch_row<D>(r, U, i, SL);
+#pragma unroll
+    for (int j = 0; j < D; j++) u[j] = cd(r[j].x, r[j].y);
+  };
+  double2* B0 = sbuf[gi][0];
+  double2* B1 = sbuf[gi][1];
+  double2 pre[D];
+  cd a[D], b[D], cc[D], d[D], e[D], s[D], t[D], acc[D];
+  // each step: stage the prefetched operand, fetch the next one, multiply
+#define YH_STEP(BUF, NEXT, STRIDE)  \
+  put_row<D>(BUF, pre, i, on);      \
+  __syncthreads();                  \
+  fetch_row<D>(pre, NEXT, i, STRIDE);
+  load_u();
+  fetch_row<D>(pre, W2, i, SL);
+  YH_STEP(B0, W3, SL) vm<D, false>(a, u, B0, 1.0);   // a = u W2
+  YH_STEP(B1, Z4, SZ) vm<D, false>(b, a, B1, 1.0);   // b = a W3
+  YH_STEP(B0, W3, SL) vm<D, false>(s, a, B0, 1.0);   // s = a Z4
+  YH_STEP(B1, Z5, SZ) vm<D, false>(t, s, B1, 1.0);   // t = a Z4 W3
+  YH_STEP(B0, W4, SL) vm<D, true>(t, b, B0, 1.0);    // t += b Z5
+  YH_STEP(B1, W5, SL)
+  vm<D, false>(cc, b, B1, 1.0);                      // c = b W4
+  vm<D, false>(acc, t, B1, 1.0);                     // acc = (a Z4 W3 + b Z5) W4
+  load_u();
+  YH_STEP(B0, W0, SL) vm<D, false>(d, u, B0, 1.0);   // d = u W5
+  YH_STEP(B1, W1, SL) vm<D, false>(e, d, B1, 1.0);   // e = d W0
+  YH_STEP(B0, Z0, SZ)
+  vm<D, false>(s, e, B0, 1.0);                       // s = e W1 (= u W5 W6)
+#pragma unroll
+  for (int j = 0; j < D; j++) cc[j] = cc[j] - s[j];  // c - f = u W8
+  YH_STEP(B1, Z2, SZ) vm<D, true>(acc, cc, B1, 1.0); // acc += u W8 Z0
+  YH_STEP(B0, Z3, SZ) vm<D, false>(s, e, B0, 1.0);   // s = e Z2
+  YH_STEP(B1, W0, SL) vm<D, false>(t, d, B1, 1.0);   // t = d Z3
+  YH_STEP(B0, W1, SL) vm<D, true>(s, t, B0, 1.0);    // s += d Z3 W0
+  YH_STEP(B1, Z1, SZ) vm<D, true>(acc, s, B1, -1.0); // acc -= (e Z2 + d Z3 W0) W1
+  load_u();
+  YH_STEP(B0, W2, SL) vm<D, false>(d, u, B0, 1.0);   // h = u Z1 (in d)
+  YH_STEP(B1, W3, SL) vm<D, false>(s, d, B1, 1.0);   // s = h W2
+  YH_STEP(B0, W4, SL) vm<D, false>(t, s, B0, 1.0);   // t = h W2 W3
+  YH_STEP(B1, W5, SL) vm<D, true>(acc, t, B1, 1.0);  // acc += h W2 W3 W4
+  YH_STEP(B0, W0, SL) vm<D, false>(s, d, B0, 1.0);   // s = h W5
+  YH_STEP(B1, W1, SL) vm<D, false>(t, s, B1, 1.0);   // t = h W5 W0
+  put_row<D>(B0, pre, i, on);
+  __syncthreads();
+  vm<D, true>(acc, t, B0, -1.0);                     // acc -= h W5 W0 W1
+#undef YH_STEP
+  // row i of U(x, mu) F, projected: sum_k Re (iT_a)_{k i} M_{i k}
+#pragma unroll
+  for (int q = 0; q < NG; q++) {
+    double re = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+      const double2 tg = itr[(q * D + k) * D + i];
+      re += tg.x * acc[k].re - tg.y * acc[k].im;
+    }
+    spart[lane][q] = re;
+  }
+  __syncthreads();
+  if (on && i == 0) {
+    const double f = -(sign * coeff * 0.5);
+#pragma unroll
+    for (int q = 0; q < NG; q++) {
+      double sum = 0.0;
+#pragma unroll
+      for (int r = 0; r < D; r++) sum += spart[lane + r][q];
+      mom[(size_t)(mu * NG + q) * g.V + id] += f * sum;
+    }
   }
 }
 
@@ -695,10 +840,15 @@ int force_clover_core(Ctx* c, double dt) {
   if ((rc = force_alloc(c))) return rc;
   double coeff = c->csw * dt / 8.0;
   const double2* links = (const double2*)c->links[PREC_F64][RECON_18];
-  dim3 grid((4 * c->g.V + 127) / 128);
-  YH_FORCE_MODELS(hipLaunchKernelGGL((k_force_clover_core<NG_, D_>), grid, dim3(128), 0, c->stream, (const double2*)c->clf,
-                                     links, c->mom, coeff, c->g, c->tab_itr));
-  YH_HIP(hipGetLastError());
+  // one launch per nu (ascending), so each momentum gets its three terms in
+  // the reference's order
+  for (int j = 0; j < 3; j++) {
+    YH_FORCE_MODELS(const int groups = 64 / D_;
+                    dim3 grid((unsigned)((4L * c->g.V + groups - 1) / groups));
+                    hipLaunchKernelGGL((k_force_clover_rows<NG_, D_>), grid, dim3(64), 0, c->stream,
+                                       (const double2*)c->clf, links, c->mom, coeff, c->g, c->tab_itr, j));
+    YH_HIP(hipGetLastError());
+  }
   return YH_OK;
 }
 
