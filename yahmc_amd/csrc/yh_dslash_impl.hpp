@@ -372,6 +372,189 @@ __global__ void __launch_bounds__(256) k_clover(void* out, float* out_nrm, const
 }
 
 // ---------------------------------------------------------------------------
+// Several right-hand sides in one launch: NR adjacent lanes take the same
+// output site, lane r working on right-hand side r with the single-RHS code.
+// Their link and clover loads are the same addresses within one wave
+// instruction, so each link and clover block is fetched from memory once
+// for the whole batch (at FP64 18-real links 1,152 of the 1,536 bytes per
+// site and right-hand side), while each lane keeps the registers of the
+// single-RHS hop.  Right-hand sides whose skip flag is set do nothing; the
+// norm reduction of the CLOV epilogue is kept per right-hand side.
+enum { MR_MAX = 8 };
+struct HopArgsM {
+  HopArgs a;                  // links, clover, scalars, geometry (field pointers unused)
+  const void* in[MR_MAX];
+  void* out[MR_MAX];
+  const void* self[MR_MAX];
+  double* red_out[MR_MAX];
+  const int* skip[MR_MAX];
+};
+
+template <int D, int PREC, int RECON, int EPI, int NR>
+__global__ void __launch_bounds__(YH_HOP_BLOCK, hop_min_waves<PREC>()) k_hop_mr(HopArgsM m) {
+  static_assert(PREC != PREC_F16, "16-bit fields carry per-site scales");
+  typedef typename SpinorStore<PREC, D>::real R;
+  const int nblk = gridDim.x;
+  const int blk = xcd_block(nblk, m.a.g.xq);
+  const int t = blk * YH_HOP_BLOCK + threadIdx.x;
+  const int k = t / NR, rr = t - (t / NR) * NR;
+  HopArgs a = m.a;
+  a.in = m.in[rr];
+  a.out = m.out[rr];
+  a.self = m.self[rr];
+  const Geom& g = a.g;
+  const bool live = k < g.VH && !(m.skip[rr] && *m.skip[rr]);
+  double nrm2 = 0.0;
+  if (live) {
+    int c[4];
+    site_coords(k, a.parity, g, c);
+    const int tt = c[0], x = c[1], y = c[2], z = c[3];
+    const int T = g.L[0];
+    Cx<R> acc[4][D];
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+#pragma unroll
+      for (int i = 0; i < D; i++) acc[s][i] = Cx<R>((R)0, (R)0);
+    const bool ap = g.bc == BC_ANTIPERIODIC;
+    {
+      int tf = (tt + 1 == T) ? 0 : tt + 1;
+      int tb = (tt == 0) ? T - 1 : tt - 1;
+      int nf = half_index(tf, x, y, z, g), nb = half_index(tb, x, y, z, g);
+      hop_dir<0, true, D, PREC, RECON, R>(acc, a, nf, k, ap && tt == 0);
+      const int h = hop_group_dep<PREC, RECON, 3>(acc);
+      hop_dir<0, false, D, PREC, RECON, R>(acc, a, nb + h, nb + h, ap && tb == 0);
+    }
+    const int dep1 = hop_group_dep<PREC, RECON, 0>(acc);
+    {
+      int xf = (x + 1 == g.L[1]) ? 0 : x + 1;
+      int xb = (x == 0) ? g.L[1] - 1 : x - 1;
+      int nf = half_index(tt, xf, y, z, g) + dep1, nb = half_index(tt, xb, y, z, g) + dep1;
+      hop_dir<1, true, D, PREC, RECON, R>(acc, a, nf, k + dep1, false);
+      const int h = hop_group_dep<PREC, RECON, 3>(acc);
+      hop_dir<1, false, D, PREC, RECON, R>(acc, a, nb + h, nb + h, false);
+    }
+    const int dep2 = hop_group_dep<PREC, RECON, 1>(acc);
+    {
+      int yf = (y + 1 == g.L[2]) ? 0 : y + 1;
+      int yb = (y == 0) ? g.L[2] - 1 : y - 1;
+      int nf = half_index(tt, x, yf, z, g) + dep2, nb = half_index(tt, x, yb, z, g) + dep2;
+      hop_dir<2, true, D, PREC, RECON, R>(acc, a, nf, k + dep2, false);
+      const int h = hop_group_dep<PREC, RECON, 3>(acc);
+      hop_dir<2, false, D, PREC, RECON, R>(acc, a, nb + h, nb + h, false);
+    }
+    const int dep3 = hop_group_dep<PREC, RECON, 2>(acc);
+    {
+      int zf = (z + 1 == g.L[3]) ? 0 : z + 1;
+      int zb = (z == 0) ? g.L[3] - 1 : z - 1;
+      int nf = half_index(tt, x, y, zf, g) + dep3, nb = half_index(tt, x, y, zb, g) + dep3;
+      hop_dir<3, true, D, PREC, RECON, R>(acc, a, nf, k + dep3, false);
+      const int h = hop_group_dep<PREC, RECON, 3>(acc);
+      hop_dir<3, false, D, PREC, RECON, R>(acc, a, nb + h, nb + h, false);
+    }
+    const int ke = k + hop_group_dep<PREC, RECON, 4>(acc);
+    const bool bz = bc_zero_slice(tt, g);
+    const R sc = bz ? (R)0 : (R)a.scale;
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+#pragma unroll
+      for (int i = 0; i < D; i++) acc[s][i] = sc * acc[s][i];
+    constexpr int N = 2 * D;
+    if constexpr (EPI == EPI_CLOVINV) {
+      Cx<R> o[4][D];
+      int kb = ke;
+#pragma unroll
+      for (int blk2 = 0; blk2 < 2; blk2++) {
+        R cl[N * N];
+        load_clover_block<PREC, D, R>(cl, a.clov, blk2, kb, g.VHpad, a.clov_nrm);
+        Cx<R> xin[N], yout[N];
+#pragma unroll
+        for (int i = 0; i < D; i++) { xin[i] = acc[2 * blk2][i]; xin[D + i] = acc[2 * blk2 + 1][i]; }
+        herm_mv<N, R>(yout, cl, xin);
+#pragma unroll
+        for (int i = 0; i < D; i++) { o[2 * blk2][i] = yout[i]; o[2 * blk2 + 1][i] = yout[D + i]; }
+        kb = ke + hop_group_dep<PREC, RECON, 5>(o);
+      }
+      store_spinor<PREC, D, R, PREC == PREC_F64>(a.out, nullptr, k, g.VHpad, o);
+    } else {
+      Cx<R> xs[4][D];
+      load_spinor<PREC, D, R>(xs, a.self, nullptr, ke, g.VHpad);
+      const R dg = (R)(a.diag + bc_cf_shift(tt, g));
+      const bool zs = a.zero_self_bc && bz;
+      int kb = ke;
+#pragma unroll
+      for (int blk2 = 0; blk2 < 2; blk2++) {
+        R cl[N * N];
+        load_clover_block<PREC, D, R>(cl, a.clov, blk2, kb, g.VHpad, a.clov_nrm);
+        Cx<R> xin[N], yout[N];
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+          xin[i] = zs ? Cx<R>((R)0, (R)0) : xs[2 * blk2][i];
+          xin[D + i] = zs ? Cx<R>((R)0, (R)0) : xs[2 * blk2 + 1][i];
+        }
+        herm_mv<N, R>(yout, cl, xin);
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+          acc[2 * blk2][i] = acc[2 * blk2][i] + yout[i] + dg * xin[i];
+          acc[2 * blk2 + 1][i] = acc[2 * blk2 + 1][i] + yout[D + i] + dg * xin[D + i];
+        }
+        kb = ke + hop_group_dep<PREC, RECON, 5>(acc);
+      }
+      if (a.gamma5) {
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+          acc[2][i] = Cx<R>(-acc[2][i].re, -acc[2][i].im);
+          acc[3][i] = Cx<R>(-acc[3][i].re, -acc[3][i].im);
+        }
+      }
+      store_spinor<PREC, D, R, PREC == PREC_F64>(a.out, nullptr, k, g.VHpad, acc);
+      if (a.reduce) {
+#pragma unroll
+        for (int s = 0; s < 4; s++)
+#pragma unroll
+          for (int i = 0; i < D; i++) nrm2 += (double)acc[s][i].re * acc[s][i].re + (double)acc[s][i].im * acc[s][i].im;
+      }
+    }
+  }
+  if constexpr (EPI == EPI_CLOV) {
+    if (a.reduce) {
+      double v[NR];
+#pragma unroll
+      for (int q = 0; q < NR; q++) v[q] = q == rr ? nrm2 : 0.0;
+      double res[NR];
+      if (block_reduce_finish<NR, YH_HOP_BLOCK>(v, a.partials, nblk, blockIdx.x, a.ticket, res) && threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NR; q++)
+          if (m.red_out[q] && !(m.skip[q] && *m.skip[q])) *m.red_out[q] = res[q];
+      }
+    }
+  }
+}
+
+template <int D, int PREC, int NR>
+static int hop_mr_epi(Ctx* c, const HopArgsM& m, int epi) {
+  dim3 grid((unsigned)(((long)c->g.VH * NR + YH_HOP_BLOCK - 1) / YH_HOP_BLOCK)), block(YH_HOP_BLOCK);
+  if (epi == EPI_CLOVINV) hipLaunchKernelGGL((k_hop_mr<D, PREC, RECON_18, EPI_CLOVINV, NR>), grid, block, 0, c->stream, m);
+  else hipLaunchKernelGGL((k_hop_mr<D, PREC, RECON_18, EPI_CLOV, NR>), grid, block, 0, c->stream, m);
+  YH_HIP(hipGetLastError());
+  return YH_OK;
+}
+
+// NR right-hand sides (2 or 4), FP64 or FP32 fields, 18-real links,
+// CLOVINV or CLOV epilogue
+template <int D>
+int hop_mr_dispatch_d(Ctx* c, const HopArgsM& m, int prec, int nr, int epi) {
+  if (prec == PREC_F64) {
+    if (nr == 2) return hop_mr_epi<D, PREC_F64, 2>(c, m, epi);
+    if (nr == 4) return hop_mr_epi<D, PREC_F64, 4>(c, m, epi);
+  } else if (prec == PREC_F32) {
+    if (nr == 2) return hop_mr_epi<D, PREC_F32, 2>(c, m, epi);
+    if (nr == 4) return hop_mr_epi<D, PREC_F32, 4>(c, m, epi);
+  }
+  set_error("multi-RHS hop: unsupported precision / batch");
+  return YH_ERR_UNSUPPORTED;
+}
+
+// ---------------------------------------------------------------------------
 template <int D, int PREC, int RECON>
 static int hop_dispatch_epi(Ctx* c, const HopArgs& a, int epi, hipStream_t st) {
   dim3 grid((c->g.VH + YH_HOP_BLOCK - 1) / YH_HOP_BLOCK), block(YH_HOP_BLOCK);

@@ -199,8 +199,12 @@ int mshift_device(Ctx* c, const yh_inv_par* par, int nrhs, int num, const double
   const int recon = default_recon(c, PREC_F64);
   struct Rhs {
     Field *r, *p, *q;
+    Field *t1 = nullptr, *t2 = nullptr;  // operator temporaries of the multi-RHS path
     std::vector<Field*> k;
   };
+  // H^2 of up to four right-hand sides per hop launch (FP64, 18-real links,
+  // clover term): batches of 4, then 2, the rest one by one
+  const bool batched = c->multi_rhs && nrhs > 1 && c->csw != 0.0 && recon == RECON_18;
   std::vector<Rhs> rh(nrhs);
   std::vector<Field*> owned;
   auto alloc = [&](Field** f) -> int {
@@ -223,6 +227,7 @@ int mshift_device(Ctx* c, const yh_inv_par* par, int nrhs, int num, const double
   for (int j = 0; j < nrhs && !rc; j++) {
     Rhs& R = rh[j];
     if ((rc = alloc(&R.r)) || (rc = alloc(&R.p)) || (rc = alloc(&R.q))) break;
+    if (batched && ((rc = alloc(&R.t1)) || (rc = alloc(&R.t2)))) break;
     R.k.resize(num);
     for (int s = 0; s < num && !rc; s++) rc = alloc(&R.k[s]);
     if (rc) break;
@@ -263,15 +268,34 @@ int mshift_device(Ctx* c, const yh_inv_par* par, int nrhs, int num, const double
   // their scalars from device memory, so the same sequence can be replayed
   auto enqueue_chunk = [&]() -> int {
     int e = YH_OK;
-    for (int i = 0; i < chunk && !e; i++)
-      for (int j = 0; j < nrhs && !e; j++) {
+    for (int i = 0; i < chunk && !e; i++) {
+      int j0 = 0;
+      while (batched && !e && nrhs - j0 >= 2) {
+        // q = H^2 p and pap = |H p|^2 for a batch of right-hand sides
+        const int nb = nrhs - j0 >= 4 ? 4 : 2;
+        Field *q[4], *p[4], *t1[4], *t2[4];
+        double* rd[4];
+        const int* sk[4];
+        for (int b = 0; b < nb; b++) {
+          Rhs& R = rh[j0 + b];
+          q[b] = R.q; p[b] = R.p; t1[b] = R.t1; t2[b] = R.t2;
+          rd[b] = &st[j0 + b].pap;
+          sk[b] = &st[j0 + b].done;
+        }
+        e = apply_hsq_mr(c, PREC_F64, par->mass, nb, q, p, t1, t2, rd, sk);
+        j0 += nb;
+      }
+      for (int j = j0; j < nrhs && !e; j++) {
         Rhs& R = rh[j];
         // q = H^2 p, and pap = |H p|^2 into st[j].pap via the hop reduction
-        if ((e = apply_op(c, PREC_F64, recon, YH_MVM_HPHI_SQ, par->mass, R.q, R.p, nullptr, 0, &st[j].done, &st[j].pap)))
-          break;
+        e = apply_op(c, PREC_F64, recon, YH_MVM_HPHI_SQ, par->mass, R.q, R.p, nullptr, 0, &st[j].done, &st[j].pap);
+      }
+      for (int j = 0; j < nrhs && !e; j++) {
+        Rhs& R = rh[j];
         if ((e = ms_x(c, st + j, mf + j, R.r, R.q))) break;
         e = ms_k(c, st + j, mf + j, R.r, R.p);
       }
+    }
     return e;
   };
   // The first chunk runs eagerly (it allocates the operator's scratch fields
