@@ -65,63 +65,19 @@ int yh_bicgstab(yh_ctx* cc, const yh_inv_par* par, yh_field* dptr, const yh_fiel
   if (!c->gauge_set) { set_error("no gauge field"); return YH_ERR_STATE; }
   if (dptr->prec != PREC_F64 || sptr->prec != PREC_F64) { set_error("solver fields must be FP64"); return YH_ERR_ARG; }
   if (!(par->prec > 0.0)) { set_error("prec must be positive"); return YH_ERR_ARG; }
-  int rc;
-  Field *r, *r0, *v, *p, *t;
-  if ((rc = need(c, PREC_F64, 2, &r)) || (rc = need(c, PREC_F64, 3, &p)) || (rc = need(c, PREC_F64, 4, &v)) ||
-      (rc = need(c, PREC_F64, 5, &r0)) || (rc = need(c, PREC_F64, 6, &t)))
-    return rc;
-  const int mvm = par->mvm;
-  const int recon = default_recon(c, PREC_F64);
-  Field* x = dptr;
-  // r = b - A x ; r0 = r ; p = r
-  if ((rc = apply_op(c, PREC_F64, recon, mvm, par->mass, p, x, nullptr, -1, nullptr))) return rc;
-  if ((rc = blas_copy(c, r, sptr, 0))) return rc;
-  if ((rc = blas_axpby(c, r, -1.0, p, 1.0, 0))) return rc;
-  if ((rc = blas_copy(c, r0, r, 0))) return rc;
-  if ((rc = blas_copy(c, p, r, 0))) return rc;
-  double norm = sqnorm(c, sptr, &rc);
-  if (rc) return rc;
-  zd rho0 = dot(c, r, r0, &rc);
-  if (rc) return rc;
-  double residue = rho0.real();
-  int it = 0;
-  if (residue < norm * par->prec) it = -1;
-  while (++it) {
-    if ((rc = apply_op(c, PREC_F64, recon, mvm, par->mass, v, p, nullptr, -1, nullptr))) return rc;
-    zd alpha = dot(c, v, r0, &rc);
-    if (rc) return rc;
-    alpha = rho0 / alpha;
-    if ((rc = blas_caxpy(c, r, -alpha.real(), -alpha.imag(), v, 0))) return rc;
-    if ((rc = apply_op(c, PREC_F64, recon, mvm, par->mass, t, r, nullptr, -1, nullptr))) return rc;
-    zd omega = dot(c, r, t, &rc);
-    if (rc) return rc;
-    double tt = sqnorm(c, t, &rc);
-    if (rc) return rc;
-    omega /= tt;
-    if ((rc = blas_caxpy(c, x, alpha.real(), alpha.imag(), p, 0))) return rc;
-    if ((rc = blas_caxpy(c, x, omega.real(), omega.imag(), r, 0))) return rc;
-    if ((rc = blas_caxpy(c, r, -omega.real(), -omega.imag(), t, 0))) return rc;
-    zd rho1 = dot(c, r, r0, &rc);
-    if (rc) return rc;
-    zd beta = (rho1 / rho0) * (alpha / omega);
-    if ((rc = blas_cscale(c, p, beta.real(), beta.imag(), 0))) return rc;
-    if ((rc = blas_axpby(c, p, 1.0, r, 1.0, 0))) return rc;
-    zd bo = beta * omega;
-    if ((rc = blas_caxpy(c, p, -bo.real(), -bo.imag(), v, 0))) return rc;
-    residue = sqnorm(c, r, &rc);
-    if (rc) return rc;
-    if (!(residue == residue)) { set_error("bicgstab: non-finite residual"); return YH_ERR_NONFINITE; }
-    if (residue < norm * par->prec) break;
-    if (par->max_iter > 0 && it >= par->max_iter) {
-      *iters = 2 * it;
-      set_error("bicgstab: max_iter reached");
-      return YH_ERR_NOCONV;
-    }
-    rho0 = rho1;
-  }
-  *iters = 2 * it;
-  if (par->log) fprintf(stderr, "[INVERTER][NOTICE] bicgstab: iterations = %d, precision = %1.4e\n", 2 * it, residue / norm);
-  return sync(c);
+  if (par->mvm < YH_MVM_DPHI || par->mvm > YH_MVM_HPHI_SQ) { set_error("bad mvm"); return YH_ERR_ARG; }
+  return bicgstab_device(c, par, PREC_F64, dptr, sptr, iters, nullptr);
+}
+
+int yh_bicgstab_mixed(yh_ctx* cc, const yh_inv_par* par, yh_field* dptr, const yh_field* sptr, int* iters,
+                      int* updates) {
+  Ctx* c = cc;
+  if (!c || !par || !dptr || !sptr || !iters) { set_error("null argument"); return YH_ERR_ARG; }
+  if (!c->gauge_set) { set_error("no gauge field"); return YH_ERR_STATE; }
+  if (dptr->prec != PREC_F64 || sptr->prec != PREC_F64) { set_error("solver fields must be FP64"); return YH_ERR_ARG; }
+  if (!(par->prec > 0.0)) { set_error("prec must be positive"); return YH_ERR_ARG; }
+  if (par->mvm < YH_MVM_DPHI || par->mvm > YH_MVM_HPHI_SQ) { set_error("bad mvm"); return YH_ERR_ARG; }
+  return bicgstab_device(c, par, PREC_F32, dptr, sptr, iters, updates);
 }
 
 int yh_cg_mshift_batched(yh_ctx* cc, const yh_inv_par* par, int nrhs, int num, const double* shift, yh_field** dptr,
@@ -612,8 +568,9 @@ int yh_mre_guess(yh_mre* m, yh_field* dst, const yh_field* b) {
 }
 
 
-// Learned-preconditioner solve with its fallback: flexible CG preconditioned
-// by the network, started from the MRE guess; if it stops without converging
+// Learned-preconditioner solve with its fallback: flexible CG (or, with
+// YH_TUNE_LEARNED_GCR, flexible GCR keeping 8 directions) preconditioned by
+// the network, started from the MRE guess; if it stops without converging
 // (iteration cap or a non-finite scalar), the plain mixed-precision CG
 // restarts from the MRE guess.  The converged solution joins the MRE history.
 int yh_cg_learned(yh_ctx* cc, const yh_inv_par* par, yh_net* net, yh_mre* mre, yh_field* dptr, const yh_field* sptr,
@@ -628,7 +585,8 @@ int yh_cg_learned(yh_ctx* cc, const yh_inv_par* par, yh_net* net, yh_mre* mre, y
   if (net) {
     yh_inv_par p = *par;
     if (p.max_iter <= 0) p.max_iter = LEARNED_MAX_ITER;
-    rc = yh_cg_net(cc, &p, net, dptr, sptr, &it, &upd);
+    rc = c->learned_gcr ? yh_gcr_net(cc, &p, net, 8, dptr, sptr, &it, &upd)
+                        : yh_cg_net(cc, &p, net, dptr, sptr, &it, &upd);
     if (rc != YH_OK && rc != YH_ERR_NOCONV && rc != YH_ERR_NONFINITE) return rc;
   }
   if (rc != YH_OK) {
