@@ -179,6 +179,28 @@ def test_trained_network_reduces_iterations():
         print(f"{name}: iterations {its[name]}, true residual^2 {res:.2e}")
         assert res < 1e-20
     assert its["learned"] < its["plain"]
+    # flexible GCR (8 kept directions) with and without the same network,
+    # and the learned solve with GCR selected
+    gcr = {}
+    for name, n in (("plain", None), ("learned", net)):
+        x = lat.field("f64")
+        x.zero()
+        gcr[name], _ = lat.gcr_net(tol, n, x, b, nkrylov=8)
+        ax = lat.field("f64")
+        lat.Hphi_sq(mass, ax, lat.field("f64").upload(x.download()))
+        r = bh[: V // 2] - ax.download()[: V // 2]
+        res = np.sum(r * r) / np.sum(bh[: V // 2] ** 2)
+        print(f"gcr {name}: iterations {gcr[name]}, true residual^2 {res:.2e}")
+        assert res < 1e-20
+    assert gcr["learned"] < gcr["plain"]
+    lat.tune(yh.TUNE_LEARNED_GCR, 1)
+    x = lat.field("f64")
+    it, _, fell = lat.cg_learned(tol, net, None, x, b)
+    lat.tune(yh.TUNE_LEARNED_GCR, 0)
+    ax = lat.field("f64")
+    lat.Hphi_sq(mass, ax, lat.field("f64").upload(x.download()))
+    r = bh[: V // 2] - ax.download()[: V // 2]
+    assert np.sum(r * r) / np.sum(bh[: V // 2] ** 2) < 1e-20 and not fell and it == gcr["learned"]
 
 
 def test_learned_solve_and_its_fallback():

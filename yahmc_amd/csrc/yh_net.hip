@@ -1248,3 +1248,130 @@ extern "C" int yh_cg_net(yh_ctx* cc, const yh_inv_par* par, yh_net* net, yh_fiel
   YH_HIP(hipStreamSynchronize(c->stream));
   return YH_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Flexible GCR (Eisenstat, Elman & Schultz 1983; flexible form of Saad's
+// FGMRES family: each direction is the preconditioned residual, so M may
+// change from step to step) on H^2 in FP32 with the network preconditioner
+// M = A A^+ (or none), inside the same FP64 reliable updates as the
+// flexible CG.  Per step:
+//   z = M r;  w = H^2 z;  for the kept j: c = (w_j^H w) / |w_j|^2,
+//   w -= c w_j, z -= c z_j;  a = (w^H r) / |w|^2;  x += a z;  r -= a w
+// which minimises |r| over the kept directions (the w_j are orthogonal).
+// At most `nkrylov` directions are kept; the oldest is dropped first
+// (truncated GCR), and a reliable update restarts the history.
+// ---------------------------------------------------------------------------
+extern "C" int yh_gcr_net(yh_ctx* cc, const yh_inv_par* par, yh_net* net, int nkrylov, yh_field* dptr,
+                          const yh_field* sptr, int* iters, int* updates) {
+  Ctx* c = cc;
+  if (!c || !par || !dptr || !sptr || !iters) { set_error("null argument"); return YH_ERR_ARG; }
+  if (!c->gauge_set) { set_error("no gauge field"); return YH_ERR_STATE; }
+  if (net && net->c != c) { set_error("network belongs to another context"); return YH_ERR_ARG; }
+  if (dptr->prec != PREC_F64 || sptr->prec != PREC_F64) { set_error("solver fields must be FP64"); return YH_ERR_ARG; }
+  if (par->mvm != YH_MVM_HPHI_SQ) { set_error("the preconditioned GCR needs Hphi_sq"); return YH_ERR_ARG; }
+  if (!(par->prec > 0.0)) { set_error("prec must be positive"); return YH_ERR_ARG; }
+  if (nkrylov < 1 || nkrylov > 32) { set_error("nkrylov must be 1..32"); return YH_ERR_ARG; }
+  YH_HIP(hipSetDevice(c->device));
+  int rc;
+  Field *r64 = tmp_field(c, PREC_F64, 2, &rc), *t64 = nullptr;
+  if (rc) return rc;
+  t64 = tmp_field(c, PREC_F64, 3, &rc);
+  if (rc) return rc;
+  Field *xs = tmp_field(c, PREC_F32, 2, &rc), *r = nullptr;
+  if (rc) return rc;
+  r = tmp_field(c, PREC_F32, 3, &rc);
+  if (rc) return rc;
+  // the kept directions z_j and w_j = H^2 z_j, plus one slot for the new pair
+  std::vector<Field*> Z(nkrylov + 1, nullptr), W(nkrylov + 1, nullptr);
+  std::vector<double> wn(nkrylov + 1, 0.0);
+  auto release = [&]() {
+    for (auto f : Z) field_free(f);
+    for (auto f : W) field_free(f);
+  };
+  for (int j = 0; j <= nkrylov; j++)
+    if ((rc = field_alloc(c, PREC_F32, &Z[j])) || (rc = field_alloc(c, PREC_F32, &W[j]))) {
+      release();
+      return rc;
+    }
+  const int recon64 = default_recon(c, PREC_F64), recon32 = default_recon(c, PREC_F32);
+  const int K = par->reliable_every > 0 ? par->reliable_every : 50;
+  const double delta = (par->delta > 0.0 && par->delta < 1.0) ? par->delta : 0.1;
+  Field* x = dptr;
+  int k = 0, nupd = 0;
+  auto run = [&]() -> int {
+    int e;
+    const double norm_b = rd_sq(c, sptr, &e);
+    if (e) return e;
+    const double target = par->prec * norm_b;
+    auto true_residual = [&](double* rr) -> int {
+      int q;
+      if ((q = apply_op(c, PREC_F64, recon64, YH_MVM_HPHI_SQ, par->mass, t64, x, nullptr, -1, nullptr))) return q;
+      if ((q = blas_copy(c, r64, sptr, 0))) return q;
+      if ((q = blas_axpby(c, r64, -1.0, t64, 1.0, 0))) return q;
+      *rr = rd_sq(c, r64, &q);
+      return q;
+    };
+    double rr;
+    if ((e = true_residual(&rr))) return e;
+    if (rr < target) return YH_OK;
+    if ((e = field_convert(c, r, 0, r64, 0)) || (e = field_zero(c, xs, YH_EVEN))) return e;
+    double maxrr = rr;
+    int kept = 0, head = 0, at_update = 0;  // ring of kept directions, `head` = next free slot
+    while (true) {
+      Field* z = Z[head];
+      Field* w = W[head];
+      if (net) {
+        if ((e = net_precondition(net, z, r))) return e;
+      } else if ((e = blas_copy(c, z, r, 0))) {
+        return e;
+      }
+      if ((e = apply_op(c, PREC_F32, recon32, YH_MVM_HPHI_SQ, par->mass, w, z, nullptr, -1, nullptr))) return e;
+      for (int i = 0; i < kept; i++) {
+        const int j = (head - 1 - i + 2 * (nkrylov + 1)) % (nkrylov + 1);
+        double d[2];
+        if ((e = reduce_read(c, 1, w, W[j], 0, d))) return e;  // sum w conj(w_j) = w_j^H w
+        const double cr = d[0] / wn[j], ci = d[1] / wn[j];
+        if ((e = blas_caxpy(c, w, -cr, -ci, W[j], 0)) || (e = blas_caxpy(c, z, -cr, -ci, Z[j], 0))) return e;
+      }
+      double ww = rd_sq(c, w, &e);
+      if (e) return e;
+      double d[2];
+      if ((e = reduce_read(c, 1, r, w, 0, d))) return e;  // w^H r
+      const double ar = d[0] / ww, ai = d[1] / ww;
+      if (!(ar == ar) || !(ai == ai)) { set_error("gcr_net: non-finite coefficient"); return YH_ERR_NONFINITE; }
+      if ((e = blas_caxpy(c, xs, ar, ai, z, 0)) || (e = blas_caxpy(c, r, -ar, -ai, w, 0))) return e;
+      wn[head] = ww;
+      head = (head + 1) % (nkrylov + 1);
+      kept = kept < nkrylov ? kept + 1 : nkrylov;
+      k++;
+      const double rrs = rd_sq(c, r, &e);
+      if (e) return e;
+      if (!(rrs == rrs)) { set_error("gcr_net: non-finite residual"); return YH_ERR_NONFINITE; }
+      if (par->log) fprintf(stderr, "[INVERTER][DEBUG] gcr_net: iter=%d rr=%.6e\n", k, rrs / norm_b);
+      if (rrs < delta * delta * maxrr || k - at_update >= K || rrs < target) {
+        // reliable update: fold the FP32 correction into x, recompute r in
+        // FP64 and restart the direction history
+        if ((e = field_convert(c, t64, 0, xs, 0)) || (e = blas_axpby(c, x, 1.0, t64, 1.0, 0))) return e;
+        if ((e = true_residual(&rr))) return e;
+        nupd++;
+        if (!(rr == rr)) { set_error("gcr_net: non-finite true residual"); return YH_ERR_NONFINITE; }
+        if (rr < target) return YH_OK;
+        if ((e = field_convert(c, r, 0, r64, 0)) || (e = field_zero(c, xs, YH_EVEN))) return e;
+        maxrr = rr;
+        at_update = k;
+        kept = 0;
+      }
+      if (par->max_iter > 0 && k >= par->max_iter) {
+        set_error("gcr_net: max_iter reached");
+        return YH_ERR_NOCONV;
+      }
+    }
+  };
+  rc = run();
+  (void)hipStreamSynchronize(c->stream);
+  release();
+  *iters = k;
+  if (updates) *updates = nupd;
+  if (par->log && !rc) fprintf(stderr, "[INVERTER][NOTICE] gcr_net: iterations = %d, updates = %d\n", k, nupd);
+  return rc;
+}
