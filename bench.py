@@ -280,7 +280,10 @@ def rhmc_multishift(yh):
     m = -1.0; 12-pole Zolotarev x^(-1/2) on [1e-4, 8]; 4 Gaussian
     pseudofermion sources through one batched multishift solve to
     per-shift tolerance 1e-10 (prec 1e-20), FP64 partial-fraction sum.
-    Timed with the chunk replayed from a HIP graph and launched eagerly."""
+    Timed with the chunk replayed from a HIP graph and launched eagerly, and
+    with the operator applied to the four right-hand sides per hop launch
+    (each link and clover block loaded once for all of them) or one
+    right-hand side per launch."""
     from yahmc_amd.synthetic import gaussian_source, random_gauge
     dims = (48, 24, 24, 24)
     lat = yh.Lattice(dims, nc=2, repr="adjoint", bc="antiperiodic", csw=1.0)
@@ -294,18 +297,32 @@ def rhmc_multishift(yh):
     out = {"lattice": list(dims), "repr": "SU(2) adjoint", "mass": -1.0, "csw": 1.0,
            "rational": {"order": rat.order, "range": [1e-4, 8.0], "max_rel_error": rat.prec},
            "rhs": len(srcs), "prec": 1e-20, "xcd_chunks": {f"{k[0]}_r{k[1]}": v for k, v in tuned.items()}}
-    for graphs in (1, 0):
+    for name, graphs, multi in (("graph", 1, 1), ("eager", 0, 1), ("graph_one_rhs_per_hop", 1, 0)):
         lat.tune(yh.TUNE_GRAPHS, graphs)
+        lat.tune(yh.TUNE_MULTI_RHS, multi)
         dst = [[lat.field("f64") for _ in shifts] for _ in srcs]
         lat.synchronize()
         t0 = time.perf_counter()
         its = lat.cg_mshift_batched(lat.inv_par(-1.0, 1e-20, "Hphi_sq"), shifts, dst, srcs)
         lat.synchronize()
         dt = time.perf_counter() - t0
-        out["graph" if graphs else "eager"] = {"seconds": round(dt, 3), "iterations": its}
+        out[name] = {"seconds": round(dt, 3), "iterations": its}
+        if name == "graph":
+            # per-shift FP64 true residual |b - (H^2 + s) x| / |b| of every solution
+            worst = 0.0
+            for j, row in enumerate(dst):
+                for s, f in zip(shifts, row):
+                    ax = lat.field("f64")
+                    lat.Hphi_sq(-1.0, ax, f)
+                    lat.spinor_mulr_add_assign(ax, s, f)
+                    lat.spinor_mulr_add_assign(ax, -1.0, srcs[j])
+                    worst = max(worst, float(np.sqrt(lat.spinor_sqnorm(ax) / lat.spinor_sqnorm(srcs[j]))))
+                    ax.free()
+            out[name]["max_true_rel_residual"] = worst
         for row in dst:
             for f in row:
                 f.free()
+    lat.tune(yh.TUNE_MULTI_RHS, 1)
     # the partial-fraction application r(H^2) b for one source, FP64 sum
     fd = lat.field("f64")
     t0 = time.perf_counter()
@@ -421,8 +438,11 @@ def main():
         prec = 1e-20
         for name, fn in (("fp64", lambda fx: (lat.cg_inverter(lat.inv_par(MASS, prec, "Hphi_sq", max_iter=20000), fx, fb), 0)),
                          ("mixed_f32", lambda fx: lat.cg_mixed(lat.inv_par(MASS, prec, "Hphi_sq", sloppy="f32", max_iter=20000), fx, fb)),
-                         ("mixed_f16", lambda fx: lat.cg_mixed(lat.inv_par(MASS, prec, "Hphi_sq", sloppy="f16", max_iter=20000), fx, fb)),
-                         ("mixed_f32_h16links", lambda fx: mixed_half_links(lat, fx, fb, prec))):
+                         ("mixed_f16", lambda fx: lat.cg_mixed(lat.inv_par(MASS, prec, "Hphi_sq", sloppy="f16", max_iter=20000,
+                                                                           reliable_every=50), fx, fb)),
+                         ("mixed_f32_h16links", lambda fx: mixed_half_links(lat, fx, fb, prec)),
+                         ("bicgstab_fp64", lambda fx: (lat.bicgstab_inverter(lat.inv_par(MASS, prec, "Hphi_sq", max_iter=20000), fx, fb), 0)),
+                         ("bicgstab_mixed_f32", lambda fx: lat.bicgstab_mixed(lat.inv_par(MASS, prec, "Hphi_sq", max_iter=20000), fx, fb))):
             fx = lat.field("f64")
             lat.synchronize()
             t = time.perf_counter()
@@ -432,12 +452,17 @@ def main():
             # FP64 true residual with the FP64 GPU operator
             cg[name] = {"seconds": round(dt, 4), "iterations": it, "reliable_updates": upd,
                         "true_rel_residual": true_residual(lat, fx, fb)}
+            if name == "mixed_f16":
+                cg[name]["reliable_every"] = 50
+            if name.startswith("bicgstab"):
+                cg[name]["note"] = "BiCGstab on H^2 (iterations = 2 x BiCGstab steps, as bicgstab.cpp returns them)"
             log(f"cg {name}: {cg[name]}")
             fx.free()
         if not args.skip_learned:
             cg.update(learned_solves(yh, lat, fb, prec, args.train_steps))
         if "fp64" in cg:
-            for k in ("mixed_f32", "mixed_f16", "mixed_f32_h16links", "fcg_plain_f32", "learned_f32", "learned_small_f32"):
+            for k in ("mixed_f32", "mixed_f16", "mixed_f32_h16links", "bicgstab_fp64", "bicgstab_mixed_f32", "fcg_plain_f32",
+                      "learned_f32", "learned_small_f32"):
                 if k in cg:
                     cg[k]["speedup_vs_fp64"] = round(cg["fp64"]["seconds"] / cg[k]["seconds"], 2)
 

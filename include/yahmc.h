@@ -157,8 +157,15 @@ int yh_spinor_mulc_add_assign(yh_ctx* ctx, yh_field* a, double re, double im, co
 /* --- solvers (include/inverter.h:18-22) ------------------------------------ */
 /* cg_inverter (library/cg.cpp:132-202): dst holds the initial guess */
 int yh_cg(yh_ctx* ctx, const yh_inv_par* par, yh_field* dst, const yh_field* src, int* iters);
-/* bicgstab_inverter (library/bicgstab.cpp:9-97); returns 2 x iterations */
+/* bicgstab_inverter (library/bicgstab.cpp:9-97); returns 2 x iterations.
+ * dst holds the initial guess.  All scalars stay on the device. */
 int yh_bicgstab(yh_ctx* ctx, const yh_inv_par* par, yh_field* dst, const yh_field* src, int* iters);
+/* project-new: the same BiCGstab recurrence on FP32 fields and operator
+ * inside FP64 reliable updates (x += x_sloppy, r = b - A x in FP64 when |r|
+ * has dropped by par->delta or every par->reliable_every iterations, 100 if
+ * 0); stops on the FP64 true residual |r|^2 < prec |b|^2.  iters = 2 x
+ * sloppy iterations, updates = reliable updates */
+int yh_bicgstab_mixed(yh_ctx* ctx, const yh_inv_par* par, yh_field* dst, const yh_field* src, int* iters, int* updates);
 /* cg_mshift (library/cg.cpp:11-130): (mvm + shift[n]) dst[n] = src */
 int yh_cg_mshift(yh_ctx* ctx, const yh_inv_par* par, int num, const double* shift, yh_field** dst,
                  const yh_field* src, int* iters);
@@ -333,6 +340,12 @@ int yh_net_wgrad_host(yh_net* net, int layer, const double* g, const double* h, 
  * the FP64 true residual |b - H^2 x|^2 < prec |b|^2 */
 int yh_cg_net(yh_ctx* ctx, const yh_inv_par* par, yh_net* net, yh_field* dst, const yh_field* src, int* iters,
               int* updates);
+/* flexible GCR on Hphi_sq in FP32 preconditioned by A A^+ (net may be NULL),
+ * keeping the last `nkrylov` (1..32) directions, inside FP64 reliable
+ * updates (which restart the direction history); stops on the FP64 true
+ * residual |b - H^2 x|^2 < prec |b|^2 */
+int yh_gcr_net(yh_ctx* ctx, const yh_inv_par* par, yh_net* net, int nkrylov, yh_field* dst, const yh_field* src,
+               int* iters, int* updates);
 /* The learned solve with its fallback: flexible CG preconditioned by `net`,
  * started from the MRE guess of `mre` (either may be NULL); if it stops
  * without converging (cap par->max_iter, 20000 when 0, or a non-finite
@@ -371,6 +384,13 @@ int yh_synchronize(yh_ctx* ctx);
  * row rebuilt as conj(r0 x r1)): 1,152 instead of 1,536 bytes per hop site;
  * 0 (default): all 18 reals */
 #define YH_TUNE_RECON12_F64 5
+/* 1 (default): the batched multishift applies H^2 to up to four right-hand
+ * sides per hop launch, loading each link and clover block once for all of
+ * them; 0: one hop launch per right-hand side */
+#define YH_TUNE_MULTI_RHS 6
+/* 1: yh_cg_learned runs flexible GCR (8 kept directions) instead of
+ * flexible CG; 0 (default): flexible CG */
+#define YH_TUNE_LEARNED_GCR 7
 int yh_tune(yh_ctx* ctx, int key, int value);
 /* autotuner (no counterpart in the reference): times the hop of every
  * precision / reconstruction this context holds for each XCD chunk size

@@ -19,7 +19,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "lib", "libyahmc.so")
 
 YH_OK = 0
-TUNE_XCD_CHUNK, TUNE_NET_PREFETCH, TUNE_GRAPHS, TUNE_HALF_LINKS, TUNE_RECON12_F64 = 1, 2, 3, 4, 5
+TUNE_XCD_CHUNK, TUNE_NET_PREFETCH, TUNE_GRAPHS, TUNE_HALF_LINKS, TUNE_RECON12_F64, TUNE_MULTI_RHS = 1, 2, 3, 4, 5, 6
+TUNE_LEARNED_GCR = 7
 ERRORS = {-1: "invalid argument", -2: "HIP error", -3: "bad state", -4: "not converged", -5: "non-finite",
           -6: "unsupported"}
 
@@ -109,6 +110,8 @@ _SIGS = {
     "yh_spinor_mulr_add_assign": ([_VP, _VP, ctypes.c_double, _VP], ctypes.c_int),
     "yh_spinor_mulc_add_assign": ([_VP, _VP, ctypes.c_double, ctypes.c_double, _VP], ctypes.c_int),
     "yh_cg": ([_VP, ctypes.POINTER(InvPar), _VP, _VP, _PI], ctypes.c_int),
+    "yh_bicgstab_mixed": ([_VP, ctypes.POINTER(InvPar), _VP, _VP, _PI, _PI], ctypes.c_int),
+    "yh_gcr_net": ([_VP, ctypes.POINTER(InvPar), _VP, ctypes.c_int, _VP, _VP, _PI, _PI], ctypes.c_int),
     "yh_bicgstab": ([_VP, ctypes.POINTER(InvPar), _VP, _VP, _PI], ctypes.c_int),
     "yh_cg_mshift": ([_VP, ctypes.POINTER(InvPar), ctypes.c_int, _D, ctypes.POINTER(_VP), _VP, _PI], ctypes.c_int),
     "yh_cg_mixed": ([_VP, ctypes.POINTER(InvPar), _VP, _VP, _PI, _PI], ctypes.c_int),
@@ -485,6 +488,14 @@ class Lattice:
         self.set_gauge(links)
         return info, abs(info["plaquette"] - self.plaquette())
 
+    def bicgstab_mixed(self, par, d, s):
+        """BiCGstab on FP32 fields inside FP64 reliable updates; returns
+        (2 x iterations, reliable updates)."""
+        it = ctypes.c_int(0)
+        up = ctypes.c_int(0)
+        _check(lib().yh_bicgstab_mixed(self.ctx, ctypes.byref(par), d.h, s.h, ctypes.byref(it), ctypes.byref(up)))
+        return it.value, up.value
+
     def minres_inverter(self, par, d, s):
         """minres_inverter (library/minres.cpp:12-118); d holds the guess."""
         it = ctypes.c_int(0)
@@ -513,6 +524,16 @@ class Lattice:
         up = ctypes.c_int(0)
         _check(lib().yh_cg_net(self.ctx, ctypes.byref(par), net.h if net is not None else None, d.h, s.h,
                                ctypes.byref(it), ctypes.byref(up)))
+        return it.value, up.value
+
+    def gcr_net(self, par, net, d, s, nkrylov=8):
+        """Flexible GCR on Hphi_sq preconditioned by net (None: plain GCR),
+        keeping the last nkrylov directions, FP32 inside FP64 reliable
+        updates; returns (iterations, reliable updates)."""
+        it = ctypes.c_int(0)
+        up = ctypes.c_int(0)
+        _check(lib().yh_gcr_net(self.ctx, ctypes.byref(par), net.h if net is not None else None, nkrylov, d.h, s.h,
+                                ctypes.byref(it), ctypes.byref(up)))
         return it.value, up.value
 
     def cg_learned(self, par, net, mre, d, s):

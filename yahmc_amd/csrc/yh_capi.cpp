@@ -187,12 +187,13 @@ int apply_op(Ctx* c, int prec, int recon, int mvm, double mass, Field* out, Fiel
   int rc;
   const bool clover = c->csw != 0.0;
   if (clover && (rc = clover_invert(c, 4.0 + mass))) return rc;
-  Field* t1 = tmp_field(c, prec, 0, &rc);
-  if (rc) return rc;
+  const bool own = c->op_tmp[0] && c->op_tmp[0]->prec == prec;
+  Field* t1 = own ? c->op_tmp[0] : tmp_field(c, prec, 0, &rc);
+  if (!own && rc) return rc;
   Field* t2 = nullptr;
   if (mvm == YH_MVM_HPHI_SQ) {
-    t2 = tmp_field(c, prec, 1, &rc);
-    if (rc) return rc;
+    t2 = own ? c->op_tmp[1] : tmp_field(c, prec, 1, &rc);
+    if (!own && rc) return rc;
   }
   auto half_op = [&](Field* dst, Field* src, int g5, int red, Field* sidef) -> int {
     HopCall h1 = {};
@@ -213,6 +214,26 @@ int apply_op(Ctx* c, int prec, int recon, int mvm, double mass, Field* out, Fiel
   if (mvm == YH_MVM_HPHI) return half_op(out, in, 1, -1, side);
   if ((rc = half_op(t2, in, 1, reduce_slot, side))) return rc;
   return half_op(out, t2, 1, -1, nullptr);
+}
+
+int apply_hsq_mr(Ctx* c, int prec, double mass, int nr, Field* const* out, Field* const* in, Field* const* t1,
+                 Field* const* t2, double* const* red, const int* const* skip) {
+  int rc;
+  if (c->csw == 0.0) { set_error("multi-RHS operator needs the clover term"); return YH_ERR_UNSUPPORTED; }
+  if ((rc = clover_invert(c, 4.0 + mass))) return rc;
+  // the two halves of Hphi, each as in apply_op (dirac.cpp:738-781)
+  auto half_op = [&](Field* const* dst, Field* const* src, double* const* rd) -> int {
+    HopCall h1 = {};
+    h1.prec = prec; h1.recon = RECON_18; h1.out_parity = 1; h1.scale = 0.5; h1.epi = EPI_CLOVINV;
+    int r = hop_launch_mr(c, h1, nr, src, t1, nullptr, nullptr, skip);
+    if (r) return r;
+    HopCall h2 = {};
+    h2.prec = prec; h2.recon = RECON_18; h2.out_parity = 0; h2.scale = -0.5; h2.epi = EPI_CLOV; h2.gamma5 = 1;
+    h2.diag = 4.0 + mass; h2.zero_self_bc = 1; h2.reduce_norm = rd != nullptr;
+    return hop_launch_mr(c, h2, nr, t1, dst, src, rd, skip);
+  };
+  if ((rc = half_op(t2, in, red))) return rc;
+  return half_op(out, t2, nullptr);
 }
 
 // ---------------------------------------------------------------------------
@@ -239,8 +260,10 @@ int reduce_read(Ctx* c, int which, const Field* a, const Field* b, int par, doub
   else if (which == 1) rc = blas_dot(c, a, b, par, 0);
   else rc = blas_dot_re(c, a, b, par, 0);
   if (rc) return rc;
-  YH_HIP(hipMemcpyAsync(out2, &c->sc->red[0], 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  YH_HIP(hipMemcpyAsync(c->red_host, &c->sc->red[0], 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   YH_HIP(hipStreamSynchronize(c->stream));
+  out2[0] = c->red_host[0];
+  out2[1] = c->red_host[1];
   return YH_OK;
 }
 
@@ -252,12 +275,13 @@ int reduce_read(Ctx* c, int which, const Field* a, const Field* b, int par, doub
 // between host checks; iterations after convergence are no-ops.
 int cg_device(Ctx* c, int prec, int recon, const yh_inv_par* par, Field* x, const Field* b, int* iters,
                      double* final_rr, double* norm_b_out) {
-  int rc;
-  Field* r = tmp_field(c, prec, 2, &rc);
+  int rc = YH_OK;
+  const bool own = c->cg_tmp[0] && c->cg_tmp[0]->prec == prec;
+  Field* r = own ? c->cg_tmp[0] : tmp_field(c, prec, 2, &rc);
   if (rc) return rc;
-  Field* p = tmp_field(c, prec, 3, &rc);
+  Field* p = own ? c->cg_tmp[1] : tmp_field(c, prec, 3, &rc);
   if (rc) return rc;
-  Field* qq = tmp_field(c, prec, 4, &rc);
+  Field* qq = own ? c->cg_tmp[2] : tmp_field(c, prec, 4, &rc);
   if (rc) return rc;
   double norm_b[2];
   if ((rc = reduce_read(c, 0, b, nullptr, 0, norm_b))) return rc;
@@ -366,12 +390,17 @@ int yh_create(const yh_lattice* lat, yh_ctx** out) {
   // +10% over one slab per XCD, +15% over the hardware round robin)
   c->g.xq = std::max(1, (int)lround((double)c->g.VH / c->g.L[0] / (256.0 * 8.0)));
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipMalloc(&c->partials, sizeof(double) * (size_t)(c->nblk + 64) * 8);
+  // partial sums: up to 8 values per workgroup of a one-site-per-thread
+  // kernel, and up to 4 values per workgroup of the multi-RHS hop, whose grid
+  // is 4 times larger
+  c->partials_cap = (size_t)(c->nblk + 64) * 16;
+  if (e == hipSuccess) e = hipMalloc(&c->partials, sizeof(double) * c->partials_cap);
   if (e == hipSuccess) e = hipMalloc(&c->ticket, sizeof(unsigned) * 64);
   if (e == hipSuccess) e = hipMemset(c->ticket, 0, sizeof(unsigned) * 64);
   if (e == hipSuccess) e = hipMalloc(&c->sc, sizeof(SolverScalars));
   if (e == hipSuccess) e = hipMemset(c->sc, 0, sizeof(SolverScalars));
   if (e == hipSuccess) e = hipHostMalloc((void**)&c->sc_host, sizeof(SolverScalars), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&c->red_host, 2 * sizeof(double), hipHostMallocDefault);
   if (e == hipSuccess) e = hipMalloc(&c->fund, sizeof(double) * 4 * (size_t)V * 2 * lat->nc * lat->nc);
   if (e == hipSuccess) e = hipMalloc(&c->tab, sizeof(double2) * (TAB_BASIS + TAB_ITR + TAB_ITF));
   if (e != hipSuccess) {
@@ -417,6 +446,7 @@ int yh_destroy(yh_ctx* c) {
   if (c->ticket) (void)hipFree(c->ticket);
   if (c->sc) (void)hipFree(c->sc);
   if (c->sc_host) (void)hipHostFree(c->sc_host);
+  if (c->red_host) (void)hipHostFree(c->red_host);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return YH_OK;
@@ -434,6 +464,7 @@ int yh_set_gauge(yh_ctx* c, const double* links) {
   if ((rc = gauge_represent(c))) return rc;
   if ((rc = gauge_derive_links(c))) return rc;
   if ((rc = clover_build(c))) return rc;
+  if (c->csw != 0.0 && (rc = clover_reserve_inverse(c))) return rc;
   YH_HIP(hipStreamSynchronize(c->stream));
   c->gauge_set = true;
   return YH_OK;
@@ -643,6 +674,14 @@ int yh_tune(yh_ctx* c, int key, int value) {
   }
   if (key == YH_TUNE_HALF_LINKS) {
     c->half_links = value != 0;
+    return YH_OK;
+  }
+  if (key == YH_TUNE_LEARNED_GCR) {
+    c->learned_gcr = value != 0;
+    return YH_OK;
+  }
+  if (key == YH_TUNE_MULTI_RHS) {
+    c->multi_rhs = value != 0;
     return YH_OK;
   }
   if (key == YH_TUNE_RECON12_F64) {

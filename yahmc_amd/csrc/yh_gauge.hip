@@ -638,12 +638,26 @@ int clover_build(Ctx* c) {
   return YH_OK;
 }
 
-int clover_invert(Ctx* c, double sigma) {
-  if (sigma == c->cl_sigma) return YH_OK;  // clover.cpp:314-321
+// the inverse buffers of every precision, allocated once per context (at
+// yh_set_gauge), so that solves never allocate
+int clover_reserve_inverse(Ctx* c) {
   int N = 2 * c->dr;
   size_t bytes = (size_t)N * N * c->g.VHpad * sizeof(double2);  // odd sites only
   if (!c->clinv[PREC_F64]) YH_HIP(hipMalloc(&c->clinv[PREC_F64], bytes));
   if (!c->logdet_site) YH_HIP(hipMalloc(&c->logdet_site, sizeof(double) * c->g.VHpad));
+  size_t b32 = (size_t)(N * N / 2) * c->g.VHpad * sizeof(float4);
+  if (!c->clinv[PREC_F32]) YH_HIP(hipMalloc(&c->clinv[PREC_F32], b32));
+  size_t b16 = (size_t)(N * N / 2) * c->g.VHpad * sizeof(uint2);
+  if (!c->clinv[PREC_F16]) YH_HIP(hipMalloc(&c->clinv[PREC_F16], b16));
+  if (!c->clinv_nrm) YH_HIP(hipMalloc(&c->clinv_nrm, sizeof(float) * 2 * c->g.VHpad));
+  return YH_OK;
+}
+
+int clover_invert(Ctx* c, double sigma) {
+  if (sigma == c->cl_sigma) return YH_OK;  // clover.cpp:314-321
+  int N = 2 * c->dr;
+  int rc = clover_reserve_inverse(c);
+  if (rc) return rc;
   const double2* src = (const double2*)c->clov[PREC_F64];
   double2* inv = (double2*)c->clinv[PREC_F64];
   switch (c->dr) {
@@ -654,16 +668,11 @@ int clover_invert(Ctx* c, double sigma) {
     default: set_error("unsupported representation dimension"); return YH_ERR_UNSUPPORTED;
   }
   YH_HIP(hipGetLastError());
-  size_t b32 = (size_t)(N * N / 2) * c->g.VHpad * sizeof(float4);
-  if (!c->clinv[PREC_F32]) YH_HIP(hipMalloc(&c->clinv[PREC_F32], b32));
   dim3 grid(grid1(c->g.VH), N * N / 2);
   hipLaunchKernelGGL(k_clover_to_f32, grid, dim3(256), 0, c->stream, inv, (float4*)c->clinv[PREC_F32], N * N / 2,
                      c->g.VHpad, c->g.VH);
   YH_HIP(hipGetLastError());
   {
-    size_t b16 = (size_t)(N * N / 2) * c->g.VHpad * sizeof(uint2);
-    if (!c->clinv[PREC_F16]) YH_HIP(hipMalloc(&c->clinv[PREC_F16], b16));
-    if (!c->clinv_nrm) YH_HIP(hipMalloc(&c->clinv_nrm, sizeof(float) * 2 * c->g.VHpad));
     hipLaunchKernelGGL(k_clover_to_q16, dim3(grid1(c->g.VH)), dim3(256), 0, c->stream, inv,
                        (uint2*)c->clinv[PREC_F16], c->clinv_nrm, N * N, c->g.VHpad, c->g.VH);
     YH_HIP(hipGetLastError());

@@ -69,7 +69,8 @@ struct Ctx {
   double* logdet_site = nullptr;  // per odd site log det (for the sum)
   bool gauge_set = false;
   // reductions
-  double* partials = nullptr;      // nblk * RED_SLOTS doubles
+  double* partials = nullptr;      // per-workgroup partial sums of the grid reductions
+  size_t partials_cap = 0;         // doubles in `partials`
   unsigned* ticket = nullptr;      // last-block counter
   SolverScalars* sc = nullptr;     // device scalars
   SolverScalars* sc_host = nullptr;  // pinned mirror
@@ -85,6 +86,8 @@ struct Ctx {
   int xq_tab[3][3] = {{-1, -1, -1}, {-1, -1, -1}, {-1, -1, -1}};
   int half_links = 0;  // FP32 operators of SU(3) fundamental read FP16 12-real links
   int recon12_f64 = 0;  // FP64 operators of SU(3) fundamental read 12-real links
+  int multi_rhs = 1;    // batched multishift: one hop launch per batch of right-hand sides
+  int learned_gcr = 0;  // the learned solve uses flexible GCR instead of flexible CG
   // solver chunks are captured once into a HIP graph and replayed
   int use_graphs = 1;
   // model tables of this context's (nc, representation), in device memory
@@ -95,6 +98,13 @@ struct Ctx {
   double2* tab_basis = nullptr;  // TAB_BASIS entries
   double2* tab_itr = nullptr;    // TAB_ITR entries
   double2* tab_itf = nullptr;    // TAB_ITF entries
+  // pinned host copy of the last reduction read back (2 doubles)
+  double* red_host = nullptr;
+  // caller-provided temporaries (the no-allocation entry point): when set,
+  // apply_op uses op_tmp[0] (odd half) and op_tmp[1] (even half) and
+  // cg_device uses cg_tmp[0..2] (r, p, q) instead of the context's own
+  Field* op_tmp[2] = {nullptr, nullptr};
+  Field* cg_tmp[3] = {nullptr, nullptr, nullptr};
 };
 enum { TAB_BASIS = 15 * 16, TAB_ITR = 15 * 64, TAB_ITF = 15 * 16 };
 // copies `n` table entries to device memory on the context stream and waits
@@ -115,6 +125,7 @@ int gauge_represent(Ctx* c);                 // fund -> F64 R18 links
 int gauge_derive_links(Ctx* c);              // F64 R18 -> other precisions
 int clover_build(Ctx* c);                    // packed FP64 clover
 int clover_invert(Ctx* c, double sigma);     // odd-site inverses (+ logdet)
+int clover_reserve_inverse(Ctx* c);          // allocate the inverse buffers once
 int clover_logdet_sum(Ctx* c, double* out);
 int gauge_download_repr(Ctx* c, double* host);    // reference gfield_repr layout, BC sign applied
 int clover_download(Ctx* c, double* host);         // reference cl_term layout
@@ -139,6 +150,8 @@ struct HopCall {
   const int* skip_flag;   // if non-null and *skip_flag, every thread returns
 };
 int hop_launch(Ctx* c, const HopCall& h);
+int hop_launch_mr(Ctx* c, const HopCall& h, int nr, const Field* const* in, Field* const* out, const Field* const* self,
+                  double* const* red, const int* const* skip);
 int clover_apply(Ctx* c, int prec, int parity, double diag, Field* out, const Field* in, int inverse);
 
 // ---- blas -------------------------------------------------------------------
@@ -184,6 +197,11 @@ Field* tmp_field(Ctx* c, int prec, int slot, int* rc);
 int default_recon(Ctx* c, int prec);
 int apply_op(Ctx* c, int prec, int recon, int mvm, double mass, Field* out, Field* in, Field* side, int reduce_slot,
              const int* skip, double* reduce_ptr = nullptr);
+// Hphi_sq on nr right-hand sides (2 or 4) through multi-RHS hop launches:
+// out[r] = H^2 in[r] with t1[r] (odd halves) and t2[r] (even halves) as
+// temporaries; red[r] receives |H in[r]|^2; skip[r] stops right-hand side r
+int apply_hsq_mr(Ctx* c, int prec, double mass, int nr, Field* const* out, Field* const* in, Field* const* t1,
+                 Field* const* t2, double* const* red, const int* const* skip);
 // hop launches of one apply_op (Hphi_sq: 4, Dphi / Hphi: 2)
 inline int hops_per_apply(int mvm) { return mvm == YH_MVM_HPHI_SQ ? 4 : 2; }
 int mshift_device(Ctx* c, const yh_inv_par* par, int nrhs, int num, const double* shift, Field** x, const Field** b,
@@ -193,6 +211,9 @@ int write_scalars(Ctx* c);
 int reduce_read(Ctx* c, int which, const Field* a, const Field* b, int par, double* out2);
 int cg_device(Ctx* c, int prec, int recon, const yh_inv_par* par, Field* x, const Field* b, int* iters,
               double* final_rr, double* norm_b_out);
+// BiCGstab with device-resident scalars: FP64 (sloppy = PREC_F64) or FP32
+// inside FP64 reliable updates (sloppy = PREC_F32); iters = 2 x iterations
+int bicgstab_device(Ctx* c, const yh_inv_par* par, int sloppy, Field* x, const Field* b, int* iters, int* updates);
 
 // CG step kernels operating on device scalars (no host round trip)
 int cg_update_xr(Ctx* c, Field* x, Field* r, const Field* p, const Field* q, int parity, int prec);
