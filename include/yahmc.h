@@ -157,6 +157,16 @@ int yh_spinor_mulc_add_assign(yh_ctx* ctx, yh_field* a, double re, double im, co
 /* --- solvers (include/inverter.h:18-22) ------------------------------------ */
 /* cg_inverter (library/cg.cpp:132-202): dst holds the initial guess */
 int yh_cg(yh_ctx* ctx, const yh_inv_par* par, yh_field* dst, const yh_field* src, int* iters);
+/* No counterpart in the reference: yh_cg without any allocation, for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The five temporary half
+ * fields (5 x 64 * D_R * VHpad bytes, VHpad = 64 * ceil(V / 128) + 64, each
+ * rounded up to 256 bytes) come from a library-owned device array of
+ * YH_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; a larger
+ * lattice returns YH_ERR_ARG.  The gauge field must be set.  Calls that
+ * overlap on one GPU (any contexts) share that array and race. */
+#define YH_NOALLOC_SCRATCH_BYTES (32u << 20)
+int yh_cg_noalloc(yh_ctx* ctx, const yh_inv_par* par, yh_field* dst, const yh_field* src, int* iters);
 /* bicgstab_inverter (library/bicgstab.cpp:9-97); returns 2 x iterations.
  * dst holds the initial guess.  All scalars stay on the device. */
 int yh_bicgstab(yh_ctx* ctx, const yh_inv_par* par, yh_field* dst, const yh_field* src, int* iters);

@@ -110,6 +110,7 @@ _SIGS = {
     "yh_spinor_mulr_add_assign": ([_VP, _VP, ctypes.c_double, _VP], ctypes.c_int),
     "yh_spinor_mulc_add_assign": ([_VP, _VP, ctypes.c_double, ctypes.c_double, _VP], ctypes.c_int),
     "yh_cg": ([_VP, ctypes.POINTER(InvPar), _VP, _VP, _PI], ctypes.c_int),
+    "yh_cg_noalloc": ([_VP, ctypes.POINTER(InvPar), _VP, _VP, _PI], ctypes.c_int),
     "yh_bicgstab_mixed": ([_VP, ctypes.POINTER(InvPar), _VP, _VP, _PI, _PI], ctypes.c_int),
     "yh_gcr_net": ([_VP, ctypes.POINTER(InvPar), _VP, ctypes.c_int, _VP, _VP, _PI, _PI], ctypes.c_int),
     "yh_bicgstab": ([_VP, ctypes.POINTER(InvPar), _VP, _VP, _PI], ctypes.c_int),
@@ -427,6 +428,13 @@ class Lattice:
     def cg_inverter(self, par, d, s):
         it = ctypes.c_int(0)
         _check(lib().yh_cg(self.ctx, ctypes.byref(par), d.h, s.h, ctypes.byref(it)))
+        return it.value
+
+    def cg_noalloc(self, par, d, s):
+        """yh_cg without allocation (temporaries in the library's per-GPU
+        scratch array; see include/yahmc.h)."""
+        it = ctypes.c_int(0)
+        _check(lib().yh_cg_noalloc(self.ctx, ctypes.byref(par), d.h, s.h, ctypes.byref(it)))
         return it.value
 
     def bicgstab_inverter(self, par, d, s):

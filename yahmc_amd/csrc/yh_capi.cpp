@@ -411,7 +411,8 @@ int yh_create(const yh_lattice* lat, yh_ctx** out) {
   c->tab_basis = c->tab;
   c->tab_itr = c->tab + TAB_BASIS;
   c->tab_itf = c->tab + TAB_BASIS + TAB_ITR;
-  int rc = gauge_init_basis(c);
+  int rc = noalloc_init(c->device);
+  if (!rc) rc = gauge_init_basis(c);
   if (!rc) rc = force_init_constants(c);
   if (!rc) rc = md_init_constants(c);
   if (!rc) rc = meson_init_tables();

@@ -209,6 +209,8 @@ int mshift_device(Ctx* c, const yh_inv_par* par, int nrhs, int num, const double
 int read_scalars(Ctx* c);
 int write_scalars(Ctx* c);
 int reduce_read(Ctx* c, int which, const Field* a, const Field* b, int par, double* out2);
+// once per device (yh_create): device address of the no-allocation scratch array
+int noalloc_init(int device);
 int cg_device(Ctx* c, int prec, int recon, const yh_inv_par* par, Field* x, const Field* b, int* iters,
               double* final_rr, double* norm_b_out);
 // BiCGstab with device-resident scalars: FP64 (sloppy = PREC_F64) or FP32
