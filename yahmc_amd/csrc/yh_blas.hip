@@ -191,6 +191,97 @@ __global__ void __launch_bounds__(256) k_cg_p(void* p, float* pn, const void* r,
   store_spinor<PP, D, R>(p, pn, k, g.VHpad, Pv);
 }
 
+// Reliable update of the mixed CG, first half: x (FP64) += xs (FP32 sloppy
+// solution), xs = 0, and, if the sloppy recurrence stopped on its own flag
+// (the iteration that raised it skipped p = r + beta p), that p step.
+template <int PP, int D>
+__global__ void __launch_bounds__(256) k_ru_begin(void* x, void* xs, void* p, float* pn, const void* rs,
+                                                  const SolverScalars* sc, Geom g) {
+  if (sc->finished) return;
+  const bool complete_p = sc->converged != 0;
+  const float beta = (float)sc->beta;
+  YH_SITE_LOOP_BEGIN
+  if (!act) return;
+  Cx<float> Xs[4][D];
+  Cx<double> X[4][D];
+  load_spinor<PREC_F32, D, float>(Xs, xs, nullptr, k, g.VHpad);
+  load_spinor<PREC_F64, D, double>(X, x, nullptr, k, g.VHpad);
+  if (complete_p) {
+    Cx<float> Pv[4][D], Rv[4][D];
+    load_spinor<PP, D, float>(Pv, p, pn, k, g.VHpad);
+    load_spinor<PREC_F32, D, float>(Rv, rs, nullptr, k, g.VHpad);
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+#pragma unroll
+      for (int i = 0; i < D; i++) Pv[s][i] = Cx<float>(Rv[s][i].re + beta * Pv[s][i].re, Rv[s][i].im + beta * Pv[s][i].im);
+    store_spinor<PP, D, float>(p, pn, k, g.VHpad, Pv);
+  }
+#pragma unroll
+  for (int s = 0; s < 4; s++)
+#pragma unroll
+    for (int i = 0; i < D; i++) {
+      X[s][i] = Cx<double>(X[s][i].re + (double)Xs[s][i].re, X[s][i].im + (double)Xs[s][i].im);
+      Xs[s][i] = Cx<float>(0.0f, 0.0f);
+    }
+  store_spinor<PREC_F64, D, double>(x, nullptr, k, g.VHpad, X);
+  store_spinor<PREC_F32, D, float>(xs, nullptr, k, g.VHpad, Xs);
+}
+
+// Second half: r = b - t with t = A x in FP64; rs = r in FP32; |r|^2 in
+// FP64.  The last workgroup ends the solve (target reached, non-finite, or
+// 10 updates in a row without a lower true residual) or restarts the
+// sloppy recurrence from the true residual.
+template <int D>
+__global__ void __launch_bounds__(256) k_ru_end(const void* b, const void* t, void* rs, SolverScalars* sc,
+                                                double* partials, unsigned* ticket, Geom g) {
+  if (sc->finished) return;
+  YH_SITE_LOOP_BEGIN
+  double v[1] = {0.0};
+  if (act) {
+    Cx<double> B[4][D], T[4][D];
+    load_spinor<PREC_F64, D, double>(B, b, nullptr, k, g.VHpad);
+    load_spinor<PREC_F64, D, double>(T, t, nullptr, k, g.VHpad);
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+#pragma unroll
+      for (int i = 0; i < D; i++) {
+        B[s][i] = Cx<double>(B[s][i].re - T[s][i].re, B[s][i].im - T[s][i].im);
+        v[0] += B[s][i].re * B[s][i].re + B[s][i].im * B[s][i].im;
+      }
+    store_spinor<PREC_F32, D, double>(rs, nullptr, k, g.VHpad, B);
+  }
+  double res[1];
+  if (block_reduce_finish<1, 256>(v, partials, gridDim.x, blockIdx.x, ticket, res) && threadIdx.x == 0) {
+    const double rr = res[0];
+    sc->rr_true = rr;
+    sc->nupd += 1;
+    if (!(rr == rr)) {
+      sc->finished = 2;
+      sc->converged = 1;
+    } else if (rr < sc->target) {
+      sc->finished = 1;
+      sc->converged = 1;
+      sc->reason = 1;
+    } else {
+      if (rr < sc->best_rr) {
+        sc->best_rr = rr;
+        sc->stalls = 0;
+      } else {
+        sc->stalls += 1;
+      }
+      if (sc->stalls >= 10) {
+        sc->finished = 3;
+        sc->converged = 1;
+      } else {
+        sc->rr = rr;
+        sc->maxrr = rr;
+        sc->converged = 0;
+        sc->reason = 0;
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 #define YH_D_SWITCH(CALL)                                   \
   switch (c->dr) {                                          \
@@ -305,6 +396,33 @@ int cg_update_p(Ctx* c, Field* p, const Field* r, int par, int prec) {
   else if (r->prec == PREC_F32 && p->prec == PREC_F16) { YH_PU(PREC_F32, PREC_F16); }
   else { set_error("unsupported CG precision pair"); return YH_ERR_UNSUPPORTED; }
 #undef YH_PU
+  YH_HIP(hipGetLastError());
+  return YH_OK;
+}
+
+int ru_begin(Ctx* c, Field* x, Field* xs, Field* p, const Field* rs) {
+  if (x->prec != PREC_F64 || xs->prec != PREC_F32 || rs->prec != PREC_F32 || p->prec == PREC_F64) {
+    set_error("reliable update: unsupported precisions");
+    return YH_ERR_ARG;
+  }
+  if (p->prec == PREC_F32) {
+    YH_D_SWITCH(hipLaunchKernelGGL((k_ru_begin<PREC_F32, D_>), gsz(c), dim3(256), 0, c->stream, x->half[0], xs->half[0],
+                                   p->half[0], p->nrm[0], rs->half[0], c->sc, c->g));
+  } else {
+    YH_D_SWITCH(hipLaunchKernelGGL((k_ru_begin<PREC_F16, D_>), gsz(c), dim3(256), 0, c->stream, x->half[0], xs->half[0],
+                                   p->half[0], p->nrm[0], rs->half[0], c->sc, c->g));
+  }
+  YH_HIP(hipGetLastError());
+  return YH_OK;
+}
+
+int ru_end(Ctx* c, const Field* b, const Field* t, Field* rs) {
+  if (b->prec != PREC_F64 || t->prec != PREC_F64 || rs->prec != PREC_F32) {
+    set_error("reliable update: unsupported precisions");
+    return YH_ERR_ARG;
+  }
+  YH_D_SWITCH(hipLaunchKernelGGL((k_ru_end<D_>), gsz(c), dim3(256), 0, c->stream, b->half[0], t->half[0], rs->half[0],
+                                 c->sc, c->partials, c->ticket, c->g));
   YH_HIP(hipGetLastError());
   return YH_OK;
 }

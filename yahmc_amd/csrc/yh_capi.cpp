@@ -401,6 +401,10 @@ int yh_create(const yh_lattice* lat, yh_ctx** out) {
   if (e == hipSuccess) e = hipMemset(c->sc, 0, sizeof(SolverScalars));
   if (e == hipSuccess) e = hipHostMalloc((void**)&c->sc_host, sizeof(SolverScalars), hipHostMallocDefault);
   if (e == hipSuccess) e = hipHostMalloc((void**)&c->red_host, 2 * sizeof(double), hipHostMallocDefault);
+  if (e == hipSuccess)
+    e = hipHostMalloc((void**)&c->sc_ring, Ctx::SC_RING * sizeof(SolverScalars), hipHostMallocDefault);
+  for (int i = 0; i < Ctx::SC_RING && e == hipSuccess; i++)
+    e = hipEventCreateWithFlags(&c->ev_ring[i], hipEventDisableTiming);
   if (e == hipSuccess) e = hipMalloc(&c->fund, sizeof(double) * 4 * (size_t)V * 2 * lat->nc * lat->nc);
   if (e == hipSuccess) e = hipMalloc(&c->tab, sizeof(double2) * (TAB_BASIS + TAB_ITR + TAB_ITF));
   if (e != hipSuccess) {
@@ -448,6 +452,9 @@ int yh_destroy(yh_ctx* c) {
   if (c->sc) (void)hipFree(c->sc);
   if (c->sc_host) (void)hipHostFree(c->sc_host);
   if (c->red_host) (void)hipHostFree(c->red_host);
+  if (c->sc_ring) (void)hipHostFree(c->sc_ring);
+  for (int i = 0; i < Ctx::SC_RING; i++)
+    if (c->ev_ring[i]) (void)hipEventDestroy(c->ev_ring[i]);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return YH_OK;

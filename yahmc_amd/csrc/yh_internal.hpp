@@ -41,11 +41,16 @@ struct SolverScalars {
   double norm_b;
   double maxrr;     // reliable-update reference norm: max |r|^2 since the last update
   double delta2;    // stop for a reliable update once |r|^2 < delta2 * maxrr (0: off)
+  double best_rr;   // mixed CG: lowest true |r|^2 so far
+  double rr_true;   // mixed CG: true |r|^2 of the last reliable update
   double red[RED_SLOTS];  // general reduction results
   int converged;    // stop flag: every solver kernel returns at once while set
   int iters;
   int reason;       // why `converged` was set: 1 target reached / non-finite, 2 reliable update due
-  int pad[13];
+  int finished;     // mixed CG: 1 converged, 2 non-finite, 3 stalled (the reliable-update kernels stop)
+  int nupd;         // mixed CG: reliable updates that ran
+  int stalls;       // mixed CG: reliable updates in a row that did not lower the true residual
+  int pad[10];
 };
 
 struct Ctx {
@@ -74,6 +79,11 @@ struct Ctx {
   unsigned* ticket = nullptr;      // last-block counter
   SolverScalars* sc = nullptr;     // device scalars
   SolverScalars* sc_host = nullptr;  // pinned mirror
+  // pinned snapshots of the scalars read back while later work is queued
+  // (mixed CG), and the events that mark each copy complete
+  enum { SC_RING = 4 };
+  SolverScalars* sc_ring = nullptr;
+  hipEvent_t ev_ring[SC_RING] = {};
   long mvm_counter = 0;
   // operator scratch (odd halves / even halves)
   Field* tmp[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -220,6 +230,14 @@ int bicgstab_device(Ctx* c, const yh_inv_par* par, int sloppy, Field* x, const F
 // CG step kernels operating on device scalars (no host round trip)
 int cg_update_xr(Ctx* c, Field* x, Field* r, const Field* p, const Field* q, int parity, int prec);
 int cg_update_p(Ctx* c, Field* p, const Field* r, int parity, int prec);
+// reliable update of the mixed CG on the device, both halves no-ops once
+// sc->finished is set.  begin: complete the p = r + beta p step an iteration
+// that raised the stop flag skipped (if sc->converged), x += xs, xs = 0.
+// end: rs = b - t (t = A x in FP64), |b - t|^2; the last workgroup decides:
+// finished (target reached, non-finite, or 10 updates without progress) or
+// restart the sloppy recurrence (rr = maxrr = true |r|^2, stop flag cleared)
+int ru_begin(Ctx* c, Field* x, Field* xs, Field* p, const Field* rs);
+int ru_end(Ctx* c, const Field* b, const Field* t, Field* rs);
 
 }  // namespace yh
 

@@ -14,6 +14,7 @@
 #include <string.h>
 #include <algorithm>
 #include <complex>
+#include <deque>
 #include <vector>
 #include "yh_internal.hpp"
 
@@ -222,7 +223,6 @@ int yh_cg_mixed(yh_ctx* cc, const yh_inv_par* par, yh_field* dptr, const yh_fiel
   };
   double rr;
   if ((rc = true_residual(&rr))) return rc;
-  int nupd = 0;
   *iters = 0;
   if (rr < target) {
     if (updates) *updates = 0;
@@ -231,84 +231,120 @@ int yh_cg_mixed(yh_ctx* cc, const yh_inv_par* par, yh_field* dptr, const yh_fiel
   if ((rc = field_convert(c, rs, 0, r64, 0)) || (rc = field_convert(c, p, 0, r64, 0)) || (rc = field_zero(c, xs, YH_EVEN)))
     return rc;
   // Reliable updates (Sleijpen & van der Vorst; Clark et al. for lattice
-  // QCD): the sloppy recurrence runs on device until |r| has dropped by
+  // QCD): the sloppy recurrence runs on the device until |r| has dropped by
   // `delta` since the last update (flag raised by the CG update kernel) or
   // `reliable_every` iterations have passed; then x += xs and the residual
   // is recomputed in FP64.  The search direction p is kept.
+  //
+  // The update itself runs on the device (ru_begin, the FP64 operator,
+  // ru_end, whose last workgroup decides whether the solve is finished), so
+  // the host never waits inside an update.  The host queues a fixed plan
+  // (segments of K sloppy iterations in chunks of at most 16, each followed
+  // by an update) up to two items ahead of the one whose scalar snapshot it
+  // reads; items queued after the solve finished, or sloppy items queued
+  // after the recurrence stopped on its own flag, return at once.  An early
+  // stop moves the next update forward.
   const double delta = (par->delta > 0.0 && par->delta < 1.0) ? par->delta : 0.1;
   SolverScalars* h = c->sc_host;
   memset(h, 0, sizeof(SolverScalars));
   h->rr = rr;
   h->maxrr = rr;
+  h->best_rr = rr;
   h->delta2 = delta * delta;
   h->target = target;
   h->norm_b = norm_b;
   if ((rc = write_scalars(c))) return rc;
-  int total = 0, at_update = 0;
-  // reliable updates in a row that do not lower the true residual: the
-  // sloppy recurrence has stopped helping, and the solve ends with NOCONV
-  // (it would otherwise loop for ever when max_iter is 0)
-  const int max_stalls = 10;
-  int stalls = 0;
-  double best_rr = rr;
-  const int chunk = 16;
-  while (true) {
-    int n = std::min(chunk, std::max(1, K - (total - at_update)));
-    if (par->max_iter > 0) n = std::min(n, std::max(1, par->max_iter - total));
-    for (int i = 0; i < n; i++) {
-      if ((rc = apply_op(c, S, recons, YH_MVM_HPHI_SQ, par->mass, q, p, nullptr, 0, &c->sc->converged))) return rc;
-      if ((rc = cg_update_xr(c, xs, rs, p, q, 0, PREC_F32))) return rc;
-      if ((rc = cg_update_p(c, p, rs, 0, PREC_F32))) return rc;
-    }
-    if ((rc = read_scalars(c))) return rc;
-    c->mvm_counter += (long)(h->iters - total) * hops_per_apply(YH_MVM_HPHI_SQ);
-    total = h->iters;
-    if (par->log)
-      fprintf(stderr, "[INVERTER][DEBUG] cg_mixed: iters=%d rr=%.6e pap=%.6e alpha=%.6e beta=%.6e stop=%d reason=%d\n",
-              h->iters, h->rr, h->red[0], h->alpha, h->beta, h->converged, h->reason);
-    if (!(h->rr == h->rr)) { set_error("cg_mixed: non-finite residual"); return YH_ERR_NONFINITE; }
-    const bool capped = par->max_iter > 0 && total >= par->max_iter;
-    if (!h->converged && total - at_update < K && !capped) continue;
-    if (h->converged) {
-      // the iteration that raised the stop flag skipped its p = r + beta p
-      // step; complete it with the iterated residual before r is replaced
-      h->converged = 0;
-      if ((rc = write_scalars(c))) return rc;
-      if ((rc = cg_update_p(c, p, rs, 0, PREC_F32))) return rc;
-    }
-    // reliable update: x += xs ; r = b - A x in FP64 ; restart the sloppy residual
-    if ((rc = field_convert(c, t64, 0, xs, 0)) || (rc = blas_axpby(c, x, 1.0, t64, 1.0, 0))) return rc;
-    if ((rc = true_residual(&rr))) return rc;
-    nupd++;
-    if (!(rr == rr)) { set_error("cg_mixed: non-finite true residual"); return YH_ERR_NONFINITE; }
-    if (rr < target) break;
-    if (rr < best_rr) {
-      best_rr = rr;
-      stalls = 0;
+  const int chunk = 16, depth = 2;
+  struct Item {
+    bool ru;  // reliable update, else n sloppy iterations
+    int n;
+    int slot;
+  };
+  std::deque<Item> inflight;
+  int slot = 0;
+  int seg = 0;          // sloppy iterations planned in the current segment
+  long planned = 0;     // sloppy iterations planned in total
+  bool ru_next = false; // the recurrence stopped early: update next
+  bool plan_done = false;
+  SolverScalars v;
+  memset(&v, 0, sizeof(v));
+  auto enqueue = [&](Item it) -> int {
+    int e;
+    if (it.ru) {
+      if ((e = ru_begin(c, x, xs, p, rs))) return e;
+      if ((e = apply_op(c, PREC_F64, recon64, YH_MVM_HPHI_SQ, par->mass, t64, x, nullptr, -1, &c->sc->finished))) return e;
+      if ((e = ru_end(c, sptr, t64, rs))) return e;
     } else {
-      stalls++;
+      for (int i = 0; i < it.n; i++) {
+        if ((e = apply_op(c, S, recons, YH_MVM_HPHI_SQ, par->mass, q, p, nullptr, 0, &c->sc->converged))) return e;
+        if ((e = cg_update_xr(c, xs, rs, p, q, 0, PREC_F32))) return e;
+        if ((e = cg_update_p(c, p, rs, 0, PREC_F32))) return e;
+      }
     }
-    if ((par->max_iter > 0 && total >= par->max_iter) || stalls >= max_stalls) {
-      *iters = total;
-      if (updates) *updates = nupd;
-      set_error(stalls >= max_stalls ? "cg_mixed: true residual stalled" : "cg_mixed: max_iter reached");
-      return YH_ERR_NOCONV;
+    it.slot = slot;
+    YH_HIP(hipMemcpyAsync(&c->sc_ring[slot], c->sc, sizeof(SolverScalars), hipMemcpyDeviceToHost, c->stream));
+    YH_HIP(hipEventRecord(c->ev_ring[slot], c->stream));
+    inflight.push_back(it);
+    slot = (slot + 1) % Ctx::SC_RING;
+    return YH_OK;
+  };
+  int err = YH_OK;
+  while (true) {
+    while (!plan_done && err == YH_OK && (int)inflight.size() < depth) {
+      Item it = {false, 0, 0};
+      const bool capped = par->max_iter > 0 && planned >= par->max_iter;
+      if (ru_next || seg >= K || capped) {
+        it.ru = true;
+        seg = 0;
+        ru_next = false;
+        if (capped) plan_done = true;  // the last update at the iteration cap
+      } else {
+        it.n = std::min(chunk, K - seg);
+        if (par->max_iter > 0) it.n = (int)std::min<long>(it.n, par->max_iter - planned);
+        seg += it.n;
+        planned += it.n;
+      }
+      if ((rc = enqueue(it))) return rc;
     }
-    if ((rc = field_convert(c, rs, 0, r64, 0)) || (rc = field_zero(c, xs, YH_EVEN))) return rc;
-    h->rr = rr;
-    h->maxrr = rr;
-    h->converged = 0;
-    h->reason = 0;
-    h->iters = total;
-    at_update = total;
-    if ((rc = write_scalars(c))) return rc;
+    if (inflight.empty()) break;
+    const Item it = inflight.front();
+    inflight.pop_front();
+    YH_HIP(hipEventSynchronize(c->ev_ring[it.slot]));
+    v = c->sc_ring[it.slot];
+    if (par->log)
+      fprintf(stderr, "[INVERTER][DEBUG] cg_mixed: %s iters=%d rr=%.6e true_rr=%.6e updates=%d stop=%d reason=%d finished=%d\n",
+              it.ru ? "update" : "sloppy", v.iters, v.rr, v.rr_true, v.nupd, v.converged, v.reason, v.finished);
+    if (err != YH_OK || v.finished) {
+      plan_done = true;  // what is still queued returns at once; drain it
+      continue;
+    }
+    if (!it.ru && !(v.rr == v.rr)) {
+      set_error("cg_mixed: non-finite residual");
+      err = YH_ERR_NONFINITE;
+      plan_done = true;
+      continue;
+    }
+    if (!it.ru && v.converged) {
+      // the recurrence stopped inside this item: the sloppy items queued
+      // after it did nothing, and the update comes next unless queued
+      planned = v.iters;
+      if (inflight.empty() || !inflight.front().ru) ru_next = true;
+    }
   }
-  *iters = total;
-  if (updates) *updates = nupd;
+  YH_HIP(hipStreamSynchronize(c->stream));
+  *iters = v.iters;
+  if (updates) *updates = v.nupd;
+  // the operator applications of the sloppy iterations and of the updates
+  // that ran (their launches carry stop flags and are not counted at launch)
+  c->mvm_counter += (long)(v.iters + v.nupd) * hops_per_apply(YH_MVM_HPHI_SQ);
+  if (err != YH_OK) return err;
+  if (v.finished == 2) { set_error("cg_mixed: non-finite true residual"); return YH_ERR_NONFINITE; }
+  if (v.finished == 3) { set_error("cg_mixed: true residual stalled"); return YH_ERR_NOCONV; }
+  if (v.finished != 1) { set_error("cg_mixed: max_iter reached"); return YH_ERR_NOCONV; }
   if (par->log)
-    fprintf(stderr, "[INVERTER][NOTICE] cg_mixed: iterations = %d, updates = %d, precision = %1.4e\n", total, nupd,
-            rr / norm_b);
-  return sync(c);
+    fprintf(stderr, "[INVERTER][NOTICE] cg_mixed: iterations = %d, updates = %d, precision = %1.4e\n", v.iters, v.nupd,
+            v.rr_true / norm_b);
+  return YH_OK;
 }
 
 }  // extern "C"
