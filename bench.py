@@ -60,7 +60,7 @@ FLOP_PER_SITE = 1320
 # passes (FETCH_SIZE, WRITE_SIZE) scaled by the same counters on the copy
 # kernel; see profiles/hop_f64r18_32x4_traffic.md and hop_f32r12_32x4_traffic.md.
 # None = not measured.
-TRAFFIC_PER_LAUNCH = {"f64_r18": 946758821, "f32_r12": 348221239}
+TRAFFIC_PER_LAUNCH = {"f64_r18": 903698562, "f32_r12": 386920650}
 
 
 def log(msg):

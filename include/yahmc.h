@@ -401,6 +401,12 @@ int yh_synchronize(yh_ctx* ctx);
 /* 1: yh_cg_learned runs flexible GCR (8 kept directions) instead of
  * flexible CG; 0 (default): flexible CG */
 #define YH_TUNE_LEARNED_GCR 7
+/* mask of storage precisions (bit YH_PREC_F64 / _F32 / _F16) whose hop
+ * kernels (D_R <= 3) copy the window of input spinors their y / z
+ * neighbours come from into LDS once per workgroup and read those
+ * neighbours there (when a (t, x) plane holds a multiple of 256 sites);
+ * default 3: FP64 and FP32 */
+#define YH_TUNE_HOP_STAGE 8
 int yh_tune(yh_ctx* ctx, int key, int value);
 /* autotuner (no counterpart in the reference): times the hop of every
  * precision / reconstruction this context holds for each XCD chunk size

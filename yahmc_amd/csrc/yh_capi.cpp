@@ -688,6 +688,11 @@ int yh_tune(yh_ctx* c, int key, int value) {
     c->learned_gcr = value != 0;
     return YH_OK;
   }
+  if (key == YH_TUNE_HOP_STAGE) {
+    if (value < 0 || value > 7) { set_error("hop staging mask: bits 0 (FP64), 1 (FP32), 2 (16-bit)"); return YH_ERR_ARG; }
+    c->hop_stage = value;
+    return YH_OK;
+  }
   if (key == YH_TUNE_MULTI_RHS) {
     c->multi_rhs = value != 0;
     return YH_OK;

@@ -28,6 +28,11 @@
 
 namespace yh {
 
+// threads per workgroup of the hop (one output site each)
+#ifndef YH_HOP_BLOCK
+#define YH_HOP_BLOCK 256
+#endif
+
 template <int MU, bool FWD>
 struct Proj;
 template <> struct Proj<0, true> { static constexpr int PI0 = 0, PI1 = 1, PC0 = 0, PC1 = 0, RI0 = 0, RI1 = 1, RC0 = 0, RC1 = 0; };
@@ -61,15 +66,19 @@ struct HopArgs {
   Geom g;
 };
 
+// one direction of the hop: the neighbour spinor is element `nidx` of `in`
+// (component stride `stride`: the input field, or its copy staged in LDS),
+// the link is site `lidx` of the link field
 template <int MU, bool FWD, int D, int PREC, int RECON, typename R>
-__device__ __forceinline__ void hop_dir(Cx<R> (&acc)[4][D], const HopArgs& a, int nidx, int lidx, bool flip) {
+__device__ __forceinline__ void hop_dir_src(Cx<R> (&acc)[4][D], const HopArgs& a, const void* in, const float* in_nrm,
+                                            int stride, int nidx, int lidx, bool flip) {
   typedef Proj<MU, FWD> P;
   constexpr int LNV = LinkStore<PREC, RECON, D>::NV;
   typedef typename LinkStore<PREC, RECON, D>::vec LV;
   Cx<R> s[4][D];
   R sc = (R)1;
-  if constexpr (PREC == PREC_F16) sc = load_spinor_q<D, R>(s, a.in, a.in_nrm, nidx, a.g.VHpad);
-  else load_spinor<PREC, D, R>(s, a.in, a.in_nrm, nidx, a.g.VHpad);
+  if constexpr (PREC == PREC_F16) sc = load_spinor_q<D, R>(s, in, in_nrm, nidx, stride);
+  else load_spinor<PREC, D, R>(s, in, in_nrm, nidx, stride);
   Cx<R> h[2][D];
 #pragma unroll
   for (int i = 0; i < D; i++) {
@@ -110,6 +119,87 @@ __device__ __forceinline__ void hop_dir(Cx<R> (&acc)[4][D], const HopArgs& a, in
     acc[3][i] = acc[3][i] + r[1][i];
     acc[0][i] = acc[0][i] + unit_mul<P::RC0>(r[P::RI0][i]);
     acc[1][i] = acc[1][i] + unit_mul<P::RC1>(r[P::RI1][i]);
+  }
+}
+
+template <int MU, bool FWD, int D, int PREC, int RECON, typename R>
+__device__ __forceinline__ void hop_dir(Cx<R> (&acc)[4][D], const HopArgs& a, int nidx, int lidx, bool flip) {
+  hop_dir_src<MU, FWD, D, PREC, RECON, R>(acc, a, a.in, a.in_nrm, a.g.VHpad, nidx, lidx, flip);
+}
+
+// LDS staging of the neighbour spinors a workgroup shares.  The 256 output
+// sites of a workgroup are consecutive in the EO order (z fastest) and, when
+// a (t, x) plane holds a multiple of 256 sites, lie in one plane.  The input
+// sites of their y and z neighbours are then (a) within H = L_Z / 2 + 1 of
+// the workgroup's own index range, or (b) across the y wrap: row L_Y - 1 of
+// the plane for the row y = 0 and row 0 for the row y = L_Y - 1.  The
+// workgroup copies the window [k0 - H, k0 + 256 + H) and the two wrap rows of
+// the input field into LDS once (global -> LDS, no registers), and its y and
+// z neighbours (four reads per staged site) come from there; t and x
+// neighbours, which no other site of the workgroup uses, come from the
+// field.  Layout per component plane (stride NT): the window, row L_Y - 1,
+// row 0.
+template <int PREC, int D>
+struct Stage {
+  typedef typename SpinorStore<PREC, D>::vec V;
+  static constexpr int NV = SpinorStore<PREC, D>::NV;
+  static __host__ __device__ int halo(const Geom& g) { return g.L[3] / 2 + 1; }
+  static __host__ __device__ int window(const Geom& g) { return YH_HOP_BLOCK + 2 * halo(g); }
+  static __host__ __device__ int stride(const Geom& g) { return window(g) + g.L[3]; }
+  static __host__ __device__ size_t bytes(const Geom& g) {
+    return (size_t)stride(g) * (NV * sizeof(V) + (PREC == PREC_F16 ? sizeof(float) : 0));
+  }
+  // workgroups never straddle a (t, x) plane, every thread has a site, and
+  // a row fits one 64-site chunk
+  static __host__ bool usable(const Geom& g) {
+    const long plane = (long)g.L[2] * g.L[3] / 2;
+    return g.VH % YH_HOP_BLOCK == 0 && plane % YH_HOP_BLOCK == 0 && g.L[3] <= 128 && bytes(g) <= 64 * 1024;
+  }
+};
+
+// Issue the copy: one 64-site chunk of one component plane per wave
+// instruction, each lane moving one 16-byte element to LDS base + lane * 16.
+// Lanes of sites outside the field load nothing (never read).
+template <int PREC, int D>
+__device__ __forceinline__ void stage_issue(unsigned char* lds_bytes, const void* in, const float* in_nrm, int k0,
+                                            const Geom& g) {
+  typedef Stage<PREC, D> St;
+  typedef typename St::V V;
+  static_assert(sizeof(V) == 16, "spinor component vectors are 16 bytes");
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int NW = St::window(g), NT = St::stride(g), hz = g.L[3] / 2;
+  const int s0 = k0 - St::halo(g);
+  const int plane = g.L[2] * hz;
+  const int p0 = k0 / plane * plane;
+  const int nchw = (NW + 63) / 64;  // window chunks (the last may be partial)
+  const int nch = nchw + 2;          // + the two wrap rows
+  // source site of LDS slot (chunk c, lane); -1: none
+  auto src_site = [&](int c) -> int {
+    if (c < nchw) {
+      const int j = c * 64 + lane;
+      const int site = s0 + j;
+      return (j < NW && site >= 0 && site < g.VH) ? site : -1;
+    }
+    if (lane >= hz) return -1;
+    return c == nchw ? p0 + (g.L[2] - 1) * hz + lane : p0 + lane;
+  };
+  auto slot = [&](int c) -> int { return c < nchw ? c * 64 : NW + (c - nchw) * hz; };
+  for (int cidx = wave; cidx < nch * St::NV; cidx += YH_HOP_BLOCK / 64) {
+    const int v = cidx / nch, c = cidx - v * nch;
+    const int site = src_site(c);
+    __attribute__((address_space(3))) void* dst =
+        (__attribute__((address_space(3))) void*)(lds_bytes + ((size_t)v * NT + slot(c)) * sizeof(V));
+    if (site >= 0)
+      __builtin_amdgcn_global_load_lds((const void*)((const V*)in + (size_t)v * g.VHpad + site), dst, 16, 0, 0);
+  }
+  if constexpr (PREC == PREC_F16) {
+    unsigned char* nb = lds_bytes + (size_t)NT * St::NV * sizeof(V);
+    for (int c = wave; c < nch; c += YH_HOP_BLOCK / 64) {
+      const int site = src_site(c);
+      __attribute__((address_space(3))) void* dst =
+          (__attribute__((address_space(3))) void*)(nb + (size_t)slot(c) * sizeof(float));
+      if (site >= 0) __builtin_amdgcn_global_load_lds((const void*)(in_nrm + site), dst, 4, 0, 0);
+    }
   }
 }
 
@@ -157,16 +247,21 @@ constexpr int hop_min_waves() {
 #ifndef YH_SCHED_WIDE
 #define YH_SCHED_WIDE 63
 #endif
-template <int PREC, int RECON, int D>
+// FP32 hops with LDS-staged y / z neighbours: t x | y | z (their own sweep:
+// 57.0 against 59.2 us unstaged at 32^4, FP32-12)
+#ifndef YH_SCHED_F32_STG
+#define YH_SCHED_F32_STG 6
+#endif
+template <int PREC, int RECON, int D, bool STG = false>
 constexpr int hop_sched() {
   return D >= 4 ? YH_SCHED_WIDE
                 : PREC == PREC_F64 ? (RECON == RECON_18 ? YH_SCHED_F64_18 : YH_SCHED_F64_12)
-                                   : PREC == PREC_F32 ? YH_SCHED_F32 : YH_SCHED_F16;
+                                   : PREC == PREC_F32 ? (STG ? YH_SCHED_F32_STG : YH_SCHED_F32) : YH_SCHED_F16;
 }
 // an opaque zero after every accumulator is final, if bit B of the mask is set
-template <int PREC, int RECON, int B, int D, typename R>
+template <int PREC, int RECON, int B, int D, typename R, bool STG = false>
 __device__ __forceinline__ int hop_group_dep(Cx<R> (&acc)[4][D]) {
-  constexpr int s = hop_sched<PREC, RECON, D>();
+  constexpr int s = hop_sched<PREC, RECON, D, STG>();
   if constexpr ((s >> B) & 1) {
     // every accumulator passes through an (empty) volatile asm, so the whole
     // group's arithmetic completes here and its loaded operands die
@@ -185,12 +280,7 @@ __device__ __forceinline__ int hop_group_dep(Cx<R> (&acc)[4][D]) {
   }
 }
 
-// threads per workgroup of the hop (one output site each)
-#ifndef YH_HOP_BLOCK
-#define YH_HOP_BLOCK 256
-#endif
-
-template <int D, int PREC, int RECON, int EPI>
+template <int D, int PREC, int RECON, int EPI, bool STG = false>
 __global__ void __launch_bounds__(YH_HOP_BLOCK, hop_min_waves<PREC>()) k_hop(HopArgs a) {
   typedef typename SpinorStore<PREC, D>::real R;
   if (a.skip && *a.skip) return;
@@ -199,6 +289,16 @@ __global__ void __launch_bounds__(YH_HOP_BLOCK, hop_min_waves<PREC>()) k_hop(Hop
   const int k = blk * YH_HOP_BLOCK + threadIdx.x;
   const Geom& g = a.g;
   double nrm2 = 0.0;
+  // staged input (STG): per component plane NT slots, the window from s0
+  // then the wrap rows L_Y - 1 and 0 of the workgroup's plane
+  extern __shared__ __align__(16) unsigned char hop_lds[];
+  typedef Stage<PREC, D> St;
+  const int NT = STG ? St::stride(g) : 0;
+  const int s0 = blk * YH_HOP_BLOCK - (STG ? St::halo(g) : 0);
+  const typename St::V* lds = (const typename St::V*)hop_lds;
+  const float* lds_nrm = (const float*)(hop_lds + (size_t)NT * St::NV * sizeof(typename St::V));
+  // the copy is issued now and waited for after the t and x directions
+  if constexpr (STG) stage_issue<PREC, D>(hop_lds, a.in, a.in_nrm, blk * YH_HOP_BLOCK, g);
   if (k < g.VH) {
     int c[4];
     site_coords(k, a.parity, g, c);
@@ -216,38 +316,61 @@ __global__ void __launch_bounds__(YH_HOP_BLOCK, hop_min_waves<PREC>()) k_hop(Hop
       int tb = (t == 0) ? T - 1 : t - 1;
       int nf = half_index(tf, x, y, z, g), nb = half_index(tb, x, y, z, g);
       hop_dir<0, true, D, PREC, RECON, R>(acc, a, nf, k, ap && t == 0);
-      const int h = hop_group_dep<PREC, RECON, 3>(acc);
+      const int h = hop_group_dep<PREC, RECON, 3, D, R, STG>(acc);
       hop_dir<0, false, D, PREC, RECON, R>(acc, a, nb + h, nb + h, ap && tb == 0);
     }
-    const int dep1 = hop_group_dep<PREC, RECON, 0>(acc);
+    const int dep1 = hop_group_dep<PREC, RECON, 0, D, R, STG>(acc);
     {
       int xf = (x + 1 == g.L[1]) ? 0 : x + 1;
       int xb = (x == 0) ? g.L[1] - 1 : x - 1;
       int nf = half_index(t, xf, y, z, g) + dep1, nb = half_index(t, xb, y, z, g) + dep1;
       hop_dir<1, true, D, PREC, RECON, R>(acc, a, nf, k + dep1, false);
-      const int h = hop_group_dep<PREC, RECON, 3>(acc);
+      const int h = hop_group_dep<PREC, RECON, 3, D, R, STG>(acc);
       hop_dir<1, false, D, PREC, RECON, R>(acc, a, nb + h, nb + h, false);
     }
-    const int dep2 = hop_group_dep<PREC, RECON, 1>(acc);
+    const int dep2 = hop_group_dep<PREC, RECON, 1, D, R, STG>(acc);
+    if constexpr (STG) {
+      // every lane's copy has landed; then every wave's (the staged
+      // kernel runs only when every thread has a site, VH a multiple of
+      // the workgroup, so the whole workgroup reaches this barrier)
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+    }
     {
       int yf = (y + 1 == g.L[2]) ? 0 : y + 1;
       int yb = (y == 0) ? g.L[2] - 1 : y - 1;
       int nf = half_index(t, x, yf, z, g) + dep2, nb = half_index(t, x, yb, z, g) + dep2;
-      hop_dir<2, true, D, PREC, RECON, R>(acc, a, nf, k + dep2, false);
-      const int h = hop_group_dep<PREC, RECON, 3>(acc);
-      hop_dir<2, false, D, PREC, RECON, R>(acc, a, nb + h, nb + h, false);
+      if constexpr (STG) {
+        // slots: the window, or the wrap rows (same z offset in the row)
+        const int hz = g.L[3] / 2, NW = St::window(g), zo = k % hz;
+        const int jf = (yf == 0) ? NW + hz + zo + dep2 : nf - s0;
+        const int jb = (yb == g.L[2] - 1 && y == 0) ? NW + zo : nb - s0;
+        hop_dir_src<2, true, D, PREC, RECON, R>(acc, a, lds, lds_nrm, NT, jf, k + dep2, false);
+        const int h = hop_group_dep<PREC, RECON, 3, D, R, STG>(acc);
+        hop_dir_src<2, false, D, PREC, RECON, R>(acc, a, lds, lds_nrm, NT, jb + h, nb + h, false);
+      } else {
+        hop_dir<2, true, D, PREC, RECON, R>(acc, a, nf, k + dep2, false);
+        const int h = hop_group_dep<PREC, RECON, 3, D, R, STG>(acc);
+        hop_dir<2, false, D, PREC, RECON, R>(acc, a, nb + h, nb + h, false);
+      }
     }
-    const int dep3 = hop_group_dep<PREC, RECON, 2>(acc);
+    const int dep3 = hop_group_dep<PREC, RECON, 2, D, R, STG>(acc);
     {
       int zf = (z + 1 == g.L[3]) ? 0 : z + 1;
       int zb = (z == 0) ? g.L[3] - 1 : z - 1;
       int nf = half_index(t, x, y, zf, g) + dep3, nb = half_index(t, x, y, zb, g) + dep3;
-      hop_dir<3, true, D, PREC, RECON, R>(acc, a, nf, k + dep3, false);
-      const int h = hop_group_dep<PREC, RECON, 3>(acc);
-      hop_dir<3, false, D, PREC, RECON, R>(acc, a, nb + h, nb + h, false);
+      if constexpr (STG) {
+        hop_dir_src<3, true, D, PREC, RECON, R>(acc, a, lds, lds_nrm, NT, nf - s0, k + dep3, false);
+        const int h = hop_group_dep<PREC, RECON, 3, D, R, STG>(acc);
+        hop_dir_src<3, false, D, PREC, RECON, R>(acc, a, lds, lds_nrm, NT, nb - s0 + h, nb + h, false);
+      } else {
+        hop_dir<3, true, D, PREC, RECON, R>(acc, a, nf, k + dep3, false);
+        const int h = hop_group_dep<PREC, RECON, 3, D, R, STG>(acc);
+        hop_dir<3, false, D, PREC, RECON, R>(acc, a, nb + h, nb + h, false);
+      }
     }
     // epilogue loads (clover block, own spinor) after the hop is final
-    const int ke = k + hop_group_dep<PREC, RECON, 4>(acc);
+    const int ke = k + hop_group_dep<PREC, RECON, 4, D, R, STG>(acc);
     const bool bz = bc_zero_slice(t, g);
     const R sc = bz ? (R)0 : (R)a.scale;
 #pragma unroll
@@ -272,7 +395,7 @@ __global__ void __launch_bounds__(YH_HOP_BLOCK, hop_min_waves<PREC>()) k_hop(Hop
         herm_mv<N, R>(yout, cl, xin);
 #pragma unroll
         for (int i = 0; i < D; i++) { o[2 * blk2][i] = yout[i]; o[2 * blk2 + 1][i] = yout[D + i]; }
-        kb = ke + hop_group_dep<PREC, RECON, 5>(o);
+        kb = ke + hop_group_dep<PREC, RECON, 5, D, R, STG>(o);
       }
       store_spinor<PREC, D, R, PREC == PREC_F64>(a.out, a.out_nrm, k, g.VHpad, o);
     } else {
@@ -298,7 +421,7 @@ __global__ void __launch_bounds__(YH_HOP_BLOCK, hop_min_waves<PREC>()) k_hop(Hop
           acc[2 * blk2][i] = acc[2 * blk2][i] + yout[i] + dg * xin[i];
           acc[2 * blk2 + 1][i] = acc[2 * blk2 + 1][i] + yout[D + i] + dg * xin[D + i];
         }
-        kb = ke + hop_group_dep<PREC, RECON, 5>(acc);
+        kb = ke + hop_group_dep<PREC, RECON, 5, D, R, STG>(acc);
       }
       if (a.gamma5) {
 #pragma unroll
@@ -558,6 +681,24 @@ int hop_mr_dispatch_d(Ctx* c, const HopArgsM& m, int prec, int nr, int epi) {
 template <int D, int PREC, int RECON>
 static int hop_dispatch_epi(Ctx* c, const HopArgs& a, int epi, hipStream_t st) {
   dim3 grid((c->g.VH + YH_HOP_BLOCK - 1) / YH_HOP_BLOCK), block(YH_HOP_BLOCK);
+  if constexpr (D <= 3) {
+    // LDS-staged y / z neighbours (yh_tune YH_TUNE_HOP_STAGE) where the
+    // geometry allows it (Stage::usable; the kernel has a barrier)
+    const size_t lb = Stage<PREC, D>::bytes(c->g);
+    // FP64 clover epilogues run one wave per SIMD (256 VGPRs) and measured
+    // slower staged (H^2 at 32^4: 880 against 859 us); only the plain FP64
+    // hop is staged (134 against 141 us)
+    const bool stage_epi = PREC != PREC_F64 || epi == EPI_PLAIN;
+    if (((c->hop_stage >> PREC) & 1) && stage_epi && Stage<PREC, D>::usable(c->g)) {
+      switch (epi) {
+        case EPI_PLAIN: hipLaunchKernelGGL((k_hop<D, PREC, RECON, EPI_PLAIN, true>), grid, block, lb, st, a); break;
+        case EPI_CLOVINV: hipLaunchKernelGGL((k_hop<D, PREC, RECON, EPI_CLOVINV, true>), grid, block, lb, st, a); break;
+        default: hipLaunchKernelGGL((k_hop<D, PREC, RECON, EPI_CLOV, true>), grid, block, lb, st, a); break;
+      }
+      YH_HIP(hipGetLastError());
+      return YH_OK;
+    }
+  }
   switch (epi) {
     case EPI_PLAIN: hipLaunchKernelGGL((k_hop<D, PREC, RECON, EPI_PLAIN>), grid, block, 0, st, a); break;
     case EPI_CLOVINV: hipLaunchKernelGGL((k_hop<D, PREC, RECON, EPI_CLOVINV>), grid, block, 0, st, a); break;

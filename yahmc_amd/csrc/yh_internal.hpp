@@ -98,6 +98,9 @@ struct Ctx {
   int recon12_f64 = 0;  // FP64 operators of SU(3) fundamental read 12-real links
   int multi_rhs = 1;    // batched multishift: one hop launch per batch of right-hand sides
   int learned_gcr = 0;  // the learned solve uses flexible GCR instead of flexible CG
+  // hop kernels (D_R <= 3) of the precisions in this mask (bit = precision)
+  // read y / z neighbours from an LDS-staged window
+  int hop_stage = (1 << PREC_F64) | (1 << PREC_F32);
   // solver chunks are captured once into a HIP graph and replayed
   int use_graphs = 1;
   // model tables of this context's (nc, representation), in device memory

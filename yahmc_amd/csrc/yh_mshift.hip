@@ -2,14 +2,16 @@
 // right-hand sides (library/cg.cpp:11-130).
 //
 // Per iteration and right-hand side: H^2 p (four fused hop kernels, the last
-// one also reducing <p, H^2 p> = |H p|^2), then two fused update kernels
-// that touch every active shift in one pass over the lattice:
-//   k_ms_x : beta and the zeta coefficients c3_s of every active shift are
-//            recomputed identically by every thread from the stored state;
-//            x_s -= beta c3_s / c2_s k_s ; r += beta q ; |r|^2 -> alpha,
-//            convergence flags (last workgroup)
-//   k_ms_k : k_s = alpha c3_s^2 / c2_s^2 k_s + c3_s r for the shifts that
-//            were active in this iteration ; p = alpha p + r
+// one also reducing <p, H^2 p> = |H p|^2), then two fused update kernels:
+//   k_ms_x : r += beta q ; |r|^2 -> alpha; the last workgroup computes the
+//            zeta coefficients c3_s, the solution step bt_s = beta c3_s / c2_s
+//            and the direction factor of every active shift and the
+//            convergence flags
+//   k_ms_k : for the shifts that were active in this iteration, in one pass
+//            over x_s and k_s:  x_s -= bt_s k_s ;  k_s = at_s k_s + c3_s r
+//            (the solution step uses k_s before its update, as the
+//            reference's order x_s -= bt k_s, then k_s = ... has it) ;
+//            p = alpha p + r
 // The host only queues chunks of iterations and reads one flag per chunk.
 // Right-hand sides are independent recurrences sharing the gauge field; they
 // are interleaved iteration by iteration so consecutive hop launches reuse
@@ -26,7 +28,7 @@ enum { MS_MAX = 48 };
 struct MsState {
   double rr, pap, beta, beta_old, alpha, norm, target;
   int done, iters, nshift, notconv;
-  double shift[MS_MAX], c1[MS_MAX], c2[MS_MAX], c3[MS_MAX], at[MS_MAX];
+  double shift[MS_MAX], c1[MS_MAX], c2[MS_MAX], c3[MS_MAX], at[MS_MAX], bt[MS_MAX];
   int act[MS_MAX], upd[MS_MAX];
 };
 
@@ -38,6 +40,7 @@ struct MsFields {
 template <int D>
 __global__ void __launch_bounds__(256) k_ms_x(MsState* st, const MsFields* __restrict__ mf, void* r, const void* q,
                                               double* partials, unsigned* ticket, Geom g) {
+  (void)mf;
   if (st->done) return;
   const double atmp = st->rr, btmp = st->pap;
   const double beta_old = st->beta;
@@ -49,20 +52,6 @@ __global__ void __launch_bounds__(256) k_ms_x(MsState* st, const MsFields* __res
   if (k < g.VH) {
     Cx<double> R[4][D], Q[4][D];
     load_spinor<PREC_F64, D, double>(R, r, nullptr, k, g.VHpad);
-    for (int s = 0; s < ns; s++) {
-      if (!st->act[s]) continue;
-      double c1 = st->c1[s], c2 = st->c2[s];
-      double c3 = c2 * c1 * beta_old / (beta * alpha * (c1 - c2) + c1 * beta_old * (1 - st->shift[s] * beta));
-      double bt = beta * (c3 / c2);
-      Cx<double> X[4][D], K[4][D];
-      load_spinor<PREC_F64, D, double>(X, mf->x[s], nullptr, k, g.VHpad);
-      load_spinor<PREC_F64, D, double>(K, mf->k[s], nullptr, k, g.VHpad);
-#pragma unroll
-      for (int a = 0; a < 4; a++)
-#pragma unroll
-        for (int i = 0; i < D; i++) X[a][i] = Cx<double>(X[a][i].re - bt * K[a][i].re, X[a][i].im - bt * K[a][i].im);
-      store_spinor<PREC_F64, D, double>(mf->x[s], nullptr, k, g.VHpad, X);
-    }
     load_spinor<PREC_F64, D, double>(Q, q, nullptr, k, g.VHpad);
 #pragma unroll
     for (int a = 0; a < 4; a++)
@@ -83,6 +72,7 @@ __global__ void __launch_bounds__(256) k_ms_x(MsState* st, const MsFields* __res
       if (!st->act[s]) continue;
       double c1 = st->c1[s], c2 = st->c2[s];
       double c3 = c2 * c1 * beta_old / (beta * alpha * (c1 - c2) + c1 * beta_old * (1 - st->shift[s] * beta));
+      st->bt[s] = beta * (c3 / c2);
       st->c3[s] = c3;
       st->at[s] = alpha_new * (c3 * c3) / (c2 * c2);
       if (rr_new * c3 * c3 < st->target) st->act[s] = 0;
@@ -113,13 +103,18 @@ __global__ void __launch_bounds__(256) k_ms_k(MsState* st, const MsFields* __res
     load_spinor<PREC_F64, D, double>(R, r, nullptr, k, g.VHpad);
     for (int s = 0; s < ns; s++) {
       if (!st->upd[s]) continue;
-      const double at = st->at[s], c3 = st->c3[s];
-      Cx<double> K[4][D];
+      const double at = st->at[s], c3 = st->c3[s], bt = st->bt[s];
+      Cx<double> K[4][D], X[4][D];
       load_spinor<PREC_F64, D, double>(K, mf->k[s], nullptr, k, g.VHpad);
+      load_spinor<PREC_F64, D, double>(X, mf->x[s], nullptr, k, g.VHpad);
 #pragma unroll
       for (int a = 0; a < 4; a++)
 #pragma unroll
-        for (int i = 0; i < D; i++) K[a][i] = Cx<double>(at * K[a][i].re + c3 * R[a][i].re, at * K[a][i].im + c3 * R[a][i].im);
+        for (int i = 0; i < D; i++) {
+          X[a][i] = Cx<double>(X[a][i].re - bt * K[a][i].re, X[a][i].im - bt * K[a][i].im);
+          K[a][i] = Cx<double>(at * K[a][i].re + c3 * R[a][i].re, at * K[a][i].im + c3 * R[a][i].im);
+        }
+      store_spinor<PREC_F64, D, double>(mf->x[s], nullptr, k, g.VHpad, X);
       store_spinor<PREC_F64, D, double>(mf->k[s], nullptr, k, g.VHpad, K);
     }
     if (move_p) {
