@@ -244,31 +244,49 @@ SLOPPY_NAME = {"f32": "FP32", "f16": "16-bit fixed point"}
 
 
 def hmc_trajectory(yh, sloppy="f16"):
-    """One HMC trajectory on 24^3 x 32 with two flavours of sextet (two-index
+    """HMC trajectories on 24^3 x 32 with two flavours of sextet (two-index
     symmetric SU(3)) clover fermions: tau = 1, o2mn with the Wilson gauge
-    monomial on level 0 (1 step) and the fermion monomial on level 1
-    (10 steps), mixed-precision CG (16-bit fixed-point sloppy operator by
-    default) to an FP64 true residual of 1e-12, FP64
-    link update and Hamiltonian.  Seeded random SU(3) links (hot start);
-    beta = 5.6 and m = -0.75 (values the workload leaves open)."""
+    monomial on level 0 (1 step) and the fermion monomial on level 1,
+    mixed-precision CG (16-bit fixed-point sloppy operator by default) to an
+    FP64 true residual of 1e-12, FP64 link update and Hamiltonian; beta = 5.6
+    and m = -0.75 (values the workload leaves open).  The seeded random SU(3)
+    field is first smoothed as the 32^4 field is (cold start, 10 pure-gauge
+    trajectories at beta = 5.6).  The timed trajectory has 10 fermion steps;
+    the same trajectory (same momenta and pseudofermions) is then repeated
+    from the same field with 20 and 40 steps, and o2mn's dH ~ dt^2 makes the
+    ratios of dH at 10 and 20 and at 20 and 40 steps approach 4."""
     from yahmc_amd.hmc import HMC, FermionMonomial, GaugeMonomial
-    from yahmc_amd.synthetic import random_gauge
+    from yahmc_amd.md import smooth
     dims = (32, 24, 24, 24)
     lat = yh.Lattice(dims, nc=3, repr="symmetric", bc="antiperiodic", csw=1.0)
-    lat.set_gauge(random_gauge(dims, 3, seed=32244))
-    plaq0 = lat.plaquette()
-    fm = FermionMonomial(-0.75, prec=1e-24, level=1, sloppy=sloppy)
-    hmc = HMC(lat, [GaugeMonomial(5.6), fm], [("o2mn", 1), ("o2mn", 10)])
-    lat.synchronize()
     t0 = time.perf_counter()
-    rec = hmc.trajectory(np.random.default_rng(32247), length=1.0, metropolis=True)
-    lat.synchronize()
-    dt = time.perf_counter() - t0
+    smooth(lat, beta=5.6, trajectories=10, length=1.0, steps=10, seed=32246)
+    lat.represent_gauge_field()
+    smooth_s = time.perf_counter() - t0
+    U = lat.get_gauge()
+    plaq0 = lat.plaquette()
+    runs = {}
+    for nsteps in (10, 20, 40):
+        lat.set_gauge(U)
+        fm = FermionMonomial(-0.75, prec=1e-24, level=1, sloppy=sloppy)
+        hmc = HMC(lat, [GaugeMonomial(5.6), fm], [("o2mn", 1), ("o2mn", nsteps)])
+        lat.synchronize()
+        t0 = time.perf_counter()
+        rec = hmc.trajectory(np.random.default_rng(32247), length=1.0, metropolis=(nsteps == 10))
+        lat.synchronize()
+        runs[nsteps] = {"seconds": round(time.perf_counter() - t0, 3), "dH": rec["dH"], "accepted": rec["accepted"],
+                        "plaquette_after": round(rec["plaquette"], 6), "cg_solves": len(fm.iterations),
+                        "cg_iterations_total": int(sum(fm.iterations))}
     out = {"lattice": list(dims), "repr": "sextet", "beta": 5.6, "mass": -0.75, "csw": 1.0,
+           "start": {"smoothing": "cold start + 10 pure-gauge trajectories, beta 5.6, 10 o2mn steps",
+                     "seconds": round(smooth_s, 2), "plaquette": round(plaq0, 6)},
            "integrator": "o2mn, gauge level 0 x 1 step, fermion level 1 x 10 steps", "tau": 1.0,
-           "seconds": round(dt, 3), "dH": rec["dH"], "accepted": rec["accepted"],
-           "plaquette_before": round(plaq0, 6), "plaquette_after": round(rec["plaquette"], 6),
-           "cg_solves": len(fm.iterations), "cg_iterations_total": int(sum(fm.iterations)),
+           "seconds": runs[10]["seconds"], "dH": runs[10]["dH"], "accepted": runs[10]["accepted"],
+           "plaquette_after": runs[10]["plaquette_after"], "cg_solves": runs[10]["cg_solves"],
+           "cg_iterations_total": runs[10]["cg_iterations_total"],
+           "fermion_steps_20": runs[20], "fermion_steps_40": runs[40],
+           "dH_ratio_10_over_20": round(runs[10]["dH"] / runs[20]["dH"], 3) if runs[20]["dH"] != 0 else None,
+           "dH_ratio_20_over_40": round(runs[20]["dH"] / runs[40]["dH"], 3) if runs[40]["dH"] != 0 else None,
            "cg": f"mixed reliable-update CG, sloppy {SLOPPY_NAME[sloppy]}, FP64 true residual 1e-12"}
     log(f"hmc trajectory: {out}")
     lat.close()
