@@ -43,5 +43,5 @@ def test_bad_lattice_rejected_before_any_device_call():
         yahmc_amd.Lattice((3, 4, 4, 4))
     assert e.value.code == -1
     with pytest.raises(yahmc_amd.YahmcError) as e:
-        yahmc_amd.Lattice((4, 4, 4, 4), nc=3, repr="adjoint")  # D = 8 is not built
+        yahmc_amd.Lattice((4, 4, 4, 4), nc=4, repr="adjoint")  # D = 15 is not built
     assert e.value.code == -6

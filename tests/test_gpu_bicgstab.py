@@ -16,7 +16,8 @@ from oracle import BC_ANTIPERIODIC, BC_OPEN, Oracle
 pytestmark = pytest.mark.gpu
 yh = pytest.importorskip("yahmc_amd")
 
-MODELS = {"su3_fund": (3, "fundamental"), "su2_adj": (2, "adjoint"), "su3_sym": (3, "symmetric")}
+MODELS = {"su3_fund": (3, "fundamental"), "su2_adj": (2, "adjoint"), "su3_sym": (3, "symmetric"),
+          "su3_adj": (3, "adjoint")}
 MVMS = {"Dphi": 0, "Hphi": 1, "Hphi_sq": 2}
 
 

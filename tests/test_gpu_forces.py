@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 yh = pytest.importorskip("yahmc_amd")
 
 MODELS = {"su3_fund": (3, "fundamental"), "su2_adj": (2, "adjoint"), "su3_sym": (3, "symmetric"),
-          "su2_fund": (2, "fundamental")}
+          "su2_fund": (2, "fundamental"), "su3_adj": (3, "adjoint")}
 
 
 def rel(a, b):
@@ -25,7 +25,7 @@ def rel(a, b):
 
 @pytest.mark.parametrize("model,csw,bc", [("su3_fund", 1.0, BC_ANTIPERIODIC), ("su3_fund", 0.0, BC_PERIODIC),
                                           ("su2_adj", 1.2, BC_ANTIPERIODIC), ("su3_sym", 1.0, BC_ANTIPERIODIC),
-                                          ("su2_fund", 1.0, BC_PERIODIC)])
+                                          ("su2_fund", 1.0, BC_PERIODIC), ("su3_adj", 1.0, BC_ANTIPERIODIC)])
 def test_hmc_force_sequence(model, csw, bc):
     print(f"\n[case] forces model={model} csw={csw} bc={bc}", flush=True)
     nc, rep = MODELS[model]

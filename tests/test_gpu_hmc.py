@@ -80,7 +80,8 @@ def oracle_trajectory(o, links, P, eta, beta, mass, prec, length, nf, ng):
     return h1 - h0, o.plaquette(), o.get_gauge(), o.get_momenta()
 
 
-@pytest.mark.parametrize("model,nc,rep", [("su3_fund", 3, "fundamental"), ("su3_sym", 3, "symmetric")])
+@pytest.mark.parametrize("model,nc,rep", [("su3_fund", 3, "fundamental"), ("su3_sym", 3, "symmetric"),
+                                          ("su3_adj", 3, "adjoint")])
 def test_trajectory_matches_oracle(model, nc, rep):
     print(f"\n[case] hmc trajectory {model}", flush=True)
     dims = (4, 4, 4, 8)

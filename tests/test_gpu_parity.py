@@ -17,7 +17,8 @@ pytestmark = pytest.mark.gpu
 yh = pytest.importorskip("yahmc_amd")
 
 MODELS = {"su3_fund": (3, "fundamental"), "su2_fund": (2, "fundamental"), "su2_adj": (2, "adjoint"),
-          "su3_sym": (3, "symmetric"), "su3_asym": (3, "antisymmetric"), "su4_asym": (4, "antisymmetric")}
+          "su3_sym": (3, "symmetric"), "su3_asym": (3, "antisymmetric"), "su4_asym": (4, "antisymmetric"),
+          "su3_adj": (3, "adjoint")}
 BCN = {BC_PERIODIC: "periodic", BC_ANTIPERIODIC: "antiperiodic", BC_OPEN: "open", BC_SF: "sf"}
 
 
@@ -94,7 +95,7 @@ def test_clover_apply_and_inverse(bc):
 @pytest.mark.parametrize("model,bc", [("su3_fund", BC_ANTIPERIODIC), ("su3_fund", BC_PERIODIC), ("su3_fund", BC_OPEN),
                                       ("su3_fund", BC_SF), ("su2_fund", BC_ANTIPERIODIC), ("su2_adj", BC_ANTIPERIODIC),
                                       ("su3_sym", BC_ANTIPERIODIC), ("su3_asym", BC_OPEN),
-                                      ("su4_asym", BC_ANTIPERIODIC)])
+                                      ("su4_asym", BC_ANTIPERIODIC), ("su3_adj", BC_ANTIPERIODIC), ("su3_adj", BC_OPEN)])
 def test_schur_operators(model, bc):
     o, lat, rng = setup(model, dims=(4, 4, 4, 4), bc=bc, csw=1.0, cf=1.1)
     mass = -0.75

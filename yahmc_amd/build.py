@@ -17,7 +17,7 @@ OBJDIR = os.path.join(HERE, "lib", "obj")
 LIBNAME = "libyahmc.so"
 ARCH = os.environ.get("YAHMC_ARCH", "gfx950")
 
-SOURCES = ["yh_dslash_d3.hip", "yh_dslash_d6.hip", "yh_gauge.hip", "yh_dslash_d2.hip",
+SOURCES = ["yh_dslash_d8.hip", "yh_dslash_d3.hip", "yh_dslash_d6.hip", "yh_gauge.hip", "yh_dslash_d2.hip",
            "yh_dslash_d4.hip", "yh_dslash.hip", "yh_blas.hip", "yh_fields.hip", "yh_capi.cpp", "yh_solvers.cpp",
            "yh_force.hip", "yh_forces_api.cpp", "yh_mshift.hip", "yh_rational.cpp", "yh_md.hip", "yh_net.hip", "yh_io.cpp", "yh_meson.hip", "yh_bicgstab.hip", "yh_noalloc.cpp"]
 

@@ -187,6 +187,7 @@ __global__ void __launch_bounds__(256) k_bi_p(void* p, const void* r, const void
     case 3: { constexpr int D_ = 3; CALL; } break;          \
     case 4: { constexpr int D_ = 4; CALL; } break;          \
     case 6: { constexpr int D_ = 6; CALL; } break;          \
+    case 8: { constexpr int D_ = 8; CALL; } break;          \
     default: set_error("unsupported representation dimension"); return YH_ERR_UNSUPPORTED; \
   }
 #define YH_BI_P(PREC, CALL)                                 \

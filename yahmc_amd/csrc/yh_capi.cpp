@@ -353,7 +353,7 @@ int yh_create(const yh_lattice* lat, yh_ctx** out) {
   for (int m = 0; m < 4; m++)
     if (lat->dims[m] < 2 || (lat->dims[m] & 1)) { set_error("every lattice extent must be even and >= 2"); return YH_ERR_ARG; }
   int dr = repr_dim(lat->nc, lat->repr);
-  if (dr < 2 || !(dr == 2 || dr == 3 || dr == 4 || dr == 6)) {
+  if (dr < 2 || !(dr == 2 || dr == 3 || dr == 4 || dr == 6 || dr == 8)) {
     set_error("unsupported (nc, representation)");
     return YH_ERR_UNSUPPORTED;
   }

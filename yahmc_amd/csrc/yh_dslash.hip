@@ -18,6 +18,8 @@ extern template int clover_apply_d<2>(Ctx*, int, int, double, Field*, const Fiel
 extern template int clover_apply_d<3>(Ctx*, int, int, double, Field*, const Field*, int);
 extern template int clover_apply_d<4>(Ctx*, int, int, double, Field*, const Field*, int);
 extern template int clover_apply_d<6>(Ctx*, int, int, double, Field*, const Field*, int);
+extern template int hop_dispatch_d<8>(Ctx*, const HopArgs&, int, int, int);
+extern template int clover_apply_d<8>(Ctx*, int, int, double, Field*, const Field*, int);
 template <int D>
 int hop_mr_dispatch_d(Ctx* c, const HopArgsM& m, int prec, int nr, int epi);
 extern template int hop_mr_dispatch_d<2>(Ctx*, const HopArgsM&, int, int, int);
@@ -97,6 +99,7 @@ int hop_launch(Ctx* c, const HopCall& h) {
     case 3: return hop_dispatch_d<3>(c, a, h.prec, h.recon, h.epi);
     case 4: return hop_dispatch_d<4>(c, a, h.prec, h.recon, h.epi);
     case 6: return hop_dispatch_d<6>(c, a, h.prec, h.recon, h.epi);
+    case 8: return hop_dispatch_d<8>(c, a, h.prec, h.recon, h.epi);
   }
   set_error("unsupported representation dimension");
   return YH_ERR_UNSUPPORTED;
@@ -174,6 +177,7 @@ int clover_apply(Ctx* c, int prec, int parity, double diag, Field* out, const Fi
     case 3: return clover_apply_d<3>(c, prec, parity, diag, out, in, inverse);
     case 4: return clover_apply_d<4>(c, prec, parity, diag, out, in, inverse);
     case 6: return clover_apply_d<6>(c, prec, parity, diag, out, in, inverse);
+    case 8: return clover_apply_d<8>(c, prec, parity, diag, out, in, inverse);
   }
   set_error("unsupported representation dimension");
   return YH_ERR_UNSUPPORTED;

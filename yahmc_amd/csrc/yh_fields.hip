@@ -44,6 +44,7 @@ __global__ void __launch_bounds__(256) k_to_aos(const void* in, const float* in_
     case 3: { constexpr int D_ = 3; CALL; } break;                      \
     case 4: { constexpr int D_ = 4; CALL; } break;                      \
     case 6: { constexpr int D_ = 6; CALL; } break;                      \
+    case 8: { constexpr int D_ = 8; CALL; } break;                      \
     default: set_error("unsupported representation dimension"); return YH_ERR_UNSUPPORTED; \
   }
 

@@ -783,6 +783,7 @@ int force_alloc(Ctx* c) {
   else if (c->nc == 2 && c->dr == 2) { constexpr int NG_ = 3, D_ = 2; __VA_ARGS__; }     \
   else if (c->nc == 2 && c->dr == 3) { constexpr int NG_ = 3, D_ = 3; __VA_ARGS__; }     \
   else if (c->nc == 3 && c->dr == 6) { constexpr int NG_ = 8, D_ = 6; __VA_ARGS__; }     \
+  else if (c->nc == 3 && c->dr == 8) { constexpr int NG_ = 8, D_ = 8; __VA_ARGS__; }     \
   else if (c->nc == 4 && c->dr == 4) { constexpr int NG_ = 15, D_ = 4; __VA_ARGS__; }    \
   else if (c->nc == 4 && c->dr == 6) { constexpr int NG_ = 15, D_ = 6; __VA_ARGS__; }    \
   else { set_error("forces: unsupported (nc, representation)"); return YH_ERR_UNSUPPORTED; }
@@ -812,6 +813,7 @@ int force_clover_site(Ctx* c, double residue, const Field* lhs, const Field* rhs
     case 3: hipLaunchKernelGGL(k_force_clover_site<3>, grid, dim3(256), 0, c->stream, f, (double2*)c->clf, residue, c->g); break;
     case 4: hipLaunchKernelGGL(k_force_clover_site<4>, grid, dim3(256), 0, c->stream, f, (double2*)c->clf, residue, c->g); break;
     case 6: hipLaunchKernelGGL(k_force_clover_site<6>, grid, dim3(256), 0, c->stream, f, (double2*)c->clf, residue, c->g); break;
+    case 8: hipLaunchKernelGGL(k_force_clover_site<8>, grid, dim3(256), 0, c->stream, f, (double2*)c->clf, residue, c->g); break;
     default: set_error("unsupported representation dimension"); return YH_ERR_UNSUPPORTED;
   }
   YH_HIP(hipGetLastError());
@@ -829,6 +831,7 @@ int force_logdet(Ctx* c, double mass, double coeff) {
     case 3: hipLaunchKernelGGL(k_force_logdet<3>, grid, dim3(128), 0, c->stream, inv, (double2*)c->clf, coeff, c->g); break;
     case 4: hipLaunchKernelGGL(k_force_logdet<4>, grid, dim3(128), 0, c->stream, inv, (double2*)c->clf, coeff, c->g); break;
     case 6: hipLaunchKernelGGL(k_force_logdet<6>, grid, dim3(128), 0, c->stream, inv, (double2*)c->clf, coeff, c->g); break;
+    case 8: hipLaunchKernelGGL(k_force_logdet<8>, grid, dim3(128), 0, c->stream, inv, (double2*)c->clf, coeff, c->g); break;
     default: set_error("unsupported representation dimension"); return YH_ERR_UNSUPPORTED;
   }
   YH_HIP(hipGetLastError());

@@ -533,7 +533,8 @@ __global__ void k_clover_to_q16(const double2* __restrict__ src, uint2* __restri
   YH_DISPATCH_MODEL(3, YH_REPR_SYMMETRIC, 6, __VA_ARGS__)      \
   YH_DISPATCH_MODEL(2, YH_REPR_SYMMETRIC, 3, __VA_ARGS__)      \
   YH_DISPATCH_MODEL(3, YH_REPR_ANTISYMMETRIC, 3, __VA_ARGS__)  \
-  YH_DISPATCH_MODEL(4, YH_REPR_ANTISYMMETRIC, 6, __VA_ARGS__)
+  YH_DISPATCH_MODEL(4, YH_REPR_ANTISYMMETRIC, 6, __VA_ARGS__)  \
+  YH_DISPATCH_MODEL(3, YH_REPR_ADJOINT, 8, __VA_ARGS__)
 
 static int grid1(long n) { return (int)((n + 255) / 256); }
 static int grid_bs(long n, int bs) { return (int)((n + bs - 1) / bs); }
@@ -591,6 +592,7 @@ int gauge_derive_links(Ctx* c) {
     case 3: return derive_links_d<3>(c);
     case 4: return derive_links_d<4>(c);
     case 6: return derive_links_d<6>(c);
+    case 8: return derive_links_d<8>(c);
   }
   set_error("unsupported representation dimension");
   return YH_ERR_UNSUPPORTED;
@@ -665,6 +667,7 @@ int clover_invert(Ctx* c, double sigma) {
     case 3: hipLaunchKernelGGL(k_clover_invert<3>, dim3(grid_bs(c->g.VH, 64)), dim3(64), 0, c->stream, src, inv, c->logdet_site, c->g, sigma); break;
     case 4: hipLaunchKernelGGL(k_clover_invert<4>, dim3(grid_bs(c->g.VH, 64)), dim3(64), 0, c->stream, src, inv, c->logdet_site, c->g, sigma); break;
     case 6: hipLaunchKernelGGL(k_clover_invert<6>, dim3(grid_bs(c->g.VH, 64)), dim3(64), 0, c->stream, src, inv, c->logdet_site, c->g, sigma); break;
+    case 8: hipLaunchKernelGGL(k_clover_invert<8>, dim3(grid_bs(c->g.VH, 64)), dim3(64), 0, c->stream, src, inv, c->logdet_site, c->g, sigma); break;
     default: set_error("unsupported representation dimension"); return YH_ERR_UNSUPPORTED;
   }
   YH_HIP(hipGetLastError());

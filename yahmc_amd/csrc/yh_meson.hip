@@ -256,6 +256,7 @@ int yh_meson_contract(yh_ctx* cc, int method, int nprop, const yh_field* const* 
     case 3: YH_MES(3); break;
     case 4: YH_MES(4); break;
     case 6: YH_MES(6); break;
+    case 8: YH_MES(8); break;
     default: set_error("unsupported representation dimension"); rc = YH_ERR_UNSUPPORTED;
   }
 #undef YH_MES

@@ -169,6 +169,7 @@ static int ms_x(Ctx* c, MsState* st, const MsFields* mf, Field* r, const Field* 
     case 3: return launch_x<3>(c, st, mf, r, q);
     case 4: return launch_x<4>(c, st, mf, r, q);
     case 6: return launch_x<6>(c, st, mf, r, q);
+    case 8: return launch_x<8>(c, st, mf, r, q);
   }
   set_error("unsupported representation dimension");
   return YH_ERR_UNSUPPORTED;
@@ -180,6 +181,7 @@ static int ms_k(Ctx* c, MsState* st, const MsFields* mf, const Field* r, Field* 
     case 3: return launch_k<3>(c, st, mf, r, p);
     case 4: return launch_k<4>(c, st, mf, r, p);
     case 6: return launch_k<6>(c, st, mf, r, p);
+    case 8: return launch_k<8>(c, st, mf, r, p);
   }
   set_error("unsupported representation dimension");
   return YH_ERR_UNSUPPORTED;
@@ -199,7 +201,8 @@ int mshift_device(Ctx* c, const yh_inv_par* par, int nrhs, int num, const double
   };
   // H^2 of up to four right-hand sides per hop launch (FP64, 18-real links,
   // clover term): batches of 4, then 2, the rest one by one
-  const bool batched = c->multi_rhs && nrhs > 1 && c->csw != 0.0 && recon == RECON_18;
+  // (not instantiated for D_R = 8, whose single-RHS hop already fills the registers)
+  const bool batched = c->multi_rhs && nrhs > 1 && c->csw != 0.0 && recon == RECON_18 && c->dr <= 6;
   std::vector<Rhs> rh(nrhs);
   std::vector<Field*> owned;
   auto alloc = [&](Field** f) -> int {
