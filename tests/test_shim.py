@@ -36,6 +36,7 @@ BUILDS = [
     ["-DNC=3", "-DREPR_FUNDAMENTAL", "-DCLOVER_TERM", "-DUPDATE_EO", "-DBC_OPEN"],
     ["-DNC=3", "-DREPR_SYMMETRIC", "-DCLOVER_TERM", "-DUPDATE_EO", "-DBC_SF"],
     ["-DNC=2", "-DREPR_ADJOINT", "-DUPDATE_EO", "-DBC_PERIODIC"],
+    ["-DNC=3", "-DREPR_ADJOINT", "-DCLOVER_TERM", "-DUPDATE_EO", "-DBC_ANTIPERIODIC"],
 ]
 
 
