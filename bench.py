@@ -166,7 +166,7 @@ def learned_solves(yh, lat, fb, prec, train_steps):
     return out
 
 
-def learned_protocol(yh, dims, ntrain=4, nheld=2, nsrc=4, steps=120, mass=-0.76):
+def learned_protocol(yh, dims, ntrain=4, nheld=2, nsrc=4, steps=120, mass=-0.76, pairs="hsq", gcr=False, npairs=8):
     """The learned-preconditioner evaluation protocol, reduced in size:
     `ntrain` training and `nheld` held-out configurations (each a cold start
     smoothed by 10 pure-gauge trajectories with its own seed), m = -0.76,
@@ -174,8 +174,12 @@ def learned_protocol(yh, dims, ntrain=4, nheld=2, nsrc=4, steps=120, mass=-0.76)
     pairs of each training configuration, then on every held-out
     configuration `nsrc` Gaussian sources solved one after another to an FP64
     true residual of 1e-10 by (a) mixed FP32 CG with the MRE guess of the
-    previous solutions (mre_past = 5) and (b) the learned solve (flexible CG
-    with the network from its own MRE guess, mixed-CG fallback)."""
+    previous solutions (mre_past = 5) and (b) the learned solve (flexible CG,
+    or flexible GCR with `gcr`, with the network from its own MRE guess,
+    mixed-CG fallback).  Training pairs: `hsq` = (H^2 v, v) for Gaussian v
+    (exact, one operator application each); `solution` = (Gaussian source,
+    its solution to an FP64 true residual of 1e-10 by the mixed FP32 CG), the
+    workload's kind."""
     from yahmc_amd.md import smooth
     from yahmc_amd.synthetic import gaussian_source
     t_all = time.perf_counter()
@@ -184,6 +188,7 @@ def learned_protocol(yh, dims, ntrain=4, nheld=2, nsrc=4, steps=120, mass=-0.76)
     for k in range(ntrain + nheld):
         smooth(lat, beta=5.6, trajectories=10, length=1.0, steps=10, seed=32246 + 1000 * (k + 1))
         cfgs.append((lat.get_gauge(), lat.plaquette()))
+        log(f"protocol: configuration {k} smoothed, plaquette {cfgs[-1][1]:.5f}")
     net = yh.Net(lat, (1, 8, 8, 1))
     rng = np.random.default_rng(32500)
     net.init_weights(rng)
@@ -191,18 +196,29 @@ def learned_protocol(yh, dims, ntrain=4, nheld=2, nsrc=4, steps=120, mass=-0.76)
     t0 = time.perf_counter()
     for k in range(ntrain):
         lat.set_gauge(cfgs[k][0])
-        xs = [lat.field("f64").upload(gaussian_source(dims, 3, seed=32600 + 100 * k + j)) for j in range(8)]
-        bs = []
-        for x in xs:
-            b = lat.field("f64")
-            lat.Hphi_sq(mass, b, x)
-            bs.append(b)
+        if pairs == "hsq":
+            xs = [lat.field("f64").upload(gaussian_source(dims, 3, seed=32600 + 100 * k + j)) for j in range(npairs)]
+            bs = []
+            for x in xs:
+                b = lat.field("f64")
+                lat.Hphi_sq(mass, b, x)
+                bs.append(b)
+        else:
+            bs = [lat.field("f64").upload(gaussian_source(dims, 3, seed=32600 + 100 * k + j)) for j in range(npairs)]
+            xs = []
+            for j, b in enumerate(bs):
+                x = lat.field("f64")
+                it, _ = lat.cg_mixed(lat.inv_par(mass, 1e-20, "Hphi_sq", sloppy="f32", max_iter=50000), x, b)
+                xs.append(x)
+                log(f"protocol: configuration {k} pair {j}: {it} CG iterations")
         hist += net.train(bs, xs, steps=steps // ntrain, batch=4, lr=2e-3, rng=rng)
+        log(f"protocol: trained on configuration {k}, loss {hist[-1]:.4f}")
         for f in bs + xs:
             f.free()
     lat.synchronize()
     t_train = time.perf_counter() - t0
     par = lat.inv_par(mass, 1e-20, "Hphi_sq", sloppy="f32")
+    lat.tune(yh.TUNE_LEARNED_GCR, 1 if gcr else 0)
     res = {"mixed_cg_mre5": {"iterations": 0, "seconds": 0.0, "max_true_rel_residual": 0.0},
            "learned_fcg_mre5": {"iterations": 0, "seconds": 0.0, "max_true_rel_residual": 0.0, "fallbacks": 0}}
     for k in range(ntrain, ntrain + nheld):
@@ -222,6 +238,7 @@ def learned_protocol(yh, dims, ntrain=4, nheld=2, nsrc=4, steps=120, mass=-0.76)
                 r["max_true_rel_residual"] = max(r["max_true_rel_residual"], true_residual(lat, fx, fb, mass))
                 if use_net is not None:
                     r["fallbacks"] += int(fell)
+                log(f"protocol: held-out configuration {k} source {j} {name}: {it} iterations")
                 fb.free()
                 fx.free()
             mre.free()
@@ -229,7 +246,10 @@ def learned_protocol(yh, dims, ntrain=4, nheld=2, nsrc=4, steps=120, mass=-0.76)
         r["seconds"] = round(r["seconds"], 3)
     out = {"lattice": list(dims), "mass": mass, "train_configs": ntrain, "held_out_configs": nheld,
            "sources_per_config": nsrc, "plaquettes": [round(c[1], 5) for c in cfgs],
-           "network": [1, 8, 8, 1], "training": {"pairs_per_config": 8, "pair_kind": "(H^2 v, v), v Gaussian",
+           "learned_solver": "flexible GCR" if gcr else "flexible CG",
+           "network": [1, 8, 8, 1], "training": {"pairs_per_config": npairs,
+                                                  "pair_kind": "(H^2 v, v), v Gaussian" if pairs == "hsq" else
+                                                  "(Gaussian source, solution to FP64 true residual 1e-10)",
                                                   "steps": len(hist), "seconds": round(t_train, 2),
                                                   "loss_first10": round(float(np.mean(hist[:10])), 4),
                                                   "loss_last10": round(float(np.mean(hist[-10:])), 4)},
@@ -374,6 +394,12 @@ def main():
     ap.add_argument("--hmc-sloppy", default="f16", choices=["f32", "f16"], help="sloppy precision of the HMC solves")
     ap.add_argument("--skip-protocol", action="store_true", help="skip the held-out learned-preconditioner protocol")
     ap.add_argument("--only-protocol", action="store_true", help="the learned-preconditioner protocol only")
+    ap.add_argument("--protocol-pairs", default="hsq", choices=["hsq", "solution"],
+                    help="training pairs of the protocol: (H^2 v, v) or (Gaussian source, solution)")
+    ap.add_argument("--protocol-gcr", action="store_true", help="the protocol's learned solve is flexible GCR")
+    ap.add_argument("--protocol-sizes", default="4,2,4,8",
+                    help="training configurations, held-out configurations, sources per held-out configuration, "
+                         "pairs per training configuration")
     ap.add_argument("--protocol-dims", default="32,16,16,16",
                     help="lattice T,X,Y,Z of the learned-preconditioner protocol (the workload's is 64,32,32,32)")
     ap.add_argument("--only-learned", action="store_true",
@@ -394,7 +420,9 @@ def main():
         print(json.dumps(rhmc_multishift(yh)))
         return
     if args.only_protocol:
-        print(json.dumps(learned_protocol(yh, tuple(int(v) for v in args.protocol_dims.split(",")))))
+        nt, nh, ns, npr = (int(v) for v in args.protocol_sizes.split(","))
+        print(json.dumps(learned_protocol(yh, tuple(int(v) for v in args.protocol_dims.split(",")), ntrain=nt, nheld=nh,
+                                          nsrc=ns, npairs=npr, pairs=args.protocol_pairs, gcr=args.protocol_gcr)))
         return
     t0 = time.perf_counter()
     links = random_gauge(DIMS, 3, seed=32244)
