@@ -264,8 +264,10 @@ int yh_wilson_fermion_force_q(yh_ctx* ctx, double dt, double mass, double residu
 /* clover_logdet_force (forces.cpp:109-119) */
 int yh_clover_logdet_force(yh_ctx* ctx, double mass, double residue);
 
-/* --- gauge molecular dynamics (FP64, periodic gauge boundaries: the
- * periodic and antiperiodic fermion BCs; open / SF return UNSUPPORTED) ----- */
+/* --- gauge molecular dynamics (FP64; open and SF boundaries as in
+ * library/bcs.cpp: spatial boundary plaquettes weighted 1/2, the fixed
+ * links' momenta zeroed before every link update and energy; SF for
+ * NC = 2, 3 only) ------------------------------------------------------- */
 /* the fundamental links (gfield_fund layout) as they are on the device */
 int yh_get_gauge(yh_ctx* ctx, double* links);
 /* represent_gauge_field (repr.cpp:265-280): rebuild the represented links
@@ -279,10 +281,18 @@ int yh_wilson_gauge_force(yh_ctx* ctx, double dt, double beta);
 int yh_update_links(yh_ctx* ctx, double eps);
 /* reunitarize (update.cpp:309-320) */
 int yh_reunitarize(yh_ctx* ctx);
-/* avr_wilson(1,1) (wilsonloop.cpp:116-133): average plaquette */
+/* avr_wilson(1,1) (wilsonloop.cpp:116-133): average plaquette, boundary
+ * plaquettes weighted, normalised by NC 6 volume (volume = (T-1) V3 for open,
+ * (T-2) V3 for SF boundaries, V otherwise) */
 int yh_plaquette(yh_ctx* ctx, double* out);
-/* momentum part of hamiltonian (update.cpp:201-216): (TF/2) sum |P|^2 */
+/* momentum part of hamiltonian (update.cpp:201-216): (TF/2) sum |P|^2,
+ * after apply_bcs_on_momentum_field (bcs.cpp:166-199) */
 int yh_momenta_energy(yh_ctx* ctx, double* out);
+/* apply_bcs_on_gauge_field (bcs.cpp:201-262) on the device fundamental
+ * links: open zeroes U_0 at t = T-1; SF sets the spatial links at t = 0 and
+ * T-2 to their boundary matrices and zeroes the other links of t = T-2 and
+ * T-1; periodic / antiperiodic leave the links alone */
+int yh_apply_bcs_on_gauge_field(yh_ctx* ctx);
 
 /* --- spectroscopy (observables/propagator.cpp, observables/mesons.cpp) ------ */
 /* calc_propagator (propagator.cpp:75-166): full-lattice D dst = src through

@@ -1421,7 +1421,49 @@ static gmat gadd(gmat a, gmat b)
 	return a;
 }
 
-/* staples_wilson(id, mu) */
+/* plaq_weight(id, mu, nu) of the open and SF boundaries (library/bcs.cpp:28-63,
+ * 72-130, include/bcs.h:34): spatial plaquettes based on the boundary slices
+ * (open: t = 0 and T-1; SF: t = 0 and T-2) count half */
+static double plaq_weight(int id, int mu, int nu)
+{
+	if (mu == 0 || nu == 0) return 1.0;
+	int t = crd[4 * id];
+	if (g_bc == BC_OPEN && (t == 0 || t == L[0] - 1)) return 0.5;
+	if (g_bc == BC_SF && (t == 0 || t == L[0] - 2)) return 0.5;
+	return 1.0;
+}
+
+/* the `volume` of the plaquette normalisation (bcs.cpp:32, 76; geometry.cpp:1150) */
+static double gauge_volume(void)
+{
+	int v3 = V / L[0];
+	if (g_bc == BC_OPEN) return (double)(L[0] - 1) * v3;
+	if (g_bc == BC_SF) return (double)(L[0] - 2) * v3;
+	return (double)V;
+}
+
+/* apply_bcs_on_momentum_field: library/bcs.cpp:166-199 */
+static void apply_bcs_on_momentum(void)
+{
+	for (int id = 0; id < V; id++) {
+		int t = crd[4 * id];
+		for (int mu = 0; mu < 4; mu++) {
+			int zero = 0;
+			if (g_bc == BC_OPEN) zero = t == L[0] - 1 && mu == 0;
+			if (g_bc == BC_SF) zero = (t == 0 && mu > 0) || t == L[0] - 2 || t == L[0] - 1;
+			if (zero) memset(mom + (4 * (size_t)id + mu) * NG, 0, sizeof(double) * NG);
+		}
+	}
+}
+
+static gmat gscale(gmat a, double w)
+{
+	for (int i = 0; i < NC; i++)
+		for (int j = 0; j < NC; j++) a.m[i][j] *= w;
+	return a;
+}
+
+/* staples_wilson(id, mu) (staples.cpp:82-107) with the plaquette weights */
 static gmat staple_wilson(int id, int mu)
 {
 	gmat st;
@@ -1429,10 +1471,10 @@ static gmat staple_wilson(int id, int mu)
 	for (int nu = 0; nu < 4; nu++) {
 		if (nu == mu) continue;
 		int o1 = nb_fw[4 * id + mu], o2 = nb_fw[4 * id + nu];
-		st = gadd(st, gmul(LNK(o1, nu), gdag(gmul(LNK(id, nu), LNK(o2, mu)))));
+		st = gadd(st, gscale(gmul(LNK(o1, nu), gdag(gmul(LNK(id, nu), LNK(o2, mu)))), plaq_weight(id, mu, nu)));
 		o1 = nb_bk[4 * id + nu];
 		o2 = nb_fw[4 * o1 + mu];
-		st = gadd(st, gmul(gdag(gmul(LNK(o1, mu), LNK(o2, nu))), LNK(o1, nu)));
+		st = gadd(st, gscale(gmul(gdag(gmul(LNK(o1, mu), LNK(o2, nu))), LNK(o1, nu)), plaq_weight(o1, nu, mu)));
 	}
 	return st;
 }
@@ -1460,7 +1502,8 @@ void or_wilson_gauge_force(double dt, double beta)
 		}
 }
 
-/* avr_wilson(1,1): plaquette average Re Tr / (NC 6 V) */
+/* avr_wilson(1,1): weighted plaquette average Re Tr / (NC 6 volume)
+ * (wilsonloop.cpp:58-133) */
 double or_plaquette(void)
 {
 	double sum = 0;
@@ -1472,17 +1515,20 @@ double or_plaquette(void)
 				gmat one = gmul(LNK(id, mu), LNK(a, nu));
 				gmat two = gmul(LNK(id, nu), LNK(b, mu));
 				gmat P = gmul(one, gdag(two));
-				for (int i = 0; i < NC; i++) sum += creal(P.m[i][i]);
+				double tr = 0;
+				for (int i = 0; i < NC; i++) tr += creal(P.m[i][i]);
+				sum += plaq_weight(id, mu, nu) * tr;
 			}
-	return sum / (NC * 6.0 * V);
+	return sum / (NC * 6.0 * gauge_volume());
 }
 
-/* MonomialGauge::action for the Wilson action: -beta * plaq * 6 V */
-double or_gauge_action(double beta) { return -beta * or_plaquette() * 6.0 * V; }
+/* MonomialGauge::action for the Wilson action: -beta * plaq * 6 volume */
+double or_gauge_action(double beta) { return -beta * or_plaquette() * 6.0 * gauge_volume(); }
 
-/* TF/2 sum |P|^2 (update.cpp:201-216) */
+/* TF/2 sum |P|^2 after the momentum BCs (update.cpp:201-216) */
 double or_kinetic(void)
 {
+	apply_bcs_on_momentum();
 	double v = 0;
 	for (size_t k = 0; k < (size_t)4 * V * NG; k++) v += mom[k] * mom[k];
 	return v * (TF / 2.0);
@@ -1591,9 +1637,10 @@ static gmat exp_alg(const double *av, double coeff)
 	return reunitarize_g(R);
 }
 
-/* update_links: U <- exp(eps P) U (update.cpp:228-247) */
+/* update_links: momentum BCs, U <- exp(eps P) U, gauge BCs (update.cpp:228-247) */
 void or_update_links(double eps)
 {
+	apply_bcs_on_momentum();
 #pragma omp parallel for
 	for (int id = 0; id < V; id++)
 		for (int mu = 0; mu < 4; mu++) {
@@ -1601,6 +1648,7 @@ void or_update_links(double eps)
 			gmat E = exp_alg(mom + (4 * (size_t)id + mu) * NG, eps);
 			gstore(l, gmul(E, gload(l)));
 		}
+	or_apply_bcs_on_gauge_field();
 }
 
 void or_reunitarize(void)

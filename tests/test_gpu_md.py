@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from fields import gaussian_spinor, random_gauge
-from oracle import BC_ANTIPERIODIC, BC_PERIODIC, Oracle
+from oracle import BC_ANTIPERIODIC, BC_OPEN, BC_PERIODIC, BC_SF, Oracle
 
 pytestmark = pytest.mark.gpu
 yh = pytest.importorskip("yahmc_amd")
@@ -103,10 +103,100 @@ def test_o2mn_trajectory_matches_oracle_and_operator_follows():
     assert np.abs(got[: o.VH] - want[: o.VH]).max() < 1e-12 * np.abs(want).max()
 
 
-def test_open_boundaries_rejected():
-    lat = yh.Lattice((4, 4, 4, 4), nc=3, repr="fundamental", bc="open", csw=0.0)
+def setup_bc(model, nc, rep, bc, dims=(6, 4, 4, 4), seed=13):
+    """Random links with the gauge-field BCs applied by the oracle
+    (bcs.cpp:201-262) and handed to the device as they are."""
+    obc = {"open": BC_OPEN, "sf": BC_SF}[bc]
+    o = Oracle(model).init(dims, bc=obc, csw=1.0, cf=1.1)
+    rng = np.random.default_rng(seed)
+    o.set_gauge(random_gauge(rng, o.V, nc))
+    o.apply_bcs_on_gauge()
+    links = o.get_gauge()
+    lat = yh.Lattice(dims, nc=nc, repr=rep, bc=bc, csw=1.0, cf=1.1)
+    lat.set_gauge(links)
+    return o, lat, rng
+
+
+BC_CASES = [("su3_fund", 3, "fundamental", "open"), ("su3_fund", 3, "fundamental", "sf"),
+            ("su2_fund", 2, "fundamental", "sf"), ("su4_asym", 4, "antisymmetric", "open")]
+
+
+@pytest.mark.parametrize("model,nc,rep,bc", BC_CASES)
+def test_boundary_gauge_md_matches_oracle(model, nc, rep, bc):
+    """Open / SF gauge boundaries (bcs.cpp:11-262): weighted plaquette and
+    staples, momentum BCs before the energy and the link update, fixed links
+    unchanged by the update; the device applies the gauge BCs to a cold
+    field as the oracle does."""
+    print(f"\n[case] boundary gauge MD {model} {bc}", flush=True)
+    o, lat, rng = setup_bc(model, nc, rep, bc)
+    assert abs(lat.plaquette() - o.plaquette()) < 1e-14
+    P = rng.normal(size=(4 * o.V, nc * nc - 1))
+    o.set_momenta(P)
+    lat.set_momenta(P)
+    o.wilson_gauge_force(0.37, 5.6)
+    lat.wilson_gauge_force(0.37, 5.6)
+    want = o.get_momenta()
+    assert np.abs(lat.get_momenta() - want).max() < 1e-13 * np.abs(want).max()
+    e_dev, e_or = lat.momenta_energy(), o.kinetic()   # both zero the fixed links' momenta
+    assert abs(e_dev - e_or) < 1e-12 * e_or
+    assert np.array_equal(lat.get_momenta() == 0, o.get_momenta() == 0)
+    U0 = o.get_gauge()
+    o.update_links(0.21)
+    lat.update_links(0.21)
+    U1 = o.get_gauge()
+    assert np.abs(lat.get_gauge() - U1).max() < 1e-13
+    # the fixed links did not move
+    zero = np.all(o.get_momenta() == 0, axis=1)
+    assert np.count_nonzero(zero) > 0
+    assert np.abs(U1[zero] - U0[zero]).max() < 1e-15
+    # cold start: the device sets the boundary links the oracle's way
+    cold = np.zeros((4 * o.V, 2 * nc * nc))
+    for i in range(nc):
+        cold[:, i * nc + i] = 1.0
+    o.set_gauge(cold)
+    o.apply_bcs_on_gauge()
+    lat.set_gauge(cold)
+    lat.apply_bcs_on_gauge_field()
+    assert np.abs(lat.get_gauge() - o.get_gauge()).max() < 1e-15
+    assert abs(lat.plaquette() - o.plaquette()) < 1e-14
+
+
+def test_open_boundary_o2mn_trajectory_matches_oracle():
+    """A five-step o2mn gauge trajectory with open boundaries: dH, links and
+    momenta agree with the oracle."""
+    print("\n[case] o2mn trajectory su3_fund open", flush=True)
+    o, lat, rng = setup_bc("su3_fund", 3, "fundamental", "open", seed=17)
+    beta = 5.6
+    P = rng.normal(size=(4 * o.V, 8))
+    o.set_momenta(P)
+    lat.set_momenta(P)
+
+    def ham(x):
+        return x.momenta_energy() - beta * x.plaquette() * 6 * lat.gauge_volume if x is lat else \
+            x.kinetic() + x.gauge_action(beta)
+
+    h0d, h0o = ham(lat), ham(o)
+    assert abs(h0d - h0o) < 1e-10 * abs(h0o)
+    a, steps, length = 0.1931833275037836, 5, 0.5
+    dt = length / steps
+    for x in (o, lat):
+        for n in range(steps):
+            x.wilson_gauge_force(a * dt if n == 0 else 2 * a * dt, beta)
+            x.update_links(dt / 2)
+            x.wilson_gauge_force((1 - 2 * a) * dt, beta)
+            x.update_links(dt / 2)
+        x.wilson_gauge_force(a * dt, beta)
+    dHd, dHo = ham(lat) - h0d, ham(o) - h0o
+    print(f"dH device {dHd:.6e} oracle {dHo:.6e}")
+    assert abs(dHd - dHo) < 1e-9 * abs(h0o)
+    assert np.abs(lat.get_gauge() - o.get_gauge()).max() < 1e-12
+    assert np.abs(lat.get_momenta() - o.get_momenta()).max() < 1e-11
+
+
+def test_sf_needs_nc_up_to_3():
+    lat = yh.Lattice((6, 4, 4, 4), nc=4, repr="antisymmetric", bc="sf", csw=0.0)
     rng = np.random.default_rng(1)
-    lat.set_gauge(random_gauge(rng, lat.V, 3))
+    lat.set_gauge(random_gauge(rng, lat.V, 4))
     with pytest.raises(yh.YahmcError) as e:
         lat.plaquette()
     assert e.value.code == -6

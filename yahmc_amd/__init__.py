@@ -137,6 +137,7 @@ _SIGS = {
     "yh_wilson_gauge_force": ([_VP, ctypes.c_double, ctypes.c_double], ctypes.c_int),
     "yh_update_links": ([_VP, ctypes.c_double], ctypes.c_int),
     "yh_reunitarize": ([_VP], ctypes.c_int),
+    "yh_apply_bcs_on_gauge_field": ([_VP], ctypes.c_int),
     "yh_plaquette": ([_VP, _D], ctypes.c_int),
     "yh_momenta_energy": ([_VP, _D], ctypes.c_int),
     "yh_begin_fermion_force": ([_VP], ctypes.c_int),
@@ -621,6 +622,23 @@ class Lattice:
         out = np.zeros(1)
         _check(lib().yh_momenta_energy(self.ctx, _dp(out)))
         return float(out[0])
+
+    def apply_bcs_on_gauge_field(self):
+        """apply_bcs_on_gauge_field (library/bcs.cpp:201-262) on the device
+        fundamental links (no-op for periodic / antiperiodic)."""
+        _check(lib().yh_apply_bcs_on_gauge_field(self.ctx))
+
+    @property
+    def gauge_volume(self):
+        """The `volume` of the plaquette normalisation and the gauge action
+        (library/bcs.cpp:32, 76; geometry.cpp:1150): (T-1) V3 for open,
+        (T-2) V3 for SF boundaries, V otherwise."""
+        v3 = self.V // self.dims[0]
+        if self.bc == BC["open"]:
+            return (self.dims[0] - 1) * v3
+        if self.bc == BC["sf"]:
+            return (self.dims[0] - 2) * v3
+        return self.V
 
     # ---- measurement ---------------------------------------------------
     def bench_hop(self, prec, recon, reps, d, s):

@@ -7,8 +7,8 @@
 // plaquette          : library/wilsonloop.cpp:58-133 (avr_wilson(1,1))
 // hamiltonian (P^2)  : library/update.cpp:201-216
 // Fundamental links stay in the reference layout (4*id+mu, re plane then im
-// plane); momenta are SoA mom[(mu * NG + a) * V + id].  Periodic gauge
-// boundaries (the plaquette weights of open / SF are not applied here).
+// plane); momenta are SoA mom[(mu * NG + a) * V + id].  Periodic, open and
+// SF gauge boundaries (bcs.cpp: boundary plaquette weights, fixed links).
 #include <algorithm>
 #include <vector>
 #include "yh_internal.hpp"
@@ -17,6 +17,11 @@
 namespace yh {
 
 typedef Cx<double> cd;
+
+// diagonal phases of the SF boundary links (bcs.cpp:208-232)
+struct SfPhases {
+  double c1[4], c2[4];
+};
 
 template <int NC>
 struct M {
@@ -242,7 +247,60 @@ __device__ __forceinline__ void reunit(M<NC>& u) {
 // fundamental generators iT_a, NC x NC: the context's table
 // tab_itf[a * 16 + i * NC + k], passed to the kernels
 
-// Wilson gauge force: mom -= algebra_project(U staple, dt beta / NC)
+// Open and SF boundaries of the gauge field (library/bcs.cpp:28-130, 166-262):
+// spatial plaquettes based on the boundary slices (open: t = 0 and T-1; SF:
+// t = 0 and T-2) count half in the action and its force, and the links the
+// boundary fixes (open: U_0 at t = T-1; SF: the spatial links at t = 0, every
+// link at t = T-2 and T-1) keep their values: their momenta are zeroed.
+__device__ __forceinline__ double plaq_w(int t, int mu, int nu, const Geom& g) {
+  if (mu == 0 || nu == 0) return 1.0;
+  if (g.bc == BC_OPEN && (t == 0 || t == g.L[0] - 1)) return 0.5;
+  if (g.bc == BC_SF && (t == 0 || t == g.L[0] - 2)) return 0.5;
+  return 1.0;
+}
+
+__device__ __forceinline__ bool link_fixed(int t, int mu, const Geom& g) {
+  if (g.bc == BC_OPEN) return t == g.L[0] - 1 && mu == 0;
+  if (g.bc == BC_SF) return (t == 0 && mu > 0) || t == g.L[0] - 2 || t == g.L[0] - 1;
+  return false;
+}
+
+// apply_bcs_on_momentum_field (bcs.cpp:166-199)
+__global__ void __launch_bounds__(256) k_mom_bcs(double* __restrict__ mom, int NG, Geom g) {
+  int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= 4 * g.V) return;
+  int id = t >> 2, mu = t & 3;
+  int c[4];
+  full_coords(id, g, c);
+  if (!link_fixed(c[0], mu, g)) return;
+  for (int a = 0; a < NG; a++) mom[(size_t)(mu * NG + a) * g.V + id] = 0.0;
+}
+
+// apply_bcs_on_gauge_field (bcs.cpp:201-262): the fixed links take their
+// boundary values: zero, or the SF matrices diag(exp(i c1_k)), diag(exp(i c2_k))
+template <int NC>
+__global__ void __launch_bounds__(256) k_gauge_bcs(double* __restrict__ fund, Geom g, SfPhases ph) {
+  int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= 4 * g.V) return;
+  int id = t >> 2, mu = t & 3;
+  int c[4];
+  full_coords(id, g, c);
+  if (!link_fixed(c[0], mu, g)) return;
+  M<NC> u;
+#pragma unroll
+  for (int i = 0; i < NC; i++)
+#pragma unroll
+    for (int j = 0; j < NC; j++) u.m[i][j] = cd(0, 0);
+  if (g.bc == BC_SF && mu > 0 && (c[0] == 0 || c[0] == g.L[0] - 2)) {
+    const double* v = c[0] == 0 ? ph.c1 : ph.c2;
+#pragma unroll
+    for (int i = 0; i < NC; i++) u.m[i][i] = cd(cos(v[i]), sin(v[i]));
+  }
+  st<NC>(fund, id, mu, u);
+}
+
+// Wilson gauge force: mom -= algebra_project(U staple, dt beta / NC), the
+// staples weighted by their plaquettes' weights (staples.cpp:82-107)
 template <int NC>
 __global__ void __launch_bounds__(128) k_gauge_force(const double* __restrict__ fund, double* __restrict__ mom,
                                                      double coeff, Geom g, const double2* __restrict__ itf) {
@@ -269,10 +327,11 @@ __global__ void __launch_bounds__(128) k_gauge_force(const double* __restrict__ 
     int b2 = mshift(cb, mu, 1, g);
     M<NC> q = mul<NC>(ld<NC>(fund, b, mu), ld<NC>(fund, b2, nu));
     M<NC> s2 = dmul<NC>(q, ld<NC>(fund, b, nu));
+    const double w1 = plaq_w(c[0], mu, nu, g), w2 = plaq_w(cb[0], nu, mu, g);
 #pragma unroll
     for (int i = 0; i < NC; i++)
 #pragma unroll
-      for (int j = 0; j < NC; j++) S.m[i][j] = S.m[i][j] + s1.m[i][j] + s2.m[i][j];
+      for (int j = 0; j < NC; j++) S.m[i][j] = S.m[i][j] + w1 * s1.m[i][j] + w2 * s2.m[i][j];
   }
   M<NC> F = mul<NC>(ld<NC>(fund, id, mu), S);
 #pragma unroll
@@ -347,8 +406,10 @@ __global__ void __launch_bounds__(256) k_plaquette(const double* __restrict__ fu
         M<NC> one = mul<NC>(ld<NC>(fund, id, mu), ld<NC>(fund, a, nu));
         M<NC> two = mul<NC>(ld<NC>(fund, id, nu), ld<NC>(fund, b, mu));
         M<NC> P = muld<NC>(one, two);
+        double tr = 0.0;
 #pragma unroll
-        for (int i = 0; i < NC; i++) v[0] += P.m[i][i].re;
+        for (int i = 0; i < NC; i++) tr += P.m[i][i].re;
+        v[0] += plaq_w(c[0], mu, nu, g) * tr;
       }
   }
   block_reduce_finish<1, 256>(v, partials, gridDim.x, blockIdx.x, ticket, out);
@@ -407,9 +468,53 @@ int md_gauge_force(Ctx* c, double dt, double beta) {
   return YH_OK;
 }
 
+static bool gauge_bcs(const Ctx* c) { return c->g.bc == BC_OPEN || c->g.bc == BC_SF; }
+
+static int mom_bcs(Ctx* c) {
+  if (!gauge_bcs(c)) return YH_OK;
+  hipLaunchKernelGGL(k_mom_bcs, dim3((4 * c->g.V + 255) / 256), dim3(256), 0, c->stream, c->mom, c->nc * c->nc - 1, c->g);
+  YH_HIP(hipGetLastError());
+  return YH_OK;
+}
+
+// the `volume` the plaquette is normalised by (bcs.cpp:32, 76; geometry.cpp:1150)
+static double gauge_volume(const Ctx* c) {
+  const double v3 = (double)c->g.V / c->g.L[0];
+  if (c->g.bc == BC_OPEN) return (c->g.L[0] - 1) * v3;
+  if (c->g.bc == BC_SF) return (c->g.L[0] - 2) * v3;
+  return (double)c->g.V;
+}
+
+int md_apply_gauge_bcs(Ctx* c) {
+  if (!gauge_bcs(c)) return YH_OK;
+  SfPhases ph = {};
+  if (c->g.bc == BC_SF) {
+    // bcs.cpp:208-232: phases v1 / L, v2 / L with L = 2 NC L_X
+    const double pi = 3.14159265358979323846;
+    const double L = 2.0 * c->nc * c->g.L[1];
+    if (c->nc == 2) {
+      ph.c1[0] = -pi / L; ph.c1[1] = pi / L;
+      ph.c2[0] = -3 * pi / L; ph.c2[1] = 3 * pi / L;
+    } else if (c->nc == 3) {
+      ph.c1[0] = -pi / L; ph.c1[1] = 0; ph.c1[2] = pi / L;
+      ph.c2[0] = -5 * pi / L; ph.c2[1] = 2 * pi / L; ph.c2[2] = 3 * pi / L;
+    } else {
+      set_error("SF boundary values are defined for NC = 2 and 3");
+      return YH_ERR_UNSUPPORTED;
+    }
+  }
+  dim3 grid((4 * c->g.V + 255) / 256);
+  YH_NC_SWITCH(hipLaunchKernelGGL((k_gauge_bcs<NC_>), grid, dim3(256), 0, c->stream, c->fund, c->g, ph));
+  YH_HIP(hipGetLastError());
+  return YH_OK;
+}
+
 int md_update_links(Ctx* c, double eps) {
   int rc;
   if ((rc = force_alloc(c))) return rc;
+  // the fixed links' momenta are zeroed, so exp(0) U leaves them as they are
+  // (update.cpp:231-243 zeroes them, updates, and sets the fixed links again)
+  if ((rc = mom_bcs(c))) return rc;
   dim3 grid((4 * c->g.V + 127) / 128);
   YH_NC_SWITCH(hipLaunchKernelGGL((k_update_links<NC_>), grid, dim3(128), 0, c->stream, c->fund, c->mom, eps, c->g));
   YH_HIP(hipGetLastError());
@@ -432,13 +537,15 @@ int md_plaquette(Ctx* c, double* out) {
   double v;
   YH_HIP(hipMemcpyAsync(&v, dst, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   YH_HIP(hipStreamSynchronize(c->stream));
-  *out = v / (c->nc * 6.0 * c->g.V);
+  *out = v / (c->nc * 6.0 * gauge_volume(c));
   return YH_OK;
 }
 
 int md_kinetic(Ctx* c, double* out) {
   int rc;
   if ((rc = force_alloc(c))) return rc;
+  // hamiltonian() applies the momentum BCs first (update.cpp:201-206)
+  if ((rc = mom_bcs(c))) return rc;
   long n = 4L * c->g.V * (c->nc * c->nc - 1);
   int grid = (int)std::min<long>((n + 255) / 256, (long)c->nblk);
   double* dst = &c->sc->red[0];
@@ -459,8 +566,8 @@ using namespace yh;
 static int md_check(Ctx* c) {
   if (!c) { set_error("null argument"); return YH_ERR_ARG; }
   if (!c->gauge_set) { set_error("no gauge field"); return YH_ERR_STATE; }
-  if (c->g.bc == YH_BC_OPEN || c->g.bc == YH_BC_SF) {
-    set_error("gauge MD is built for periodic gauge boundaries");
+  if (c->g.bc == YH_BC_SF && c->nc > 3) {
+    set_error("SF boundary values are defined for NC = 2 and 3");
     return YH_ERR_UNSUPPORTED;
   }
   YH_HIP(hipSetDevice(c->device));
@@ -519,6 +626,12 @@ int yh_plaquette(yh_ctx* c, double* out) {
   int rc = md_check(c);
   if (rc) return rc;
   return md_plaquette(c, out);
+}
+
+int yh_apply_bcs_on_gauge_field(yh_ctx* c) {
+  int rc = md_check(c);
+  if (rc) return rc;
+  return md_sync(c, md_apply_gauge_bcs(c));
 }
 
 int yh_momenta_energy(yh_ctx* c, double* out) {

@@ -38,7 +38,7 @@ class GaugeMonomial:
         pass
 
     def action(self, lat):
-        return -self.beta * lat.plaquette() * 6.0 * lat.V
+        return -self.beta * lat.plaquette() * 6.0 * lat.gauge_volume
 
     def update(self, lat, dt):
         lat.wilson_gauge_force(dt, self.beta)

@@ -25,9 +25,9 @@ def o2mn_gauge(lat, beta, length, steps):
 
 
 def hamiltonian(lat, beta):
-    """TF/2 sum P^2 - beta * plaq * 6V (update.cpp:201-226 with the Wilson
-    gauge action)."""
-    return lat.momenta_energy() - beta * lat.plaquette() * 6.0 * lat.V
+    """TF/2 sum P^2 - beta * plaq * 6 volume (update.cpp:201-226 with the
+    Wilson gauge action; volume = V with periodic gauge boundaries)."""
+    return lat.momenta_energy() - beta * lat.plaquette() * 6.0 * lat.gauge_volume
 
 
 def gauge_trajectory(lat, beta, rng, length=1.0, steps=10, metropolis=True):
@@ -52,12 +52,15 @@ def gauge_trajectory(lat, beta, rng, length=1.0, steps=10, metropolis=True):
 
 
 def cold_start(lat):
-    """All links = 1 (gaugefield.cpp cold start)."""
+    """All links = 1 (gaugefield.cpp cold start), then the gauge-field
+    boundary conditions (gaugefield.cpp:193)."""
     n = lat.nc
     links = np.zeros((4 * lat.V, 2 * n * n))
     for i in range(n):
         links[:, i * n + i] = 1.0
     lat.set_gauge(links)
+    lat.apply_bcs_on_gauge_field()
+    lat.represent_gauge_field()
 
 
 def smooth(lat, beta=5.6, trajectories=10, length=1.0, steps=10, seed=32246, metropolis=False, cold=True):
