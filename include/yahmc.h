@@ -259,7 +259,10 @@ int yh_clover_fermion_force(yh_ctx* ctx, double residue, const yh_field* lhs, co
 /* wilson_fermion_force (forces.cpp:175-227): overwrites the odd halves of
  * lhs and rhs with D_oo^-1 D_oe X_e as the reference does */
 int yh_wilson_fermion_force(yh_ctx* ctx, double dt, double mass, double residue, yh_field* lhs, yh_field* rhs);
-/* wilson_fermion_force_q (forces.cpp:230-236) */
+/* wilson_fermion_force_q (forces.cpp:230-236): overwrites the odd half of
+ * rhs with D_oo^-1 D_oe X_e; for open and SF boundaries it also zeroes the
+ * boundary time slices of rhs's even half, as the reference's in-place
+ * Hphi(tmp, rhs) does (dirac.cpp:700) before the force reads it */
 int yh_wilson_fermion_force_q(yh_ctx* ctx, double dt, double mass, double residue, yh_field* rhs);
 /* clover_logdet_force (forces.cpp:109-119) */
 int yh_clover_logdet_force(yh_ctx* ctx, double mass, double residue);
@@ -417,6 +420,11 @@ int yh_synchronize(yh_ctx* ctx);
  * neighbours there (when a (t, x) plane holds a multiple of 256 sites);
  * default 3: FP64 and FP32 */
 #define YH_TUNE_HOP_STAGE 8
+/* order in which the clover-force kernel's workgroups take their (site
+ * block, mu) work: -1 = all sites of one mu after another; q >= 0 = the
+ * four mu of a site block together, in runs of q workgroups per XCD
+ * (0, default = one contiguous slab of the lattice per XCD) */
+#define YH_TUNE_FORCE_ORDER 9
 int yh_tune(yh_ctx* ctx, int key, int value);
 /* autotuner (no counterpart in the reference): times the hop of every
  * precision / reconstruction this context holds for each XCD chunk size

@@ -10,13 +10,15 @@ import numpy as np
 import pytest
 
 from fields import gaussian_spinor, random_gauge
-from oracle import BC_ANTIPERIODIC, BC_PERIODIC, Oracle
+from oracle import BC_ANTIPERIODIC, BC_OPEN, BC_PERIODIC, Oracle
 
 pytestmark = pytest.mark.gpu
 yh = pytest.importorskip("yahmc_amd")
 
 MODELS = {"su3_fund": (3, "fundamental"), "su2_adj": (2, "adjoint"), "su3_sym": (3, "symmetric"),
-          "su2_fund": (2, "fundamental"), "su3_adj": (3, "adjoint")}
+          "su2_fund": (2, "fundamental"), "su3_adj": (3, "adjoint"), "su3_asym": (3, "antisymmetric"),
+          "su4_asym": (4, "antisymmetric")}
+BC_NAMES = {BC_PERIODIC: "periodic", BC_ANTIPERIODIC: "antiperiodic", BC_OPEN: "open"}
 
 
 def rel(a, b):
@@ -25,17 +27,19 @@ def rel(a, b):
 
 @pytest.mark.parametrize("model,csw,bc", [("su3_fund", 1.0, BC_ANTIPERIODIC), ("su3_fund", 0.0, BC_PERIODIC),
                                           ("su2_adj", 1.2, BC_ANTIPERIODIC), ("su3_sym", 1.0, BC_ANTIPERIODIC),
-                                          ("su2_fund", 1.0, BC_PERIODIC), ("su3_adj", 1.0, BC_ANTIPERIODIC)])
+                                          ("su2_fund", 1.0, BC_PERIODIC), ("su3_adj", 1.0, BC_ANTIPERIODIC),
+                                          ("su3_asym", 1.1, BC_PERIODIC), ("su4_asym", 1.0, BC_ANTIPERIODIC),
+                                          ("su3_fund", 1.0, BC_OPEN)])
 def test_hmc_force_sequence(model, csw, bc):
     print(f"\n[case] forces model={model} csw={csw} bc={bc}", flush=True)
     nc, rep = MODELS[model]
-    dims = (4, 4, 4, 4)
+    dims = (6, 4, 4, 4) if bc == BC_OPEN else (4, 4, 4, 4)
     o = Oracle(model).init(dims, bc=bc, csw=csw)
     rng = np.random.default_rng(5)
     links = random_gauge(rng, o.V, nc)
     o.set_gauge(links)
     o.represent()
-    lat = yh.Lattice(dims, nc=nc, repr=rep, bc={0: "periodic", 1: "antiperiodic"}[bc], csw=csw)
+    lat = yh.Lattice(dims, nc=nc, repr=rep, bc=BC_NAMES[bc], csw=csw)
     lat.set_gauge(links)
     mass, dt = -0.3, 0.7
     ng = nc * nc - 1

@@ -22,6 +22,7 @@ YH_OK = 0
 TUNE_XCD_CHUNK, TUNE_NET_PREFETCH, TUNE_GRAPHS, TUNE_HALF_LINKS, TUNE_RECON12_F64, TUNE_MULTI_RHS = 1, 2, 3, 4, 5, 6
 TUNE_LEARNED_GCR = 7
 TUNE_HOP_STAGE = 8
+TUNE_FORCE_ORDER = 9
 ERRORS = {-1: "invalid argument", -2: "HIP error", -3: "bad state", -4: "not converged", -5: "non-finite",
           -6: "unsupported"}
 

@@ -693,6 +693,11 @@ int yh_tune(yh_ctx* c, int key, int value) {
     c->hop_stage = value;
     return YH_OK;
   }
+  if (key == YH_TUNE_FORCE_ORDER) {
+    if (value < -1) { set_error("force order: -1, or a run length >= 0"); return YH_ERR_ARG; }
+    c->force_order = value;
+    return YH_OK;
+  }
   if (key == YH_TUNE_MULTI_RHS) {
     c->multi_rhs = value != 0;
     return YH_OK;

@@ -13,6 +13,7 @@
 // planes in the reference order 01, 02, 12, 03, 13, 23.
 #include <vector>
 #include <mutex>
+#include <utility>
 #include "yh_internal.hpp"
 
 namespace yh {
@@ -185,10 +186,23 @@ __device__ __forceinline__ int fshift(const int c[4], int mu, int d, const Geom&
   return fidx(n[0], n[1], n[2], n[3], g);
 }
 
+// spinor of site id; zero on the time slices the open and SF boundaries
+// clear (bcs.cpp:139-164): the reference's Dphi_oe(X, X) at the start of the
+// force zeroes those slices of the whole field, both halves
 template <int D>
 __device__ __forceinline__ void load_full_spinor(cd (&s)[4][D], const void* const (&h)[2], int id, const Geom& g) {
   int p = id >= g.VH;
   load_spinor<PREC_F64, D, double>(s, h[p], nullptr, id - p * g.VH, g.VHpad);
+  if (g.bc == BC_OPEN || g.bc == BC_SF) {
+    int c[4];
+    fcoords(id, g, c);
+    if (bc_zero_slice(c[0], g)) {
+#pragma unroll
+      for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int i = 0; i < D; i++) s[a][i] = cd(0, 0);
+    }
+  }
 }
 
 // represented link U_R(id, mu) with the antiperiodic sign of the reference's
@@ -530,207 +544,422 @@ __device__ __forceinline__ void madd(cd (&A)[D][D], const cd (&B)[D][D], double 
 //   F = U(x, mu) [W8 Z0 + Z1 W8 - W5 (W0 Z2 W1 + Z3 W6) + (W2 Z4 W3 + W7 Z5) W4]
 // and P(x, mu) += proj(F, sign c_sw dt / 8).
 //
-// Work split: one launch per nu (the three nu != mu in ascending order, so
-// every momentum receives its three terms in the reference's order and no
-// two lanes write the same momentum); within a launch a group of D lanes
-// handles one (x, mu), lane i computing row i of F.  Row i is a chain of
-// row-vector by matrix products starting from u = row i of U(x, mu):
-//   a = u W2, b = a W3, c = b W4, d = u W5, e = d W0, f = e W1, h = u Z1
-//   F_i = (a Z4 W3 + b Z5) W4 + (c - f) Z0 - (e Z2 + d Z3 W0) W1
-//         + h W2 W3 W4 - h W5 W0 W1
-// (22 products, no D x D intermediate).  For each product the group stages
-// the right-hand matrix in LDS (lane i fetches row i, or column i for a
-// dagger, one step ahead), and every lane of the group reads the whole
-// matrix from there.  The projection is summed over the group's lanes in
-// fixed order.  A lane holds about seven row vectors (D complex each).
+// k_force_clover_x: the three nu of one (x, mu) in one pass (the sign folded
+// into F, one projection per (x, mu)), with the reference's association of
+// the products (forces.cpp:80-96), row by row:
+//   W6 = W0 L(W1)          T1 = W0 L(Z2)          S = T1 L(W1) + Z3 L(W6)
+//   W8 = W7 L(W4) - W5 L(W6)   with W7 = W2 L(W3)
+//   Q  = (W2 L(Z4)) L(W3) + W7 L(Z5)
+//   F += sign [Q L(W4) + W8 L(Z0) + Z1 L(W8) - W5 L(S)]
+// and after the three nu one product M = U(x, mu) L(F), projected.  L(X)
+// is a matrix the group reads whole from LDS; the others are row vectors in
+// registers (row i in lane i of the group).  14 products per nu and one at
+// the end: 43 per (x, mu), against 48 for the reference's per-nu U fmat.
+//
+// The right-hand link and clover-force matrices reach LDS by direct DMA
+// (global_load_lds, 16 bytes a lane): for column j lane (g, i) fetches the
+// stored element (i, j) into slot word j * 64 + lane, so the stored (r, c)
+// of group g sits at c * 64 + g D + r, and a dagger operand is read
+// transposed and conjugated.  Computed matrices are written the same way.
+// Three slots (three LDS objects) rotate.  A step issues the DMAs of later
+// steps (lgkmcnt 0 first, so a slot's earlier reads and writes are done
+// before it is refilled), waits with vmcnt for the DMA of its own slot
+// (vector memory operations complete in issue order; the count is the DMA
+// instructions issued after it, D per matrix, and row loads only add to the
+// operations after it), and multiplies.  The slot reads are inline asm that
+// completes its own reads, since the compiler would otherwise wait for every
+// outstanding DMA before each one.  The block is one wave: LDS traffic needs
+// no barrier, only the waits and compiler fences.  Work items run mu
+// fastest, each XCD taking a contiguous slab of the lattice (xcd_block), so
+// the four mu of a site share its links and clover-force matrices in L2.
 template <int D>
-struct Op {
-  const double2* p;  // element (r, c) of the stored matrix at p[(r * D + c) * stride]
-  double sgn;
-  bool dag;          // the operand is the conjugate transpose of the stored matrix
+struct FX {
+  static constexpr int G = 64 / D;          // groups (x, mu) per wave
+  static constexpr int SLOT = D * 64;       // double2 words per slot
 };
 
-template <int D>
-__device__ __forceinline__ Op<D> link_op(const double2* links, int id, int mu, bool dag, const Geom& g) {
-  const MSrc m = link_src<D>(links, id, mu, dag, g);
-  return Op<D>{m.p, m.sgn, dag};
+// step boundary: memory operations stay on their side of it (program order
+// of the issues); the waits are the compiler's own
+__device__ __forceinline__ void fx_wait() { __asm__ volatile("" ::: "memory"); }
+
+// an integer the compiler cannot see through: addresses built from it are
+// computed where they are used instead of being hoisted and kept live
+__device__ __forceinline__ int fx_opq(int v) {
+  __asm__ volatile("" : "+v"(v));
+  return v;
 }
 
+// element 0 of the stored matrix: represented link (site, dir) or clover
+// force matrix (plane num, site), and its element stride
 template <int D>
-__device__ __forceinline__ Op<D> clf_op(const double2* clf, int num, int id, int V) {
-  return Op<D>{clf + (size_t)num * D * D * V + id, 1.0, false};
+__device__ __forceinline__ const double2* fx_link(const double2* links, int site, int dir, const Geom& g) {
+  const int p = site >= g.VH;
+  return links + (size_t)(p * 4 + dir) * D * D * g.VHpad + (site - p * g.VH);
+}
+template <int D>
+__device__ __forceinline__ const double2* fx_clf(const double2* clf, int num, int site, const Geom& g) {
+  return clf + (size_t)num * D * D * g.V + site;
 }
 
-// row i of the operand (sign and conjugation applied); `stride` is VHpad for
-// links and V for clover-force matrices.  The index passes through an empty
-// volatile asm first: the loads are of read-only kernel data, and without it
-// the compiler issues the loads of every step at the start of the kernel
+// DMA the stored matrix into a slot: lane (g, i) fetches its row i.  The
+// slot's earlier LDS reads and writes are complete first (lgkmcnt 0), so the
+// asynchronous DMA write cannot overtake them
 template <int D>
-__device__ __forceinline__ void fetch_row(double2 (&r)[D], const Op<D>& o, int i, size_t stride) {
-  int z = 0;
-  __asm__ volatile("" : "+v"(z));
-  const double2* p = o.p + z;
+__device__ __forceinline__ void fx_dma(double2* slot, const double2* p, int i, size_t stride) {
+  __asm__ volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0) only
+  __asm__ volatile("" ::: "memory");
+  const double2* q = p + (size_t)(i * D) * stride;
 #pragma unroll
   for (int j = 0; j < D; j++) {
-    const double2 v = p[(size_t)(o.dag ? j * D + i : i * D + j) * stride];
-    r[j] = o.dag ? make_double2(o.sgn * v.x, -o.sgn * v.y) : make_double2(o.sgn * v.x, o.sgn * v.y);
+    __attribute__((address_space(3))) void* dst = (__attribute__((address_space(3))) void*)(slot + j * 64);
+    __builtin_amdgcn_global_load_lds((const void*)(q + (size_t)j * stride), dst, 16, 0, 0);
   }
 }
 
+// row i of the stored matrix (DAG: of its conjugate transpose) into registers
+template <int D, bool DAG>
+__device__ __forceinline__ void fx_row(cd (&r)[D], const double2* p, int i, size_t stride) {
+  const double2* q = p + (size_t)(DAG ? i : i * D) * stride;
+#pragma unroll
+  for (int j = 0; j < D; j++) {
+    const double2 v = q[(size_t)(DAG ? j * D : j) * stride];
+    r[j] = DAG ? cd(v.x, -v.y) : cd(v.x, v.y);
+  }
+}
+
+// row i of a register matrix into a slot (stored layout)
 template <int D>
-__device__ __forceinline__ void put_row(double2* buf, const double2 (&r)[D], int i, bool on) {
-  if (on) {
+__device__ __forceinline__ void fx_put(double2* slot, const cd (&r)[D], int lane) {
 #pragma unroll
-    for (int j = 0; j < D; j++) buf[i * D + j] = r[j];
-  }
+  for (int j = 0; j < D; j++) slot[j * 64 + lane] = make_double2(r[j].re, r[j].im);
 }
 
-// an opaque zero that the compiler can only compute after every component
-// of v is final
-template <int D>
-__device__ __forceinline__ int after_row(cd (&v)[D]) {
-#pragma unroll
-  for (int k = 0; k < D; k++) {
-    __asm__ volatile("" : "+v"(v[k].re));
-    __asm__ volatile("" : "+v"(v[k].im));
-  }
-  int z;
-  __asm__ volatile("v_mov_b32 %0, 0" : "=v"(z));
-  return z;
+// LDS reads of the clover-force kernel's slots, written as inline asm so
+// that the compiler does not add its own wait for every outstanding DMA in
+// front of them (it cannot tell the slots apart across the loop): N 16-byte
+// reads at immediate offsets from one address, completed inside the block
+// (lgkmcnt 0), so no output register is left in flight.  The caller waits
+// for the DMA that filled the slot (vmcnt) before the block.
+typedef double fx_d2 __attribute__((ext_vector_type(2)));
+template <int O0, int O1, int O2>
+__device__ __forceinline__ void fx_rd(fx_d2 (&v)[3], unsigned a) {
+  __asm__ volatile(
+      "ds_read_b128 %0, %3 offset:%4\n\t"
+      "ds_read_b128 %1, %3 offset:%5\n\t"
+      "ds_read_b128 %2, %3 offset:%6\n\t"
+      "s_waitcnt lgkmcnt(0)"
+      : "=v"(v[0]), "=v"(v[1]), "=v"(v[2])
+      : "v"(a), "i"(O0), "i"(O1), "i"(O2));
+}
+template <int O0, int O1, int O2, int O3>
+__device__ __forceinline__ void fx_rd(fx_d2 (&v)[4], unsigned a) {
+  __asm__ volatile(
+      "ds_read_b128 %0, %4 offset:%5\n\t"
+      "ds_read_b128 %1, %4 offset:%6\n\t"
+      "ds_read_b128 %2, %4 offset:%7\n\t"
+      "ds_read_b128 %3, %4 offset:%8\n\t"
+      "s_waitcnt lgkmcnt(0)"
+      : "=v"(v[0]), "=v"(v[1]), "=v"(v[2]), "=v"(v[3])
+      : "v"(a), "i"(O0), "i"(O1), "i"(O2), "i"(O3));
+}
+template <int O0, int O1, int O2, int O3, int O4, int O5>
+__device__ __forceinline__ void fx_rd(fx_d2 (&v)[6], unsigned a) {
+  __asm__ volatile(
+      "ds_read_b128 %0, %6 offset:%7\n\t"
+      "ds_read_b128 %1, %6 offset:%8\n\t"
+      "ds_read_b128 %2, %6 offset:%9\n\t"
+      "ds_read_b128 %3, %6 offset:%10\n\t"
+      "ds_read_b128 %4, %6 offset:%11\n\t"
+      "ds_read_b128 %5, %6 offset:%12\n\t"
+      "s_waitcnt lgkmcnt(0)"
+      : "=v"(v[0]), "=v"(v[1]), "=v"(v[2]), "=v"(v[3]), "=v"(v[4]), "=v"(v[5])
+      : "v"(a), "i"(O0), "i"(O1), "i"(O2), "i"(O3), "i"(O4), "i"(O5));
+}
+template <int O0, int O1, int O2, int O3, int O4, int O5, int O6, int O7>
+__device__ __forceinline__ void fx_rd(fx_d2 (&v)[8], unsigned a) {
+  __asm__ volatile(
+      "ds_read_b128 %0, %8 offset:%9\n\t"
+      "ds_read_b128 %1, %8 offset:%10\n\t"
+      "ds_read_b128 %2, %8 offset:%11\n\t"
+      "ds_read_b128 %3, %8 offset:%12\n\t"
+      "ds_read_b128 %4, %8 offset:%13\n\t"
+      "ds_read_b128 %5, %8 offset:%14\n\t"
+      "ds_read_b128 %6, %8 offset:%15\n\t"
+      "ds_read_b128 %7, %8 offset:%16\n\t"
+      "s_waitcnt lgkmcnt(0)"
+      : "=v"(v[0]), "=v"(v[1]), "=v"(v[2]), "=v"(v[3]), "=v"(v[4]), "=v"(v[5]), "=v"(v[6]), "=v"(v[7])
+      : "v"(a), "i"(O0), "i"(O1), "i"(O2), "i"(O3), "i"(O4), "i"(O5), "i"(O6), "i"(O7));
+}
+template <int O0, int O1, int O2, int O3, int O4, int O5, int O6, int O7, int O8, int O9, int O10, int O11>
+__device__ __forceinline__ void fx_rd(fx_d2 (&v)[12], unsigned a) {
+  __asm__ volatile(
+      "ds_read_b128 %0, %12 offset:%13\n\t"
+      "ds_read_b128 %1, %12 offset:%14\n\t"
+      "ds_read_b128 %2, %12 offset:%15\n\t"
+      "ds_read_b128 %3, %12 offset:%16\n\t"
+      "ds_read_b128 %4, %12 offset:%17\n\t"
+      "ds_read_b128 %5, %12 offset:%18\n\t"
+      "ds_read_b128 %6, %12 offset:%19\n\t"
+      "ds_read_b128 %7, %12 offset:%20\n\t"
+      "ds_read_b128 %8, %12 offset:%21\n\t"
+      "ds_read_b128 %9, %12 offset:%22\n\t"
+      "ds_read_b128 %10, %12 offset:%23\n\t"
+      "ds_read_b128 %11, %12 offset:%24\n\t"
+      "s_waitcnt lgkmcnt(0)"
+      : "=v"(v[0]), "=v"(v[1]), "=v"(v[2]), "=v"(v[3]), "=v"(v[4]), "=v"(v[5]), "=v"(v[6]), "=v"(v[7]), "=v"(v[8]), "=v"(v[9]), "=v"(v[10]), "=v"(v[11])
+      : "v"(a), "i"(O0), "i"(O1), "i"(O2), "i"(O3), "i"(O4), "i"(O5), "i"(O6), "i"(O7), "i"(O8), "i"(O9), "i"(O10), "i"(O11));
+}
+template <int O0, int O1, int O2, int O3, int O4, int O5, int O6, int O7, int O8, int O9, int O10, int O11, int O12, int O13, int O14, int O15>
+__device__ __forceinline__ void fx_rd(fx_d2 (&v)[16], unsigned a) {
+  __asm__ volatile(
+      "ds_read_b128 %0, %16 offset:%17\n\t"
+      "ds_read_b128 %1, %16 offset:%18\n\t"
+      "ds_read_b128 %2, %16 offset:%19\n\t"
+      "ds_read_b128 %3, %16 offset:%20\n\t"
+      "ds_read_b128 %4, %16 offset:%21\n\t"
+      "ds_read_b128 %5, %16 offset:%22\n\t"
+      "ds_read_b128 %6, %16 offset:%23\n\t"
+      "ds_read_b128 %7, %16 offset:%24\n\t"
+      "ds_read_b128 %8, %16 offset:%25\n\t"
+      "ds_read_b128 %9, %16 offset:%26\n\t"
+      "ds_read_b128 %10, %16 offset:%27\n\t"
+      "ds_read_b128 %11, %16 offset:%28\n\t"
+      "ds_read_b128 %12, %16 offset:%29\n\t"
+      "ds_read_b128 %13, %16 offset:%30\n\t"
+      "ds_read_b128 %14, %16 offset:%31\n\t"
+      "ds_read_b128 %15, %16 offset:%32\n\t"
+      "s_waitcnt lgkmcnt(0)"
+      : "=v"(v[0]), "=v"(v[1]), "=v"(v[2]), "=v"(v[3]), "=v"(v[4]), "=v"(v[5]), "=v"(v[6]), "=v"(v[7]), "=v"(v[8]), "=v"(v[9]), "=v"(v[10]), "=v"(v[11]), "=v"(v[12]), "=v"(v[13]), "=v"(v[14]), "=v"(v[15])
+      : "v"(a), "i"(O0), "i"(O1), "i"(O2), "i"(O3), "i"(O4), "i"(O5), "i"(O6), "i"(O7), "i"(O8), "i"(O9), "i"(O10), "i"(O11), "i"(O12), "i"(O13), "i"(O14), "i"(O15));
 }
 
-// out (+)= s * (r M), M row-major in LDS.  M is read two rows at a time, and
-// each pair's reads take an index that depends (through an empty volatile
-// asm) on the previous pair's products: the workgroup is one wave, so the
-// compiler drops its barriers and would otherwise issue the LDS reads of
-// every step at once and spill them
-template <int D, bool ACC>
-__device__ __forceinline__ void vm(cd (&out)[D], cd (&r)[D], const double2* M, double s) {
-  cd t[D];
+// byte offset of element (k, j) of the group's matrix B in a slot, from the
+// group's base: stored (k, j) at word j * 64 + k, or for B = stored^+ the
+// stored (j, k) at word k * 64 + j
+template <bool DAG>
+__host__ __device__ constexpr int fx_off(int k, int j) { return DAG ? k * 1024 + j * 16 : j * 1024 + k * 16; }
+
+template <int D, bool DAG, int K0, int NK, size_t... I>
+__device__ __forceinline__ void fx_rd_rows(fx_d2 (&v)[NK * D], unsigned a, std::index_sequence<I...>) {
+  fx_rd<fx_off<DAG>(K0 + (int)I / D, (int)I % D)...>(v, a);
+}
+
+// out (+)= s (r B), B the group's matrix in a slot (DAG: B = stored^+),
+// `a` the LDS byte address of the slot plus 16 g D; out and r are different
+// arrays.  B is read two rows at a time.
+template <int D, bool DAG, bool ACC, int K0 = 0>
+__device__ __forceinline__ void fx_vm(cd (&out)[D], cd (&r)[D], unsigned a, double s) {
+  if constexpr (K0 < D) {
+    constexpr int NK = (D - K0 >= 2 && D != 8) ? 2 : 1;
+    fx_d2 v[NK * D];
+    fx_rd_rows<D, DAG, K0, NK>(v, a, std::make_index_sequence<NK * D>{});
 #pragma unroll
-  for (int j = 0; j < D; j++) t[j] = cd(0, 0);
-  int z = after_row<D>(r);
+    for (int dk = 0; dk < NK; dk++) {
+      const int k = K0 + dk;
+      const cd rk = s * r[k];
 #pragma unroll
-  for (int k = 0; k < D; k++) {
-    if (k > 0 && (k & 1) == 0) z = after_row<D>(t);
-#pragma unroll
-    for (int j = 0; j < D; j++) {
-      const double2 m = M[z + k * D + j];
-      cmac(t[j], r[k], cd(m.x, m.y));
+      for (int j = 0; j < D; j++) {
+        const fx_d2 m = v[dk * D + j];
+        const double mi = DAG ? -m.y : m.y;
+        // out += rk m as four fused multiply-adds
+        double re = (!ACC && k == 0) ? 0.0 : out[j].re, im = (!ACC && k == 0) ? 0.0 : out[j].im;
+        re = __builtin_fma(rk.re, m.x, re);
+        re = __builtin_fma(-rk.im, mi, re);
+        im = __builtin_fma(rk.re, mi, im);
+        im = __builtin_fma(rk.im, m.x, im);
+        out[j] = cd(re, im);
+      }
     }
+    fx_vm<D, DAG, ACC, K0 + NK>(out, r, a, s);
   }
+}
+
+// wait until at most N vector memory operations are outstanding (the DMA
+// that filled the slot about to be read has landed: DMAs and loads complete
+// in issue order)
+template <int N>
+__device__ __forceinline__ void fx_vmcnt() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __asm__ volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
+  __asm__ volatile("" ::: "memory");
+}
+
+__device__ __forceinline__ unsigned fx_lds(const double2* p) {
+  return (unsigned)(size_t)(__attribute__((address_space(3))) const double2*)p;
+}
+
+// antiperiodic sign of the represented link (site with coordinates c, dir)
+// (bcs.cpp:264-272)
+__device__ __forceinline__ double fx_sgn(const int (&c)[4], int dir, const Geom& g) {
+  return (g.bc == BC_ANTIPERIODIC && dir == 0 && c[0] == 0) ? -1.0 : 1.0;
+}
+
+// coordinates of c + d e_dir (d = +-1, periodic wrap) and their EO index
+__device__ __forceinline__ int fx_nb(const int (&c)[4], int dir, int d, const Geom& g, int (&o)[4]) {
 #pragma unroll
-  for (int j = 0; j < D; j++) out[j] = ACC ? out[j] + s * t[j] : s * t[j];
+  for (int a = 0; a < 4; a++) o[a] = c[a];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+    if (a == dir) {
+      int v = o[a] + d;
+      v = v < 0 ? v + g.L[a] : (v >= g.L[a] ? v - g.L[a] : v);
+      o[a] = v;
+    }
+  return full_index(o[0], o[1], o[2], o[3], g);
 }
 
 template <int NG, int D>
-__global__ void __launch_bounds__(64) k_force_clover_rows(const double2* __restrict__ clf,
-                                                          const double2* __restrict__ links, double* __restrict__ mom,
-                                                          double coeff, Geom g, const double2* __restrict__ itr,
-                                                          int nu_index) {
-  constexpr int G = 64 / D;  // groups per wave
-  __shared__ double2 sbuf[G][2][D * D];
-  __shared__ double spart[64][NG];
+__global__ void __launch_bounds__(64) k_force_clover_x(const double2* __restrict__ clf,
+                                                       const double2* __restrict__ links, double* __restrict__ mom,
+                                                       double coeff, Geom g, const double2* __restrict__ itr,
+                                                       int order) {
+  typedef FX<D> F;
+  __shared__ __align__(16) double2 sl0[F::SLOT], sl1[F::SLOT], sl2[F::SLOT];
+  double2* const sl[3] = {sl0, sl1, sl2};
   const int lane = threadIdx.x;
-  const int gi = min(lane / D, G - 1);
+  const int gi = min(lane / D, F::G - 1);
   const int i = lane % D;
-  const int group = blockIdx.x * G + gi;
-  const bool on = lane < G * D && group < 4 * g.V;
-  // tail lanes compute on a valid site and write nothing
-  const int gq = group < 4 * g.V ? group : 4 * g.V - 1;
-  const int mu = gq / g.V, id = gq - mu * g.V;
-  const int nu = nu_index + (nu_index >= mu ? 1 : 0);
-  int c[4];
+  const int gD = gi * D;
+  const unsigned a0 = fx_lds(sl0) + 16 * gD, a1 = fx_lds(sl1) + 16 * gD, a2 = fx_lds(sl2) + 16 * gD;
+  // work item: (site block, mu), from a 2D grid (order < 0) or a 1D grid
+  // with mu fastest, remapped so that each XCD takes runs of `order` items
+  int sb, mu;
+  if (order < 0) { sb = blockIdx.x; mu = blockIdx.y; }
+  else { const int b = xcd_block(gridDim.x, order); sb = b >> 2; mu = b & 3; }
+  const int grp = sb * F::G + gi;
+  const bool on = lane < F::G * D && grp < g.V;
+  const int id = grp < g.V ? grp : g.V - 1;  // tail lanes compute on a valid site and write nothing
+  const size_t SL = g.VHpad, SZ = g.V;
+  int c[4], c1[4];
   fcoords(id, g, c);
-  const int o1 = fshift(c, mu, 1, g);  // x + mu
-  const int o2 = fshift(c, nu, 1, g);  // x + nu
-  const int o3 = fshift(c, nu, -1, g); // x - nu
-  int c3[4], c2[4];
-  fcoords(o3, g, c3);
-  fcoords(o2, g, c2);
-  const int o4 = fshift(c3, mu, 1, g); // x + mu - nu
-  const int o5 = fshift(c2, mu, 1, g); // x + mu + nu
-  int num, sign;
-  if (mu < nu) { num = nu * (nu - 1) / 2 + mu; sign = 1; }
-  else { num = mu * (mu - 1) / 2 + nu; sign = -1; }
-  const Op<D> W0 = link_op<D>(links, o3, mu, true, g), W1 = link_op<D>(links, o3, nu, false, g),
-              W2 = link_op<D>(links, o1, nu, false, g), W3 = link_op<D>(links, o2, mu, true, g),
-              W4 = link_op<D>(links, id, nu, true, g), W5 = link_op<D>(links, o4, nu, true, g);
-  const Op<D> Z0 = clf_op<D>(clf, num, id, g.V), Z1 = clf_op<D>(clf, num, o1, g.V),
-              Z2 = clf_op<D>(clf, num, o3, g.V), Z3 = clf_op<D>(clf, num, o4, g.V),
-              Z4 = clf_op<D>(clf, num, o5, g.V), Z5 = clf_op<D>(clf, num, o2, g.V);
-  const size_t SL = g.VHpad, SZ = g.V;  // strides of links and of clover-force matrices
-  // u = row i of U(x, mu), loaded again where it is used rather than kept live
-  const Op<D> U = link_op<D>(links, id, mu, false, g);
-  cd u[D];
-  auto load_u = [&]() {
-    double2 r[D];
-    fetch_row<D>(r, U, i, SL);
+  const int o1 = fx_nb(c, mu, 1, g, c1);  // x + mu
+  cd Ft[D], S[D], W8[D], W7[D], Q[D], T[D], A[D];
 #pragma unroll
-    for (int j = 0; j < D; j++) u[j] = cd(r[j].x, r[j].y);
-  };
-  double2* B0 = sbuf[gi][0];
-  double2* B1 = sbuf[gi][1];
-  double2 pre[D];
-  cd a[D], b[D], cc[D], d[D], e[D], s[D], t[D], acc[D];
-  // each step: stage the prefetched operand, fetch the next one, multiply
-#define YH_STEP(BUF, NEXT, STRIDE)  \
-  put_row<D>(BUF, pre, i, on);      \
-  __syncthreads();                  \
-  fetch_row<D>(pre, NEXT, i, STRIDE);
-  load_u();
-  fetch_row<D>(pre, W2, i, SL);
-  YH_STEP(B0, W3, SL) vm<D, false>(a, u, B0, 1.0);   // a = u W2
-  YH_STEP(B1, Z4, SZ) vm<D, false>(b, a, B1, 1.0);   // b = a W3
-  YH_STEP(B0, W3, SL) vm<D, false>(s, a, B0, 1.0);   // s = a Z4
-  YH_STEP(B1, Z5, SZ) vm<D, false>(t, s, B1, 1.0);   // t = a Z4 W3
-  YH_STEP(B0, W4, SL) vm<D, true>(t, b, B0, 1.0);    // t += b Z5
-  YH_STEP(B1, W5, SL)
-  vm<D, false>(cc, b, B1, 1.0);                      // c = b W4
-  vm<D, false>(acc, t, B1, 1.0);                     // acc = (a Z4 W3 + b Z5) W4
-  load_u();
-  YH_STEP(B0, W0, SL) vm<D, false>(d, u, B0, 1.0);   // d = u W5
-  YH_STEP(B1, W1, SL) vm<D, false>(e, d, B1, 1.0);   // e = d W0
-  YH_STEP(B0, Z0, SZ)
-  vm<D, false>(s, e, B0, 1.0);                       // s = e W1 (= u W5 W6)
-#pragma unroll
-  for (int j = 0; j < D; j++) cc[j] = cc[j] - s[j];  // c - f = u W8
-  YH_STEP(B1, Z2, SZ) vm<D, true>(acc, cc, B1, 1.0); // acc += u W8 Z0
-  YH_STEP(B0, Z3, SZ) vm<D, false>(s, e, B0, 1.0);   // s = e Z2
-  YH_STEP(B1, W0, SL) vm<D, false>(t, d, B1, 1.0);   // t = d Z3
-  YH_STEP(B0, W1, SL) vm<D, true>(s, t, B0, 1.0);    // s += d Z3 W0
-  YH_STEP(B1, Z1, SZ) vm<D, true>(acc, s, B1, -1.0); // acc -= (e Z2 + d Z3 W0) W1
-  load_u();
-  YH_STEP(B0, W2, SL) vm<D, false>(d, u, B0, 1.0);   // h = u Z1 (in d)
-  YH_STEP(B1, W3, SL) vm<D, false>(s, d, B1, 1.0);   // s = h W2
-  YH_STEP(B0, W4, SL) vm<D, false>(t, s, B0, 1.0);   // t = h W2 W3
-  YH_STEP(B1, W5, SL) vm<D, true>(acc, t, B1, 1.0);  // acc += h W2 W3 W4
-  YH_STEP(B0, W0, SL) vm<D, false>(s, d, B0, 1.0);   // s = h W5
-  YH_STEP(B1, W1, SL) vm<D, false>(t, s, B1, 1.0);   // t = h W5 W0
-  put_row<D>(B0, pre, i, on);
-  __syncthreads();
-  vm<D, true>(acc, t, B0, -1.0);                     // acc -= h W5 W0 W1
-#undef YH_STEP
-  // row i of U(x, mu) F, projected: sum_k Re (iT_a)_{k i} M_{i k}
-#pragma unroll
-  for (int q = 0; q < NG; q++) {
-    double re = 0.0;
-#pragma unroll
-    for (int k = 0; k < D; k++) {
-      const double2 tg = itr[(q * D + k) * D + i];
-      re += tg.x * acc[k].re - tg.y * acc[k].im;
-    }
-    spart[lane][q] = re;
+  for (int j = 0; j < D; j++) Ft[j] = cd(0, 0);
+  // Each step issues DMAs for later steps (in the order they are needed),
+  // waits for the DMA of its own slot by counting the DMA instructions
+  // issued after it (D per matrix), and multiplies.  The row operands are
+  // ordinary loads the compiler waits for.
+  // prologue: W3 = U(x + nu, mu)^+ and the row of W2 = U(x + mu, nu) for the first nu
+  {
+    const int nu = mu == 0 ? 1 : 0;
+    int cn[4];
+    fx_dma<D>(sl[0], fx_link<D>(links, fx_nb(c, nu, 1, g, cn), mu, g), i, SL);
+    fx_row<D, false>(A, fx_link<D>(links, o1, nu, g), i, SL);
   }
-  __syncthreads();
-  if (on && i == 0) {
-    const double f = -(sign * coeff * 0.5);
+#pragma unroll 1
+  for (int jn = 0; jn < 3; jn++) {
+    const int nu = jn + (jn >= mu ? 1 : 0);
+    int c2[4], c3[4], c4[4], c5[4];
+    const int o2 = fx_nb(c, nu, 1, g, c2);    // x + nu
+    const int o3 = fx_nb(c, nu, -1, g, c3);   // x - nu
+    const int o4 = fx_nb(c3, mu, 1, g, c4);   // x + mu - nu
+    const int o5 = fx_nb(c2, mu, 1, g, c5);   // x + mu + nu
+    int num;
+    double sign;
+    if (mu < nu) { num = nu * (nu - 1) / 2 + mu; sign = 1.0; }
+    else { num = mu * (mu - 1) / 2 + nu; sign = -1.0; }
+    // antiperiodic signs of W0..W5
+    const double s0 = fx_sgn(c3, mu, g), s1 = fx_sgn(c3, nu, g), s2 = fx_sgn(c1, nu, g),
+                 s3 = fx_sgn(c2, mu, g), s4 = fx_sgn(c, nu, g), s5 = fx_sgn(c4, nu, g);
+    // W7 = W2 W3                                (A: row of W2; W3 in s0)
+    fx_dma<D>(sl[1], fx_clf<D>(clf, num, fx_opq(o5), g), fx_opq(i), SZ);  // Z4
+    fx_dma<D>(sl[2], fx_clf<D>(clf, num, fx_opq(o2), g), fx_opq(i), SZ);  // Z5
+    fx_vmcnt<2 * D>();                                                      // W3
+    fx_vm<D, true, false>(W7, A, a0, s2 * s3);
+    fx_wait();
+    // T = W2 Z4                                                  Z4 s1
+    fx_vmcnt<D>();
+    fx_vm<D, false, false>(T, A, a1, s2);
+    fx_wait();
+    // Q = T W3                                                   W3 s0
+    fx_dma<D>(sl[1], fx_link<D>(links, fx_opq(id), nu, g), fx_opq(i), SL);  // W4
+    fx_vm<D, true, false>(Q, T, a0, s3);
+    fx_wait();
+    // Q += W7 Z5                                                 Z5 s2
+    fx_dma<D>(sl[0], fx_link<D>(links, fx_opq(o3), nu, g), fx_opq(i), SL);  // W1
+    fx_row<D, true>(A, fx_link<D>(links, fx_opq(o3), mu, g), fx_opq(i), SL);  // W0
+    fx_vmcnt<2 * D>();
+    fx_vm<D, false, true>(Q, W7, a2, 1.0);
+    fx_wait();
+    // F += sign Q W4                                             W4 s1
+    fx_dma<D>(sl[2], fx_clf<D>(clf, num, fx_opq(o3), g), fx_opq(i), SZ);  // Z2
+    fx_vmcnt<2 * D>();
+    fx_vm<D, true, true>(Ft, Q, a1, sign * s4);
+    fx_wait();
+    // W8 = W7 W4                                                 W4 s1
+    fx_vm<D, true, false>(W8, W7, a1, s4);
+    fx_wait();
+    // W6 = W0 W1 (in T)                                          W1 s0
+    fx_vmcnt<D>();
+    fx_vm<D, false, false>(T, A, a0, s0 * s1);
+    fx_wait();
+    // T1 = W0 Z2 (in T), W6 -> s1                                Z2 s2
+    fx_put<D>(sl[1], T, lane);
+    fx_vmcnt<0>();
+    fx_vm<D, false, false>(T, A, a2, s0);
+    fx_wait();
+    // S = T1 W1                                                  W1 s0
+    fx_dma<D>(sl[2], fx_clf<D>(clf, num, fx_opq(id), g), fx_opq(i), SZ);  // Z0
+    fx_row<D, true>(A, fx_link<D>(links, fx_opq(o4), nu, g), fx_opq(i), SL);  // W5
+    fx_vm<D, false, false>(S, T, a0, s1);
+    fx_wait();
+    // W8 -= W5 W6                                                W6 s1
+    fx_vm<D, false, true>(W8, A, a1, -s5);
+    fx_row<D, false>(A, fx_clf<D>(clf, num, fx_opq(o4), g), fx_opq(i), SZ);  // Z3
+    fx_wait();
+    // S += Z3 W6                                                 W6 s1
+    fx_vm<D, false, true>(S, A, a1, 1.0);
+    fx_row<D, false>(A, fx_clf<D>(clf, num, fx_opq(o1), g), fx_opq(i), SZ);  // Z1
+    fx_wait();
+    // F += sign W8 Z0; W8 -> s0, S -> s1                         Z0 s2
+    fx_put<D>(sl[0], W8, lane);
+    fx_put<D>(sl[1], S, lane);
+    fx_vmcnt<0>();
+    fx_vm<D, false, true>(Ft, W8, a2, sign);
+    fx_wait();
+    // F += sign Z1 W8                                            W8 s0
+    fx_vm<D, false, true>(Ft, A, a0, sign);
+    fx_row<D, true>(A, fx_link<D>(links, fx_opq(o4), nu, g), fx_opq(i), SL);  // W5
+    fx_wait();
+    // F -= sign W5 S                                             S s1
+    // with the next nu's W3 and W2 row, or after the last nu the row of U
+    if (jn < 2) {
+      const int nx = jn + 1 + (jn + 1 >= mu ? 1 : 0);
+      int cn[4];
+      fx_dma<D>(sl[0], fx_link<D>(links, fx_opq(fx_nb(c, nx, 1, g, cn)), mu, g), fx_opq(i), SL);
+      fx_vm<D, false, true>(Ft, A, a1, -sign * s5);
+      fx_row<D, false>(A, fx_link<D>(links, fx_opq(o1), nx, g), fx_opq(i), SL);
+    } else {
+      fx_vm<D, false, true>(Ft, A, a1, -sign * s5);
+      fx_row<D, false>(A, fx_link<D>(links, fx_opq(id), mu, g), fx_opq(i), SL);
+    }
+    fx_wait();
+  }
+  // M = U(x, mu) F, then the projection sum_{k,r} Re (iT_q)_{k r} M_{r k}
+  // with lane i of the group taking q = i, i + D, ...
+  fx_put<D>(sl[2], Ft, lane);
+  fx_wait();
+  fx_vm<D, false, false>(T, A, a2, fx_sgn(c, mu, g));
+  fx_put<D>(sl[0], T, lane);
+  fx_wait();
+  const double f = -coeff * 0.5;
 #pragma unroll
-    for (int q = 0; q < NG; q++) {
-      double sum = 0.0;
+  for (int q0 = 0; q0 < NG; q0 += D) {
+    const int q = q0 + i;
+    if (q < NG) {
+      double re = 0.0;
 #pragma unroll
-      for (int r = 0; r < D; r++) sum += spart[lane + r][q];
-      mom[(size_t)(mu * NG + q) * g.V + id] += f * sum;
+      for (int k = 0; k < D; k++)
+#pragma unroll
+        for (int r = 0; r < D; r++) {
+          const double2 tg = itr[(q * D + k) * D + r];
+          const double2 m = sl0[k * 64 + gD + r];
+          re += tg.x * m.x - tg.y * m.y;
+        }
+      if (on) mom[(size_t)(mu * NG + q) * g.V + id] += f * re;
     }
   }
 }
@@ -843,15 +1072,33 @@ int force_clover_core(Ctx* c, double dt) {
   if ((rc = force_alloc(c))) return rc;
   double coeff = c->csw * dt / 8.0;
   const double2* links = (const double2*)c->links[PREC_F64][RECON_18];
-  // one launch per nu (ascending), so each momentum gets its three terms in
-  // the reference's order
-  for (int j = 0; j < 3; j++) {
-    YH_FORCE_MODELS(const int groups = 64 / D_;
-                    dim3 grid((unsigned)((4L * c->g.V + groups - 1) / groups));
-                    hipLaunchKernelGGL((k_force_clover_rows<NG_, D_>), grid, dim3(64), 0, c->stream,
-                                       (const double2*)c->clf, links, c->mom, coeff, c->g, c->tab_itr, j));
-    YH_HIP(hipGetLastError());
-  }
+  YH_FORCE_MODELS(const int groups = 64 / D_;
+                  const unsigned nsb = (unsigned)((c->g.V + groups - 1) / groups);
+                  dim3 grid = c->force_order < 0 ? dim3(nsb, 4) : dim3(4 * nsb);
+                  hipLaunchKernelGGL((k_force_clover_x<NG_, D_>), grid, dim3(64), 0, c->stream,
+                                     (const double2*)c->clf, links, c->mom, coeff, c->g, c->tab_itr,
+                                     c->force_order));
+  YH_HIP(hipGetLastError());
+  return YH_OK;
+}
+
+// zero the time slices the open / SF boundaries clear (bcs.cpp:139-164) in
+// one half of an FP64 field
+__global__ void __launch_bounds__(256) k_zero_bc_half(double2* __restrict__ h, int parity, int ncomp, Geom g) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= g.VH) return;
+  int c[4];
+  site_coords(k, parity, g, c);
+  if (!bc_zero_slice(c[0], g)) return;
+  for (int v = 0; v < ncomp; v++) h[(size_t)v * g.VHpad + k] = make_double2(0.0, 0.0);
+}
+
+int zero_bc_half(Ctx* c, Field* f, int parity) {
+  if (c->g.bc != BC_OPEN && c->g.bc != BC_SF) return YH_OK;
+  if (f->prec != PREC_F64) { set_error("zero_bc_half: FP64 fields only"); return YH_ERR_ARG; }
+  hipLaunchKernelGGL(k_zero_bc_half, dim3((c->g.VH + 255) / 256), dim3(256), 0, c->stream, (double2*)f->half[parity],
+                     parity, 4 * c->dr, c->g);
+  YH_HIP(hipGetLastError());
   return YH_OK;
 }
 

@@ -108,6 +108,10 @@ int yh_wilson_fermion_force_q(yh_ctx* c, double dt, double mass, double residue,
   Field* tmp = tmp_field(c, PREC_F64, 7, &rc);
   if (rc) return rc;
   if ((rc = apply_op(c, PREC_F64, RECON_18, YH_MVM_HPHI, mass, tmp, rhs, rhs, -1, nullptr))) return rc;
+  // the reference's Hphi(tmp, rhs) leaves the boundary slices of rhs zeroed
+  // (dirac.cpp:700 on the in-place odd half), and the force's Dphi_oe then
+  // reads that zeroed even half
+  if ((rc = zero_bc_half(c, rhs, 0))) return rc;
   return yh_wilson_fermion_force(c, dt, mass, residue, static_cast<yh_field*>(tmp), rhs);
 }
 

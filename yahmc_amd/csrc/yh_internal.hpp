@@ -101,6 +101,7 @@ struct Ctx {
   // hop kernels (D_R <= 3) of the precisions in this mask (bit = precision)
   // read y / z neighbours from an LDS-staged window
   int hop_stage = (1 << PREC_F64) | (1 << PREC_F32);
+  int force_order = 0;  // YH_TUNE_FORCE_ORDER
   // solver chunks are captured once into a HIP graph and replayed
   int use_graphs = 1;
   // model tables of this context's (nc, representation), in device memory
@@ -186,6 +187,7 @@ int force_clover_site(Ctx* c, double residue, const Field* lhs, const Field* rhs
 int force_logdet(Ctx* c, double mass, double coeff);
 int force_clover_core(Ctx* c, double dt);
 int clover_force_zero(Ctx* c);
+int zero_bc_half(Ctx* c, Field* f, int parity);
 int momenta_upload(Ctx* c, const double* host);
 int momenta_download(Ctx* c, double* host);
 int clover_force_download(Ctx* c, double* host);
