@@ -202,79 +202,6 @@ __device__ __forceinline__ int shift(const int c[4], int mu, int d, const Geom& 
   return full_index(n[0], n[1], n[2], n[3], g);
 }
 
-// sum of the four plaquette leaves at x in the (mu, nu) plane, represented
-// (library/wilsonloop.cpp:5-31)
-template <int NC, int REPR, int D>
-__device__ void leaf_sum(cd (&Q)[D][D], const double* fund, int id, const int c[4], int mu, int nu, const Geom& g,
-                         const double2* __restrict__ basis) {
-  cd A[NC][NC], B[NC][NC], T1[NC][NC], T2[NC][NC], L[NC][NC], R[D][D];
-#pragma unroll
-  for (int i = 0; i < D; i++)
-#pragma unroll
-    for (int j = 0; j < D; j++) Q[i][j] = cd(0, 0);
-  int xm = shift(c, mu, 1, g), xn = shift(c, nu, 1, g);
-  int bm = shift(c, mu, -1, g), bn = shift(c, nu, -1, g);
-  int cb[4];
-  coords_of(bm, g, cb);
-  int bm_pn = shift(cb, nu, 1, g);   // x - mu + nu
-  int bm_mn = shift(cb, nu, -1, g);  // x - mu - nu
-  int cn[4];
-  coords_of(bn, g, cn);
-  int bn_pm = shift(cn, mu, 1, g);   // x - nu + mu
-  // leaf 0: U(x,mu) U(x+mu,nu) [U(x,nu) U(x+nu,mu)]^+
-  load_fund<NC>(A, fund, id, mu);
-  load_fund<NC>(B, fund, xm, nu);
-  mm<NC>(T1, A, B);
-  load_fund<NC>(A, fund, id, nu);
-  load_fund<NC>(B, fund, xn, mu);
-  mm<NC>(T2, A, B);
-  mmd<NC>(L, T1, T2);
-  represent<NC, REPR, D>(R, L, basis);
-#pragma unroll
-  for (int i = 0; i < D; i++)
-#pragma unroll
-    for (int j = 0; j < D; j++) Q[i][j] = Q[i][j] + R[i][j];
-  // leaf 1: U(x,nu) [U(x-mu,nu) U(x-mu+nu,mu)]^+ U(x-mu,mu)
-  load_fund<NC>(A, fund, bm, nu);
-  load_fund<NC>(B, fund, bm_pn, mu);
-  mm<NC>(T1, A, B);
-  load_fund<NC>(A, fund, id, nu);
-  mmd<NC>(T2, A, T1);
-  load_fund<NC>(A, fund, bm, mu);
-  mm<NC>(L, T2, A);
-  represent<NC, REPR, D>(R, L, basis);
-#pragma unroll
-  for (int i = 0; i < D; i++)
-#pragma unroll
-    for (int j = 0; j < D; j++) Q[i][j] = Q[i][j] + R[i][j];
-  // leaf 2: [U(x-mu-nu,nu) U(x-mu,mu)]^+ U(x-mu-nu,mu) U(x-nu,nu)
-  load_fund<NC>(A, fund, bm_mn, nu);
-  load_fund<NC>(B, fund, bm, mu);
-  mm<NC>(T1, A, B);
-  load_fund<NC>(A, fund, bm_mn, mu);
-  mdm<NC>(T2, T1, A);
-  load_fund<NC>(A, fund, bn, nu);
-  mm<NC>(L, T2, A);
-  represent<NC, REPR, D>(R, L, basis);
-#pragma unroll
-  for (int i = 0; i < D; i++)
-#pragma unroll
-    for (int j = 0; j < D; j++) Q[i][j] = Q[i][j] + R[i][j];
-  // leaf 3: U(x-nu,nu)^+ U(x-nu,mu) U(x-nu+mu,nu) U(x,mu)^+
-  load_fund<NC>(A, fund, bn, nu);
-  load_fund<NC>(B, fund, bn, mu);
-  mdm<NC>(T1, A, B);
-  load_fund<NC>(A, fund, bn_pm, nu);
-  mm<NC>(T2, T1, A);
-  load_fund<NC>(A, fund, id, mu);
-  mmd<NC>(L, T2, A);
-  represent<NC, REPR, D>(R, L, basis);
-#pragma unroll
-  for (int i = 0; i < D; i++)
-#pragma unroll
-    for (int j = 0; j < D; j++) Q[i][j] = Q[i][j] + R[i][j];
-}
-
 // packed Hermitian store: n reals then strictly-lower triangle
 template <int N>
 __device__ __forceinline__ void store_packed(double2* base, int stride, int site, int vec0, const cd (&M)[N][N]) {
@@ -312,140 +239,340 @@ __device__ __forceinline__ void load_packed(cd (&M)[N][N], const double2* base, 
     }
 }
 
-// clover term per site -> packed blocks [parity][2 blocks * n^2/2 vec][site]
+// ---------------------------------------------------------------------------
+// Clover term per site -> packed blocks [parity][2 blocks * n^2/2 vec][site]
 // (library/clover.cpp:120-168; the dense form is -(c_sw/16) sum_{mu<nu}
-//  (1/2)[g_mu, g_nu] (x) (Q_mu_nu - Q_mu_nu^+))
+// (1/2)[g_mu, g_nu] (x) (Q_mu_nu - Q_mu_nu^+), Q_mu_nu the represented sum
+// of the four plaquette leaves, wilsonloop.cpp:5-31).  A group of D_R lanes
+// per site, lane i computing row i of every D_R x D_R matrix, so that no
+// lane holds more than a few rows.  Per plane, lanes 0..3 of the group each
+// form one leaf in the fundamental as a product of four links,
+// L = F1 F2 F3 F4, and put it in LDS; every lane then adds row i of R(L) for
+// the four leaves:
+//   fundamental: R = L
+//   two-index:   R[i][m] = sum_cd (b_m)_cd Y_cd,  Y = L^T conj(b_i) L
+//   adjoint:     R[i][m] = sum_cd Y_cd (b_m)_dc,  Y = L^+ b_i L
+// (both follow from represent()).  The at/bt/ct/dt combinations need column
+// i of the plane sum too, which the group exchanges through LDS.  The block
+// is one wave; LDS traffic needs only compiler fences (a wave's LDS
+// operations execute in order).
+__device__ __forceinline__ void cb_fence() { __asm__ volatile("" ::: "memory"); }
+
 template <int NC, int REPR, int D>
-__global__ void k_clover_build(const double* __restrict__ fund, double2* __restrict__ out, Geom g, double csw,
-                               const double2* __restrict__ basis) {
-  int id = blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= g.V) return;
-  constexpr int N = 2 * D;
+__global__ void __launch_bounds__(64) k_clover_build_rows(const double* __restrict__ fund, double2* __restrict__ out,
+                                                         Geom g, double csw, const double2* __restrict__ basis) {
+  constexpr int G = 64 / D, N = 2 * D, NN = NC * NC;
+  __shared__ double2 lv[G][4][NN];  // the plane's four leaves
+  __shared__ double2 qx[G][D * D];  // row exchange
+  __shared__ double2 bl[D * 16];    // the representation basis
+  const int lane = threadIdx.x;
+  for (int e = lane; e < D * 16; e += 64) bl[e] = basis[e];
+  cb_fence();
+  const int gi = min(lane / D, G - 1);
+  const int i = lane % D;
+  const int grp = blockIdx.x * G + gi;
+  const bool on = lane < G * D && grp < g.V;
+  const int id = grp < g.V ? grp : g.V - 1;  // tail lanes compute on a valid site and write nothing
   int c[4];
-  coords_of(id, g, c);
-  int p = id >= g.VH;
-  int k = id - p * g.VH;
+  full_coords(id, g, c);
+  bool zero = csw == 0.0;
+  if (g.bc == BC_SF) zero = zero || c[0] == 0 || c[0] == g.L[0] - 1 || c[0] == g.L[0] - 2;
+  if (g.bc == BC_OPEN) zero = zero || c[0] == 0 || c[0] == g.L[0] - 1;
+  cd r1[D], r2[D];  // at, bt rows, then ct, dt rows
+  const double cc = -csw / 16.0;
+  const int p = id >= g.VH, k = id - p * g.VH;
   double2* base = out + (size_t)p * (N * N) * g.VHpad;
-  cd at[D][D], bt[D][D], ct[D][D], dt[D][D], Q[D][D];
-  bool zero = false;
-  if (g.bc == BC_SF) zero = (c[0] == 0 || c[0] == g.L[0] - 1 || c[0] == g.L[0] - 2);
-  if (g.bc == BC_OPEN) zero = (c[0] == 0 || c[0] == g.L[0] - 1);
-  if (csw == 0.0) zero = true;
-  if (zero) {
-    for (int v = 0; v < N * N; v++) base[(size_t)v * g.VHpad + k] = make_double2(0, 0);
-    return;
+  // output addresses are formed where they are stored (from an opaque base:
+  // hoisted out of the plane loop they would hold dozens of registers)
+  double2* ob = base;
+  auto put_re = [&](int vec0, int o, double v) {
+    if (on) ((double*)(ob + (size_t)(vec0 + o / 2) * g.VHpad + k))[o & 1] = v;
+  };
+  auto put_cx = [&](int vec0, int o, cd v) {  // o even
+    if (on) ob[(size_t)(vec0 + o / 2) * g.VHpad + k] = make_double2(v.re, v.im);
+  };
+  auto lo = [](int r, int s) { return N + 2 * (r * (r - 1) / 2 + s); };  // packed index of (r, s), r > s
+  // planes 03, 12, 01, 02, 13, 23 (clover.cpp:135-156)
+#pragma unroll 1
+  for (int pl = 0; pl < 6; pl++) {
+    const int mu = pl == 0 ? 0 : pl == 1 ? 1 : pl == 2 ? 0 : pl == 3 ? 0 : pl == 4 ? 1 : 2;
+    const int nu = pl == 0 ? 3 : pl == 1 ? 2 : pl == 2 ? 1 : pl == 3 ? 2 : pl == 4 ? 3 : 3;
+    // leaves: lane q < 4 of the group forms leaf q
+    for (int q = i; q < 4; q += D) {
+      int xm = shift(c, mu, 1, g), xn = shift(c, nu, 1, g);
+      int bm = shift(c, mu, -1, g), bn = shift(c, nu, -1, g);
+      int cb[4], cn[4];
+      coords_of(bm, g, cb);
+      coords_of(bn, g, cn);
+      const int bm_pn = shift(cb, nu, 1, g), bm_mn = shift(cb, nu, -1, g), bn_pm = shift(cn, mu, 1, g);
+      // the four leaves as products F1 F2 F3 F4 (leaf 0: U(x,mu) U(x+mu,nu)
+      // U(x+nu,mu)^+ U(x,nu)^+, and the others around x)
+      int s[4], d[4];
+      bool h[4];
+      if (q == 0) {
+        s[0] = id; d[0] = mu; h[0] = false; s[1] = xm; d[1] = nu; h[1] = false;
+        s[2] = xn; d[2] = mu; h[2] = true; s[3] = id; d[3] = nu; h[3] = true;
+      } else if (q == 1) {
+        s[0] = id; d[0] = nu; h[0] = false; s[1] = bm_pn; d[1] = mu; h[1] = true;
+        s[2] = bm; d[2] = nu; h[2] = true; s[3] = bm; d[3] = mu; h[3] = false;
+      } else if (q == 2) {
+        s[0] = bm; d[0] = mu; h[0] = true; s[1] = bm_mn; d[1] = nu; h[1] = true;
+        s[2] = bm_mn; d[2] = mu; h[2] = false; s[3] = bn; d[3] = nu; h[3] = false;
+      } else {
+        s[0] = bn; d[0] = nu; h[0] = true; s[1] = bn; d[1] = mu; h[1] = false;
+        s[2] = bn_pm; d[2] = nu; h[2] = false; s[3] = id; d[3] = mu; h[3] = true;
+      }
+      // L = F1 F2 F3 F4, each factor a link or its dagger, multiplied in
+      // place row by row
+      cd L[NC][NC], F[NC][NC];
+      auto ldf = [&](cd (&u)[NC][NC], int st, int dr, bool dg) {
+        const double* pf = fund + (size_t)(4 * st + dr) * 2 * NC * NC;
+#pragma unroll
+        for (int a = 0; a < NC; a++)
+#pragma unroll
+          for (int b = 0; b < NC; b++) {
+            const int e = dg ? b * NC + a : a * NC + b;
+            u[a][b] = cd(pf[e], dg ? -pf[NC * NC + e] : pf[NC * NC + e]);
+          }
+      };
+      ldf(L, s[0], d[0], h[0]);
+#pragma unroll
+      for (int f = 1; f < 4; f++) {
+        ldf(F, s[f], d[f], h[f]);
+#pragma unroll
+        for (int a = 0; a < NC; a++) {
+          cd row[NC];
+#pragma unroll
+          for (int b = 0; b < NC; b++) {
+            cd t(0, 0);
+#pragma unroll
+            for (int kk = 0; kk < NC; kk++) cmac(t, L[a][kk], F[kk][b]);
+            row[b] = t;
+          }
+#pragma unroll
+          for (int b = 0; b < NC; b++) L[a][b] = row[b];
+        }
+      }
+#pragma unroll
+      for (int a = 0; a < NC; a++)
+#pragma unroll
+        for (int b = 0; b < NC; b++) lv[gi][q][a * NC + b] = make_double2(L[a][b].re, L[a][b].im);
+    }
+    cb_fence();
+    // row i of the plane sum Q = sum_leaves R(L)
+    cd qr[D];
+#pragma unroll
+    for (int m = 0; m < D; m++) qr[m] = cd(0, 0);
+#pragma unroll 1
+    for (int q = 0; q < 4 && REPR == YH_REPR_FUNDAMENTAL; q++) {
+      // R(L) = L: row i straight from LDS
+#pragma unroll
+      for (int m = 0; m < D; m++) {
+        const double2 v = lv[gi][q][i * NC + m];
+        qr[m] = qr[m] + cd(v.x, v.y);
+      }
+    }
+#pragma unroll 1
+    for (int q = 0; q < 4 && REPR != YH_REPR_FUNDAMENTAL; q++) {
+      cd L[NC][NC];
+#pragma unroll
+      for (int a = 0; a < NC; a++)
+#pragma unroll
+        for (int b = 0; b < NC; b++) {
+          const double2 v = lv[gi][q][a * NC + b];
+          L[a][b] = cd(v.x, v.y);
+        }
+      // column dd of T = b_i' L and of Y = L^T T (two-index, b_i' = conj b_i)
+      // or L^+ T (adjoint), and its terms of row i of R(L)
+#pragma unroll
+      for (int dd = 0; dd < NC; dd++) {
+        // basis from LDS through an opaque zero per column, so that the
+        // compiler neither hoists the reads out of the loops nor issues them
+        // all at once
+        int z = 0;
+        __asm__ volatile("" : "+v"(z));
+        const double2* bs = bl + z;
+        cd tc[NC], yc[NC];
+#pragma unroll
+        for (int a = 0; a < NC; a++) {
+          cd t(0, 0);
+#pragma unroll
+          for (int b = 0; b < NC; b++) {
+            const double2 v = bs[i * 16 + a * NC + b];
+            cmac(t, REPR == YH_REPR_ADJOINT ? cd(v.x, v.y) : cd(v.x, -v.y), L[b][dd]);
+          }
+          tc[a] = t;
+        }
+#pragma unroll
+        for (int cc2 = 0; cc2 < NC; cc2++) {
+          cd y(0, 0);
+#pragma unroll
+          for (int a = 0; a < NC; a++) {
+            if constexpr (REPR == YH_REPR_ADJOINT) cmac_conj(y, L[a][cc2], tc[a]);
+            else cmac(y, L[a][cc2], tc[a]);
+          }
+          yc[cc2] = y;
+        }
+#pragma unroll
+        for (int m = 0; m < D; m++) {
+#pragma unroll
+          for (int cc2 = 0; cc2 < NC; cc2++) {
+            const double2 bb = REPR == YH_REPR_ADJOINT ? bs[m * 16 + dd * NC + cc2] : bs[m * 16 + cc2 * NC + dd];
+            cmac(qr[m], yc[cc2], cd(bb.x, bb.y));
+          }
+        }
+      }
+    }
+    // column i through LDS
+    cb_fence();
+#pragma unroll
+    for (int m = 0; m < D; m++) qx[gi][i * D + m] = make_double2(qr[m].re, qr[m].im);
+    cb_fence();
+    cd qc[D];
+#pragma unroll
+    for (int m = 0; m < D; m++) {
+      const double2 v = qx[gi][m * D + i];
+      qc[m] = cd(v.x, v.y);
+    }
+    cb_fence();
+    // Q[i][j] = qr[j], Q[j][i] = qc[j] (clover.cpp:149-156)
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+      const cd a = qr[j], b = qc[j];
+      if (pl == 0) r1[j] = cd(a.im + b.im, b.re - a.re);                    // at
+      else if (pl == 1) r2[j] = cd(a.im + b.im, a.re - b.re);               // bt
+      else if (pl == 2) r1[j] = cd(a.im + b.im, b.re - a.re);               // ct
+      else if (pl == 3) r1[j] = r1[j] + cd(-a.re + b.re, -a.im - b.im);     // ct
+      else if (pl == 4) r2[j] = cd(a.re - b.re, a.im + b.im);               // dt
+      else r2[j] = r2[j] + cd(a.im + b.im, -a.re + b.re);                   // dt
+    }
+    if (pl == 1 || pl == 5) {
+      int z = 0;
+      __asm__ volatile("" : "+v"(z));
+      ob = base + z;
+    }
+    if (pl == 1) {
+      // A = cc (at - bt) (block 0, diagonal blocks), C (block 1)
+#pragma unroll
+      for (int j = 0; j < D; j++) {
+        const cd at = r1[j], bt = r2[j];
+        cd A(cc * (at.re - bt.re), cc * (at.im + bt.im));
+        cd C(-cc * (at.re + bt.re), cc * (bt.im - at.im));
+        if (zero) A = C = cd(0, 0);
+        if (j == i) {
+          put_re(0, i, A.re);
+          put_re(0, D + i, -A.re);
+          put_re(N * N / 2, i, C.re);
+          put_re(N * N / 2, D + i, -C.re);
+        } else if (j < i) {
+          put_cx(0, lo(i, j), A);
+          put_cx(0, lo(D + i, D + j), cd(-A.re, -A.im));
+          put_cx(N * N / 2, lo(i, j), C);
+          put_cx(N * N / 2, lo(D + i, D + j), cd(-C.re, -C.im));
+        }
+      }
+    } else if (pl == 5) {
+      // B (block 0) and D (block 1) off-diagonal blocks, stored as their
+      // conjugates in the lower left
+#pragma unroll
+      for (int j = 0; j < D; j++) {
+        const cd ct = r1[j], dt = r2[j];
+        cd B(cc * (ct.re - dt.re), cc * (ct.im - dt.im));
+        cd Dm(-cc * (ct.re + dt.re), -cc * (ct.im + dt.im));
+        if (zero) B = Dm = cd(0, 0);
+        put_cx(0, lo(D + j, i), conj(B));
+        put_cx(N * N / 2, lo(D + j, i), conj(Dm));
+      }
+    }
   }
-  // plane order of clover.cpp:135-140: 01, 02, 03, 12, 13, 23
-  // at: from Q03; bt: Q12; ct: Q01, Q02; dt: Q13, Q23 (clover.cpp:149-156)
-  leaf_sum<NC, REPR, D>(Q, fund, id, c, 0, 3, g, basis);
-#pragma unroll
-  for (int i = 0; i < D; i++)
-#pragma unroll
-    for (int j = 0; j < D; j++) at[i][j] = cd(Q[i][j].im + Q[j][i].im, Q[j][i].re - Q[i][j].re);
-  leaf_sum<NC, REPR, D>(Q, fund, id, c, 1, 2, g, basis);
-#pragma unroll
-  for (int i = 0; i < D; i++)
-#pragma unroll
-    for (int j = 0; j < D; j++) bt[i][j] = cd(Q[i][j].im + Q[j][i].im, Q[i][j].re - Q[j][i].re);
-  leaf_sum<NC, REPR, D>(Q, fund, id, c, 0, 1, g, basis);
-#pragma unroll
-  for (int i = 0; i < D; i++)
-#pragma unroll
-    for (int j = 0; j < D; j++) ct[i][j] = cd(Q[i][j].im + Q[j][i].im, Q[j][i].re - Q[i][j].re);
-  leaf_sum<NC, REPR, D>(Q, fund, id, c, 0, 2, g, basis);
-#pragma unroll
-  for (int i = 0; i < D; i++)
-#pragma unroll
-    for (int j = 0; j < D; j++) ct[i][j] = ct[i][j] + cd(-Q[i][j].re + Q[j][i].re, -Q[i][j].im - Q[j][i].im);
-  leaf_sum<NC, REPR, D>(Q, fund, id, c, 1, 3, g, basis);
-#pragma unroll
-  for (int i = 0; i < D; i++)
-#pragma unroll
-    for (int j = 0; j < D; j++) dt[i][j] = cd(Q[i][j].re - Q[j][i].re, Q[i][j].im + Q[j][i].im);
-  leaf_sum<NC, REPR, D>(Q, fund, id, c, 2, 3, g, basis);
-#pragma unroll
-  for (int i = 0; i < D; i++)
-#pragma unroll
-    for (int j = 0; j < D; j++) dt[i][j] = dt[i][j] + cd(Q[i][j].im + Q[j][i].im, -Q[i][j].re + Q[j][i].re);
-  double cc = -csw / 16.0;
-  cd M[N][N];
-  // block 0 = [[A, B], [B^+, -A]], block 1 = [[C, D], [D^+, -C]]  (dirac.cpp:63-89)
-#pragma unroll
-  for (int i = 0; i < D; i++)
-#pragma unroll
-    for (int j = 0; j < D; j++) {
-      cd A(cc * (at[i][j].re - bt[i][j].re), cc * (at[i][j].im + bt[i][j].im));
-      cd B(cc * (ct[i][j].re - dt[i][j].re), cc * (ct[i][j].im - dt[i][j].im));
-      M[i][j] = A;
-      M[D + i][D + j] = cd(-A.re, -A.im);
-      M[i][D + j] = B;
-      M[D + j][i] = conj(B);
-    }
-  store_packed<N>(base, g.VHpad, k, 0, M);
-#pragma unroll
-  for (int i = 0; i < D; i++)
-#pragma unroll
-    for (int j = 0; j < D; j++) {
-      cd C(-cc * (at[i][j].re + bt[i][j].re), cc * (bt[i][j].im - at[i][j].im));
-      cd Dm(-cc * (ct[i][j].re + dt[i][j].re), -cc * (ct[i][j].im + dt[i][j].im));
-      M[i][j] = C;
-      M[D + i][D + j] = cd(-C.re, -C.im);
-      M[i][D + j] = Dm;
-      M[D + j][i] = conj(Dm);
-    }
-  store_packed<N>(base, g.VHpad, k, N * N / 2, M);
 }
 
-// odd-site inverse of sigma + c_f shift + clover, per chiral block, through
-// the reference's unpivoted LDL^+ (solver.cpp:97-117); also log det
+// odd-site inverse of sigma + c_f shift + clover, per chiral block, and its
+// log det (the reference factors with an unpivoted LDL^+, solver.cpp:97-117).
+// A group of N = 2 D_R lanes per odd site, lane r holding row r of the
+// block: in-place Gauss-Jordan without pivoting, whose pivots are the d_i of
+// the unpivoted LDL^+ (both are the ratios of the leading principal minors),
+// so log det = sum log |d_i|.  Each
+// step's normalised pivot row reaches the group through LDS (the block is
+// one wave: a wave's LDS operations execute in order, compiler fences
+// suffice).  The two chiral blocks of a site run one after the other in
+// the same group, which then sums the 2N log |d_i| in fixed order.
 template <int D>
-__global__ void k_clover_invert(const double2* __restrict__ clov, double2* __restrict__ inv, double* __restrict__ logdet,
-                                Geom g, double sigma) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= g.VH) return;
-  constexpr int N = 2 * D;
+__global__ void __launch_bounds__(64) k_clover_invert_rows(const double2* __restrict__ clov, double2* __restrict__ inv,
+                                                          double* __restrict__ logdet, Geom g, double sigma) {
+  constexpr int N = 2 * D, G = 64 / N;
+  __shared__ double2 pr[G][N];   // pivot row
+  __shared__ double lds_ld[G][N];
+  const int lane = threadIdx.x;
+  const int gi = min(lane / N, G - 1);
+  const int r = lane % N;
+  const int grp = blockIdx.x * G + gi;
+  const bool on = lane < G * N && grp < g.VH;
+  const int k = grp < g.VH ? grp : g.VH - 1;  // tail lanes compute on a valid site and write nothing
   int c[4];
   site_coords(k, 1, g, c);
-  double s = sigma + bc_cf_shift(c[0], g);
+  const double s = sigma + bc_cf_shift(c[0], g);
   const double2* src = clov + (size_t)1 * (N * N) * g.VHpad;  // odd parity
+  auto lo = [](int a, int b) { return N + 2 * (a * (a - 1) / 2 + b); };  // packed real index of (a, b), a > b
+  auto get = [&](const double2* base, int vec0, int o) -> double {
+    return ((const double*)(base + (size_t)(vec0 + o / 2) * g.VHpad + k))[o & 1];
+  };
   double ld = 0.0;
+#pragma unroll 1
   for (int blk = 0; blk < 2; blk++) {
-    cd A[N][N];
-    load_packed<N>(A, src, g.VHpad, k, blk * N * N / 2);
-    for (int i = 0; i < N; i++) A[i][i].re += s;
-    // LDL^+: A = L D L^+, L unit lower, D real
-    for (int i = 0; i < N; i++) {
-      double d = A[i][i].re;
-      for (int q = 0; q < i; q++) d -= (A[i][q].re * A[i][q].re + A[i][q].im * A[i][q].im) * A[q][q].re;
-      A[i][i] = cd(d, 0);
-      for (int j = i + 1; j < N; j++) {
-        cd v = A[j][i];
-        for (int q = 0; q < i; q++) {
-          cd t = A[j][q] * conj(A[i][q]);
-          v = v - A[q][q].re * t;
+    const int vec0 = blk * N * N / 2;
+    // row r of sigma + clover
+    cd a[N];
+#pragma unroll
+    for (int j = 0; j < N; j++) {
+      if (j == r) a[j] = cd(get(src, vec0, r) + s, 0.0);
+      else if (j < r) a[j] = cd(get(src, vec0, lo(r, j)), get(src, vec0, lo(r, j) + 1));
+      else a[j] = cd(get(src, vec0, lo(j, r)), -get(src, vec0, lo(j, r) + 1));
+    }
+#pragma unroll
+    for (int kk = 0; kk < N; kk++) {
+      if (r == kk) {
+        const double d = a[kk].re;  // real for a Hermitian block, as the LDL^+ d
+        const double ip = 1.0 / d;
+        ld += log(fabs(d));
+#pragma unroll
+        for (int j = 0; j < N; j++) a[j] = j == kk ? cd(ip, 0.0) : ip * a[j];
+#pragma unroll
+        for (int j = 0; j < N; j++) pr[gi][j] = make_double2(a[j].re, a[j].im);
+      }
+      __asm__ volatile("" ::: "memory");
+      if (r != kk) {
+        const cd f = a[kk];
+#pragma unroll
+        for (int j = 0; j < N; j++) {
+          const double2 v = pr[gi][j];
+          const cd pj(v.x, v.y);
+          a[j] = j == kk ? cd(0.0, 0.0) - f * pj : a[j] - f * pj;
         }
-        A[j][i] = (1.0 / d) * v;
       }
+      __asm__ volatile("" ::: "memory");
     }
-    double prod = 1.0;
-    for (int i = 0; i < N; i++) prod *= A[i][i].re;
-    ld += log(fabs(prod));
-    // explicit inverse column by column
-    cd X[N][N];
-    for (int col = 0; col < N; col++) {
-      cd x[N];
-      for (int i = 0; i < N; i++) x[i] = cd(i == col ? 1.0 : 0.0, 0);
-      for (int i = 1; i < N; i++)
-        for (int q = 0; q < i; q++) x[i] = x[i] - A[i][q] * x[q];
-      for (int i = N - 1; i >= 0; i--) {
-        x[i] = (1.0 / A[i][i].re) * x[i];
-        for (int q = i + 1; q < N; q++) x[i] = x[i] - conj(A[q][i]) * x[q];
-      }
-      for (int i = 0; i < N; i++) X[i][col] = x[i];
+    // row r of the inverse: diagonal and strictly lower part, packed
+    if (on) {
+      double2* base = inv;
+      double dr = 0.0;  // a[r] without indexing the register array by a lane value
+#pragma unroll
+      for (int j = 0; j < N; j++) dr = j == r ? a[j].re : dr;
+      ((double*)(base + (size_t)(vec0 + r / 2) * g.VHpad + k))[r & 1] = dr;
+#pragma unroll
+      for (int j = 0; j < N; j++)
+        if (j < r) base[(size_t)(vec0 + lo(r, j) / 2) * g.VHpad + k] = make_double2(a[j].re, a[j].im);
     }
-    store_packed<N>(inv, g.VHpad, k, blk * N * N / 2, X);
   }
-  logdet[k] = ld;
+  // log det of both blocks: lane r holds the log |d| of pivot r of each
+  lds_ld[gi][r] = ld;
+  __asm__ volatile("" ::: "memory");
+  if (r == 0) {
+    double t = 0.0;
+#pragma unroll
+    for (int j = 0; j < N; j++) t += lds_ld[gi][j];
+    if (on) logdet[k] = t;
+  }
 }
 
 // recon-18 F64 -> other link precisions
@@ -608,8 +735,9 @@ int clover_build(Ctx* c) {
   c->cl_sigma = 0xF00F;  // invalidate the inverse (clover.cpp:337)
   int rc = YH_OK;
   auto launch = [&]() -> int {
-    YH_ALL_MODELS(hipLaunchKernelGGL((k_clover_build<NC_, REPR_, D_>), dim3(grid_bs(c->g.V, 128)), dim3(128), 0, c->stream,
-                                     c->fund, out, c->g, c->csw, c->tab_basis);
+    // a group of D_R lanes per site
+    YH_ALL_MODELS(hipLaunchKernelGGL((k_clover_build_rows<NC_, REPR_, D_>), dim3(grid_bs(c->g.V, 64 / D_)), dim3(64), 0,
+                                     c->stream, c->fund, out, c->g, c->csw, c->tab_basis);
                   YH_HIP(hipGetLastError()));
     set_error("unsupported (nc, repr) model");
     return YH_ERR_UNSUPPORTED;
@@ -663,11 +791,11 @@ int clover_invert(Ctx* c, double sigma) {
   const double2* src = (const double2*)c->clov[PREC_F64];
   double2* inv = (double2*)c->clinv[PREC_F64];
   switch (c->dr) {
-    case 2: hipLaunchKernelGGL(k_clover_invert<2>, dim3(grid_bs(c->g.VH, 64)), dim3(64), 0, c->stream, src, inv, c->logdet_site, c->g, sigma); break;
-    case 3: hipLaunchKernelGGL(k_clover_invert<3>, dim3(grid_bs(c->g.VH, 64)), dim3(64), 0, c->stream, src, inv, c->logdet_site, c->g, sigma); break;
-    case 4: hipLaunchKernelGGL(k_clover_invert<4>, dim3(grid_bs(c->g.VH, 64)), dim3(64), 0, c->stream, src, inv, c->logdet_site, c->g, sigma); break;
-    case 6: hipLaunchKernelGGL(k_clover_invert<6>, dim3(grid_bs(c->g.VH, 64)), dim3(64), 0, c->stream, src, inv, c->logdet_site, c->g, sigma); break;
-    case 8: hipLaunchKernelGGL(k_clover_invert<8>, dim3(grid_bs(c->g.VH, 64)), dim3(64), 0, c->stream, src, inv, c->logdet_site, c->g, sigma); break;
+    case 2: hipLaunchKernelGGL(k_clover_invert_rows<2>, dim3(grid_bs(c->g.VH, 64 / (2 * 2))), dim3(64), 0, c->stream, src, inv, c->logdet_site, c->g, sigma); break;
+    case 3: hipLaunchKernelGGL(k_clover_invert_rows<3>, dim3(grid_bs(c->g.VH, 64 / (2 * 3))), dim3(64), 0, c->stream, src, inv, c->logdet_site, c->g, sigma); break;
+    case 4: hipLaunchKernelGGL(k_clover_invert_rows<4>, dim3(grid_bs(c->g.VH, 64 / (2 * 4))), dim3(64), 0, c->stream, src, inv, c->logdet_site, c->g, sigma); break;
+    case 6: hipLaunchKernelGGL(k_clover_invert_rows<6>, dim3(grid_bs(c->g.VH, 64 / (2 * 6))), dim3(64), 0, c->stream, src, inv, c->logdet_site, c->g, sigma); break;
+    case 8: hipLaunchKernelGGL(k_clover_invert_rows<8>, dim3(grid_bs(c->g.VH, 64 / (2 * 8))), dim3(64), 0, c->stream, src, inv, c->logdet_site, c->g, sigma); break;
     default: set_error("unsupported representation dimension"); return YH_ERR_UNSUPPORTED;
   }
   YH_HIP(hipGetLastError());
