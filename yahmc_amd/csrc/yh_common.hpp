@@ -65,17 +65,22 @@ template <typename R>
 __host__ __device__ inline Cx<R> operator*(R s, Cx<R> a) { return Cx<R>(s * a.re, s * a.im); }
 template <typename R>
 __host__ __device__ inline Cx<R> conj(Cx<R> a) { return Cx<R>(a.re, -a.im); }
-// a += b * c
+// a += b * c, as four fused multiply-adds (written as a + (x - y) the
+// compiler needs a multiply, an FMA and an add per component)
 template <typename R>
 __host__ __device__ inline void cmac(Cx<R>& a, Cx<R> b, Cx<R> c) {
-  a.re += b.re * c.re - b.im * c.im;
-  a.im += b.re * c.im + b.im * c.re;
+  a.re = fma(b.re, c.re, a.re);
+  a.re = fma(-b.im, c.im, a.re);
+  a.im = fma(b.re, c.im, a.im);
+  a.im = fma(b.im, c.re, a.im);
 }
 // a += conj(b) * c
 template <typename R>
 __host__ __device__ inline void cmac_conj(Cx<R>& a, Cx<R> b, Cx<R> c) {
-  a.re += b.re * c.re + b.im * c.im;
-  a.im += b.re * c.im - b.im * c.re;
+  a.re = fma(b.re, c.re, a.re);
+  a.re = fma(b.im, c.im, a.re);
+  a.im = fma(b.re, c.im, a.im);
+  a.im = fma(-b.im, c.re, a.im);
 }
 
 // multiplication by one of {+1, -1, +i, -i} (codes 0..3)
