@@ -254,9 +254,12 @@ __device__ __forceinline__ void load_packed(cd (&M)[N][N], const double2* base, 
 //   adjoint:     R[i][m] = sum_cd Y_cd (b_m)_dc,  Y = L^+ b_i L
 // (both follow from represent()).  The at/bt/ct/dt combinations need column
 // i of the plane sum too, which the group exchanges through LDS.  The block
-// is one wave; LDS traffic needs only compiler fences (a wave's LDS
+// is one wave; LDS traffic needs only wavefront-scope fences (a wave's LDS
 // operations execute in order).
-__device__ __forceinline__ void cb_fence() { __asm__ volatile("" ::: "memory"); }
+// LDS exchange between lanes of one wave: a wavefront-scope fence (the
+// wave's LDS operations execute in order, so it emits no wait; it keeps the
+// compiler from moving LDS accesses across it)
+__device__ __forceinline__ void cb_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
 
 template <int NC, int REPR, int D>
 __global__ void __launch_bounds__(64) k_clover_build_rows(const double* __restrict__ fund, double2* __restrict__ out,
@@ -494,8 +497,8 @@ __global__ void __launch_bounds__(64) k_clover_build_rows(const double* __restri
 // the unpivoted LDL^+ (both are the ratios of the leading principal minors),
 // so log det = sum log |d_i|.  Each
 // step's normalised pivot row reaches the group through LDS (the block is
-// one wave: a wave's LDS operations execute in order, compiler fences
-// suffice).  The two chiral blocks of a site run one after the other in
+// one wave: a wave's LDS operations execute in order, wavefront-scope
+// fences suffice).  The two chiral blocks of a site run one after the other in
 // the same group, which then sums the 2N log |d_i| in fixed order.
 template <int D>
 __global__ void __launch_bounds__(64) k_clover_invert_rows(const double2* __restrict__ clov, double2* __restrict__ inv,
@@ -540,7 +543,7 @@ __global__ void __launch_bounds__(64) k_clover_invert_rows(const double2* __rest
 #pragma unroll
         for (int j = 0; j < N; j++) pr[gi][j] = make_double2(a[j].re, a[j].im);
       }
-      __asm__ volatile("" ::: "memory");
+      cb_fence();
       if (r != kk) {
         const cd f = a[kk];
 #pragma unroll
@@ -550,7 +553,7 @@ __global__ void __launch_bounds__(64) k_clover_invert_rows(const double2* __rest
           a[j] = j == kk ? cd(0.0, 0.0) - f * pj : a[j] - f * pj;
         }
       }
-      __asm__ volatile("" ::: "memory");
+      cb_fence();
     }
     // row r of the inverse: diagonal and strictly lower part, packed
     if (on) {
@@ -566,7 +569,7 @@ __global__ void __launch_bounds__(64) k_clover_invert_rows(const double2* __rest
   }
   // log det of both blocks: lane r holds the log |d| of pivot r of each
   lds_ld[gi][r] = ld;
-  __asm__ volatile("" ::: "memory");
+  cb_fence();
   if (r == 0) {
     double t = 0.0;
 #pragma unroll
