@@ -559,9 +559,13 @@ void wilson_fermion_force(double dt, double mass, double residue, SpinorField& l
   momenta_in(c);
   check(yh_wilson_fermion_force(c, dt, mass, residue, x, y), "wilson_fermion_force");
   momenta_out(c);
-  // the odd halves now hold D_oo^-1 D_oe X_e (forces.cpp:183-187)
+  // the odd halves now hold D_oo^-1 D_oe X_e (forces.cpp:183-187); the
+  // reference's in-place Dphi_oe also zeroed the boundary slices of both
+  // fields for open / SF (dirac.cpp:700)
   down(x, lhs, YH_ODD);
   down(y, rhs, YH_ODD);
+  apply_bcs_on_spinor_field(lhs);
+  apply_bcs_on_spinor_field(rhs);
 }
 
 void wilson_fermion_force_q(double dt, double mass, double residue, SpinorField& rhs) {
@@ -572,6 +576,7 @@ void wilson_fermion_force_q(double dt, double mass, double residue, SpinorField&
   check(yh_wilson_fermion_force_q(c, dt, mass, residue, y), "wilson_fermion_force_q");
   momenta_out(c);
   down(y, rhs, YH_ODD);
+  apply_bcs_on_spinor_field(rhs);  // as the reference's in-place Hphi and Dphi_oe leave it
 }
 
 void clover_logdet_force(double mass, double residue) {
