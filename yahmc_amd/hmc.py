@@ -47,7 +47,17 @@ class GaugeMonomial:
 class FermionMonomial:
     """Two flavours of EO Wilson-clover fermions (MonomialHMC, hmc.cpp)."""
 
-    def __init__(self, mass, dm=0.0, prec=1e-24, level=1, sloppy="f32", max_iter=100000, keep_pairs=0):
+    # reliable-update threshold of the monomial's mixed CG: an FP64 update
+    # whenever the sloppy residual has dropped by DELTA.  Each update is an
+    # FP64 H^2 (about twelve 16-bit iterations at D_R = 6), and the sloppy
+    # recurrence stays accurate over two decades at 16 bits: on the sextet
+    # trajectory 0.01 gives the iterations and dH of 0.1 with half the updates
+    # (2.36 -> 2.12 s); 0.001 costs extra iterations, 1e-4 breaks the 16-bit
+    # recurrence down.
+    DELTA = 0.01
+
+    def __init__(self, mass, dm=0.0, prec=1e-24, level=1, sloppy="f32", max_iter=100000, keep_pairs=0,
+                 delta=DELTA):
         if level == 0:
             raise ValueError("fermion monomials cannot be on integrator level 0 (update.cpp)")
         self.omass = mass
@@ -56,6 +66,7 @@ class FermionMonomial:
         self.level = level
         self.sloppy = sloppy
         self.max_iter = max_iter
+        self.delta = delta
         self.pf = None
         self.la = None
         self.iterations = []
@@ -88,7 +99,8 @@ class FermionMonomial:
 
     def _solve(self, lat):
         self.y.zero()
-        par = lat.inv_par(self.smass, self.prec, "Hphi_sq", sloppy=self.sloppy, max_iter=self.max_iter)
+        par = lat.inv_par(self.smass, self.prec, "Hphi_sq", sloppy=self.sloppy, max_iter=self.max_iter,
+                          delta=self.delta)
         it, _ = lat.cg_mixed(par, self.y, self.g5pf)
         self.iterations.append(it)
         if self.keep_pairs:
@@ -124,7 +136,7 @@ class HasenbuschMonomial:
     CLOVER_TERM_EO branch).  Inverses of Hphi are taken as
     H^-1 b = (H^2)^-1 H b with the mixed-precision CG on H^2."""
 
-    def __init__(self, mass, dm, prec=1e-24, level=1, sloppy="f32", max_iter=100000):
+    def __init__(self, mass, dm, prec=1e-24, level=1, sloppy="f32", max_iter=100000, delta=FermionMonomial.DELTA):
         if level == 0:
             raise ValueError("fermion monomials cannot be on integrator level 0 (update.cpp)")
         self.omass = mass
@@ -133,6 +145,7 @@ class HasenbuschMonomial:
         self.level = level
         self.sloppy = sloppy
         self.max_iter = max_iter
+        self.delta = delta
         self.pf = None
         self.la = -1.0
         self.iterations = []
@@ -143,7 +156,7 @@ class HasenbuschMonomial:
 
     def _h2_inverse(self, lat, mass, dst, src):
         dst.zero()
-        par = lat.inv_par(mass, self.prec, "Hphi_sq", sloppy=self.sloppy, max_iter=self.max_iter)
+        par = lat.inv_par(mass, self.prec, "Hphi_sq", sloppy=self.sloppy, max_iter=self.max_iter, delta=self.delta)
         it, _ = lat.cg_mixed(par, dst, src)
         self.iterations.append(it)
 
