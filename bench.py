@@ -314,6 +314,30 @@ def hmc_trajectory(yh, sloppy="f16"):
     return out
 
 
+def force_core(yh, reps=3):
+    """The clover-force core alone (for profiling): 24^3 x 32 sextet, seeded
+    random SU(3) links, clover force matrices from two Gaussian fields, then
+    `reps` end_fermion_force calls, each one k_force_clover_x launch."""
+    from yahmc_amd.synthetic import gaussian_source, random_gauge
+    dims = (32, 24, 24, 24)
+    lat = yh.Lattice(dims, nc=3, repr="symmetric", bc="antiperiodic", csw=1.0)
+    lat.set_gauge(random_gauge(dims, 3, seed=32250))
+    lat.set_momenta(np.zeros((4 * lat.V, 8)))
+    a = lat.field("f64").upload(gaussian_source(dims, 6, seed=32251, parity="both"))
+    b = lat.field("f64").upload(gaussian_source(dims, 6, seed=32252, parity="both"))
+    ts = []
+    for _ in range(reps):
+        lat.begin_fermion_force()
+        lat.clover_fermion_force(1.0, a, b)
+        lat.synchronize()
+        t0 = time.perf_counter()
+        lat.end_fermion_force(0.1)
+        lat.synchronize()
+        ts.append(time.perf_counter() - t0)
+    lat.close()
+    return {"lattice": list(dims), "repr": "sextet", "end_fermion_force_ms": [round(t * 1e3, 3) for t in ts]}
+
+
 def rhmc_multishift(yh):
     """The RHMC solve of the SU(2) adjoint workload: 24^3 x 48, clover,
     m = -1.0; 12-pole Zolotarev x^(-1/2) on [1e-4, 8]; 4 Gaussian
@@ -392,6 +416,7 @@ def main():
     ap.add_argument("--skip-rhmc", action="store_true")
     ap.add_argument("--only-rhmc", action="store_true", help="the batched multishift workload only")
     ap.add_argument("--only-hmc", action="store_true", help="the sextet HMC trajectory only")
+    ap.add_argument("--only-force", action="store_true", help="the sextet clover-force core only (for profiling)")
     ap.add_argument("--hmc-sloppy", default="f16", choices=["f32", "f16"], help="sloppy precision of the HMC solves")
     ap.add_argument("--skip-protocol", action="store_true", help="skip the held-out learned-preconditioner protocol")
     ap.add_argument("--only-protocol", action="store_true", help="the learned-preconditioner protocol only")
@@ -416,6 +441,9 @@ def main():
 
     if args.only_hmc:
         print(json.dumps(hmc_trajectory(yh, args.hmc_sloppy)))
+        return
+    if args.only_force:
+        print(json.dumps(force_core(yh)))
         return
     if args.only_rhmc:
         print(json.dumps(rhmc_multishift(yh)))
