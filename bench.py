@@ -308,7 +308,8 @@ def hmc_trajectory(yh, sloppy="f16"):
            "dH_ratio_10_over_20": round(runs[10]["dH"] / runs[20]["dH"], 3) if runs[20]["dH"] != 0 else None,
            "dH_ratio_20_over_40": round(runs[20]["dH"] / runs[40]["dH"], 3) if runs[40]["dH"] != 0 else None,
            "cg": f"mixed reliable-update CG, sloppy {SLOPPY_NAME[sloppy]}, FP64 true residual 1e-12, "
-                 f"update when the residual drops by {FermionMonomial.DELTA}"}
+                 f"update when the residual drops by {FermionMonomial.DELTA} or every "
+                 f"{FermionMonomial.RELIABLE_EVERY} iterations"}
     log(f"hmc trajectory: {out}")
     lat.close()
     return out

@@ -47,17 +47,20 @@ class GaugeMonomial:
 class FermionMonomial:
     """Two flavours of EO Wilson-clover fermions (MonomialHMC, hmc.cpp)."""
 
-    # reliable-update threshold of the monomial's mixed CG: an FP64 update
-    # whenever the sloppy residual has dropped by DELTA.  Each update is an
-    # FP64 H^2 (about twelve 16-bit iterations at D_R = 6), and the sloppy
-    # recurrence stays accurate over two decades at 16 bits: on the sextet
-    # trajectory 0.01 gives the iterations and dH of 0.1 with half the updates
-    # (2.36 -> 2.12 s); 0.001 costs extra iterations, 1e-4 breaks the 16-bit
-    # recurrence down.
+    # reliable updates of the monomial's mixed CG: an FP64 update whenever
+    # the sloppy residual has dropped by DELTA, and at the latest every
+    # RELIABLE_EVERY iterations.  Each update is an FP64 H^2 (about twelve
+    # 16-bit iterations at D_R = 6).  On the sextet trajectory DELTA = 0.01
+    # gives the iterations and dH of 0.1 with half the updates (2.36 -> 2.12 s;
+    # 0.001 costs extra iterations, 1e-4 breaks the 16-bit recurrence down);
+    # on slowly converging systems the 16-bit recurrence drifts before it
+    # gains two decades (32^4 fundamental: 1,344 iterations with no cap
+    # against 783 with updates every 50), hence the cap.
     DELTA = 0.01
+    RELIABLE_EVERY = 50
 
     def __init__(self, mass, dm=0.0, prec=1e-24, level=1, sloppy="f32", max_iter=100000, keep_pairs=0,
-                 delta=DELTA):
+                 delta=DELTA, reliable_every=RELIABLE_EVERY):
         if level == 0:
             raise ValueError("fermion monomials cannot be on integrator level 0 (update.cpp)")
         self.omass = mass
@@ -67,6 +70,7 @@ class FermionMonomial:
         self.sloppy = sloppy
         self.max_iter = max_iter
         self.delta = delta
+        self.reliable_every = reliable_every
         self.pf = None
         self.la = None
         self.iterations = []
@@ -100,7 +104,7 @@ class FermionMonomial:
     def _solve(self, lat):
         self.y.zero()
         par = lat.inv_par(self.smass, self.prec, "Hphi_sq", sloppy=self.sloppy, max_iter=self.max_iter,
-                          delta=self.delta)
+                          delta=self.delta, reliable_every=self.reliable_every)
         it, _ = lat.cg_mixed(par, self.y, self.g5pf)
         self.iterations.append(it)
         if self.keep_pairs:
@@ -136,7 +140,8 @@ class HasenbuschMonomial:
     CLOVER_TERM_EO branch).  Inverses of Hphi are taken as
     H^-1 b = (H^2)^-1 H b with the mixed-precision CG on H^2."""
 
-    def __init__(self, mass, dm, prec=1e-24, level=1, sloppy="f32", max_iter=100000, delta=FermionMonomial.DELTA):
+    def __init__(self, mass, dm, prec=1e-24, level=1, sloppy="f32", max_iter=100000, delta=FermionMonomial.DELTA,
+                 reliable_every=FermionMonomial.RELIABLE_EVERY):
         if level == 0:
             raise ValueError("fermion monomials cannot be on integrator level 0 (update.cpp)")
         self.omass = mass
@@ -146,6 +151,7 @@ class HasenbuschMonomial:
         self.sloppy = sloppy
         self.max_iter = max_iter
         self.delta = delta
+        self.reliable_every = reliable_every
         self.pf = None
         self.la = -1.0
         self.iterations = []
@@ -156,7 +162,8 @@ class HasenbuschMonomial:
 
     def _h2_inverse(self, lat, mass, dst, src):
         dst.zero()
-        par = lat.inv_par(mass, self.prec, "Hphi_sq", sloppy=self.sloppy, max_iter=self.max_iter, delta=self.delta)
+        par = lat.inv_par(mass, self.prec, "Hphi_sq", sloppy=self.sloppy, max_iter=self.max_iter, delta=self.delta,
+                          reliable_every=self.reliable_every)
         it, _ = lat.cg_mixed(par, dst, src)
         self.iterations.append(it)
 
