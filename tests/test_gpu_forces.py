@@ -67,3 +67,29 @@ def test_hmc_force_sequence(model, csw, bc):
     o.end_fermion_force(dt)
     lat.end_fermion_force(dt)
     assert rel(lat.get_momenta() - mom0, o.get_momenta() - mom0) < 1e-12
+
+
+def test_clover_force_core_work_orders_agree():
+    """The clover-force core gives the same momenta (bit for bit) whichever
+    order its workgroups take the (site block, mu) items in."""
+    print("\n[case] clover-force core work orders", flush=True)
+    nc, rep = MODELS["su3_sym"]
+    dims = (4, 4, 4, 8)
+    lat = yh.Lattice(dims, nc=nc, repr=rep, bc="antiperiodic", csw=1.0)
+    rng = np.random.default_rng(11)
+    V = lat.V
+    lat.set_gauge(random_gauge(rng, V, nc))
+    a = lat.field("f64").upload(gaussian_spinor(rng, V, 6))
+    b = lat.field("f64").upload(gaussian_spinor(rng, V, 6))
+    mom0 = rng.normal(size=(4 * V, nc * nc - 1))
+    out = []
+    for order in (-1, 0, 1, 4):
+        lat.tune(yh.TUNE_FORCE_ORDER, order)
+        lat.set_momenta(mom0)
+        lat.begin_fermion_force()
+        lat.clover_fermion_force(1.0, a, b)
+        lat.end_fermion_force(0.3)
+        out.append(lat.get_momenta())
+    assert np.abs(out[0] - mom0).max() > 0
+    for m in out[1:]:
+        assert np.array_equal(m, out[0])
