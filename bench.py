@@ -309,7 +309,8 @@ def hmc_trajectory(yh, sloppy="f16"):
            "dH_ratio_20_over_40": round(runs[20]["dH"] / runs[40]["dH"], 3) if runs[40]["dH"] != 0 else None,
            "cg": f"mixed reliable-update CG, sloppy {SLOPPY_NAME[sloppy]}, FP64 true residual 1e-12, "
                  f"update when the residual drops by {FermionMonomial.DELTA} or every "
-                 f"{FermionMonomial.RELIABLE_EVERY} iterations"}
+                 f"{FermionMonomial.RELIABLE_EVERY} iterations, MRE guess from "
+                 f"{FermionMonomial.MRE_PAST} past solution(s)"}
     log(f"hmc trajectory: {out}")
     lat.close()
     return out

@@ -50,6 +50,14 @@ zd dot(Ctx* c, const Field* a, const Field* b, int* rc) {
   return zd(v[0], v[1]);
 }
 
+// the first n reduction slots, device to host, in one synchronisation
+int read_red(Ctx* c, int n, const double** out) {
+  YH_HIP(hipMemcpyAsync(c->sc_host->red, c->sc->red, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  YH_HIP(hipStreamSynchronize(c->stream));
+  *out = c->sc_host->red;
+  return YH_OK;
+}
+
 double dot_re(Ctx* c, const Field* a, const Field* b, int* rc) {
   double v[2] = {0, 0};
   *rc = reduce_read(c, 2, a, b, 0, v);
@@ -574,28 +582,41 @@ int yh_mre_guess(yh_mre* m, yh_field* dst, const yh_field* b) {
   if ((rc = field_zero(c, dst, YH_EVEN))) return rc;
   if (n == 0) return sync(c);
   // Gram-Schmidt on the stored solutions (mre.cpp:15-33); the span, and so
-  // the Galerkin solution, is what matters
+  // the Galerkin solution, is what matters.  Each step queues |s_i|^2 and
+  // the overlaps s_i^+ s_j (j > i) into the reduction slots and reads them
+  // back with one synchronisation.
+  const double* red;
   for (int i = 0; i < n; i++) {
-    double nn = sqnorm(c, m->s[i], &rc);
-    if (rc) return rc;
-    if ((rc = blas_cscale(c, m->s[i], 1.0 / sqrt(nn), 0.0, 0))) return rc;
-    for (int j = i + 1; j < n; j++) {
-      zd r = dot(c, m->s[i], m->s[j], &rc);  // sum s_i conj(s_j)
-      if (rc) return rc;
-      // subtract the projection of s_j on s_i: (s_i^+ s_j) s_i = conj(r) s_i
-      if ((rc = blas_caxpy(c, m->s[j], -r.real(), r.imag(), m->s[i], 0))) return rc;
-    }
+    if ((rc = blas_sqnorm(c, m->s[i], 0, 0))) return rc;
+    for (int j = i + 1; j < n; j++)
+      if ((rc = blas_dot(c, m->s[i], m->s[j], 0, 2 * j))) return rc;  // sum s_i conj(s_j)
+    if ((rc = read_red(c, 2 * n, &red))) return rc;
+    const double inv = 1.0 / sqrt(red[0]);
+    std::vector<zd> r(n);
+    for (int j = i + 1; j < n; j++) r[j] = zd(red[2 * j], red[2 * j + 1]) * inv;
+    if ((rc = blas_cscale(c, m->s[i], inv, 0.0, 0))) return rc;
+    // subtract the projection of s_j on the unit s_i: (s_i^+ s_j) s_i = conj(r) s_i
+    for (int j = i + 1; j < n; j++)
+      if ((rc = blas_caxpy(c, m->s[j], -r[j].real(), r[j].imag(), m->s[i], 0))) return rc;
   }
+  // Galerkin matrix A_ji = s_j^+ M s_i and right-hand side s_i^+ b, one
+  // synchronisation per column; for the hermitian H and H^2 the entries
+  // above the diagonal are the conjugates of those below
+  const bool herm = m->mvm == YH_MVM_HPHI || m->mvm == YH_MVM_HPHI_SQ;
   const int recon = default_recon(c, PREC_F64);
   std::vector<zd> A(n * n), rhs(n), xs;
   for (int i = 0; i < n; i++) {
     if ((rc = apply_op(c, PREC_F64, recon, m->mvm, m->mass, m->out, m->s[i], nullptr, -1, nullptr))) return rc;
-    for (int j = 0; j < n; j++) {
-      A[j * n + i] = dot(c, m->out, m->s[j], &rc);  // s_j^+ M s_i, stored as sum (M s_i) conj(s_j)
-      if (rc) return rc;
+    const int j0 = herm ? i : 0;
+    for (int j = j0; j < n; j++)
+      if ((rc = blas_dot(c, m->out, m->s[j], 0, 2 * j))) return rc;  // sum (M s_i) conj(s_j)
+    if ((rc = blas_dot(c, b, m->s[i], 0, 2 * n))) return rc;         // s_i^+ b
+    if ((rc = read_red(c, 2 * n + 2, &red))) return rc;
+    for (int j = j0; j < n; j++) {
+      A[j * n + i] = zd(red[2 * j], red[2 * j + 1]);
+      if (herm && j > i) A[i * n + j] = std::conj(A[j * n + i]);
     }
-    rhs[i] = dot(c, b, m->s[i], &rc);  // s_i^+ b
-    if (rc) return rc;
+    rhs[i] = zd(red[2 * n], red[2 * n + 1]);
   }
   if ((rc = small_solve(n, A, rhs, xs))) return rc;
   for (int k = 0; k < n; k++)
