@@ -278,6 +278,11 @@ int yh_get_gauge(yh_ctx* ctx, double* links);
 int yh_represent_gauge_field(yh_ctx* ctx);
 /* wilson_gauge_force (forces.cpp:262-283): mom -= proj(U staple, dt beta/NC) */
 int yh_wilson_gauge_force(yh_ctx* ctx, double dt, double beta);
+/* improved_gauge_force (forces.cpp:285-310): mom -= proj(c0 U S_plaq +
+ * c1 U S_rect, dt beta/NC), S_rect the six weighted 1 x 2 rectangles through
+ * the link for each nu != mu (staples_symanzik, staples.cpp:7-80); the
+ * rectangle weights of the open and SF boundaries are bcs.cpp:30-31, 72-120 */
+int yh_improved_gauge_force(yh_ctx* ctx, double dt, double beta, double c0, double c1);
 /* update_links (update.cpp:228-247): U <- exp(eps P) U; the represented
  * links go stale until yh_represent_gauge_field, as set_gauge_field_state(true)
  * marks them in the reference */
@@ -288,6 +293,11 @@ int yh_reunitarize(yh_ctx* ctx);
  * plaquettes weighted, normalised by NC 6 volume (volume = (T-1) V3 for open,
  * (T-2) V3 for SF boundaries, V otherwise) */
 int yh_plaquette(yh_ctx* ctx, double* out);
+/* avr_wilson(height, width) (wilsonloop.cpp:58-133): the average over sites
+ * and planes mu > nu of Re Tr of the loop with height links along mu and
+ * width along nu, plaquettes and 1 x 2 rectangles weighted by their boundary
+ * weights, normalised like yh_plaquette; 1 <= height, width <= 16 */
+int yh_avr_wilson(yh_ctx* ctx, int height, int width, double* out);
 /* momentum part of hamiltonian (update.cpp:201-216): (TF/2) sum |P|^2,
  * after apply_bcs_on_momentum_field (bcs.cpp:166-199) */
 int yh_momenta_energy(yh_ctx* ctx, double* out);

@@ -102,6 +102,10 @@ class Oracle:
         lib.or_wilson_gauge_force.argtypes = [ctypes.c_double, ctypes.c_double]
         lib.or_plaquette.restype = ctypes.c_double
         lib.or_gauge_action.argtypes = [ctypes.c_double]
+        lib.or_avr_wilson.argtypes = [ctypes.c_int, ctypes.c_int]
+        lib.or_avr_wilson.restype = ctypes.c_double
+        lib.or_improved_gauge_force.argtypes = [ctypes.c_double] * 4
+        lib.or_gauge_volume.restype = ctypes.c_double
         lib.or_gauge_action.restype = ctypes.c_double
         lib.or_kinetic.restype = ctypes.c_double
         lib.or_update_links.argtypes = [ctypes.c_double]
@@ -324,6 +328,25 @@ class Oracle:
 
     def gauge_action(self, beta):
         return self.lib.or_gauge_action(beta)
+
+    def avr_wilson(self, h, w):
+        """avr_wilson(h, w) (wilsonloop.cpp:116-133), h, w <= 16"""
+        assert 1 <= h <= 16 and 1 <= w <= 16
+        return self.lib.or_avr_wilson(h, w)
+
+    def improved_gauge_force(self, dt, beta, c0, c1):
+        self.lib.or_improved_gauge_force(dt, beta, c0, c1)
+
+    def improved_gauge_action(self, beta, c0):
+        """MonomialGauge::action (monomials/gauge.cpp:20-38) with
+        c1 = (1 - c0) / 8"""
+        c1 = (1.0 - c0) / 8.0
+        v1 = self.avr_wilson(2, 1) + self.avr_wilson(1, 2)
+        return -beta * (c0 * self.plaquette() + c1 * v1) * 6.0 * self.gauge_volume()
+
+    def gauge_volume(self):
+        """`volume` of the gauge action (bcs.cpp:32, 76)"""
+        return self.lib.or_gauge_volume()
 
     def kinetic(self):
         return self.lib.or_kinetic()

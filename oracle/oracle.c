@@ -36,6 +36,10 @@
  *                  formulas transcribed independently in the test
  *   solvers        true residuals, the reference's self-check tolerances
  *   forces         finite differences of the action
+ *   gauge MD       finite differences of the Wilson and the improved
+ *                  (plaquette + rectangle) actions, matrix exponential,
+ *                  o2mn dt^2 scaling and reversibility, Wilson loops against
+ *                  explicit numpy link products
  */
 #include <complex.h>
 #include <math.h>
@@ -1442,6 +1446,8 @@ static double gauge_volume(void)
 	return (double)V;
 }
 
+double or_gauge_volume(void) { return gauge_volume(); }
+
 /* apply_bcs_on_momentum_field: library/bcs.cpp:166-199 */
 static void apply_bcs_on_momentum(void)
 {
@@ -1524,6 +1530,130 @@ double or_plaquette(void)
 
 /* MonomialGauge::action for the Wilson action: -beta * plaq * 6 volume */
 double or_gauge_action(double beta) { return -beta * or_plaquette() * 6.0 * gauge_volume(); }
+
+/* ------------------------------------------------------------------ */
+/* Symanzik-improved gauge action: improved_gauge_force                 */
+/* (library/forces.cpp:285-310), staples_symanzik (staples.cpp:7-80),   */
+/* wilson_loop / avr_wilson (wilsonloop.cpp:58-133), rect_weight        */
+/* (library/bcs.cpp:30-31, 72-120; include/bcs.h:40).                   */
+/* ------------------------------------------------------------------ */
+
+/* rect_weight(id, a, b): the weight of the 1 x 2 rectangle based at id with
+ * its short side along a and its long side along b.  Open: the plaquette
+ * weights (the same array, bcs.cpp:31).  SF: 0 for a < b spatial at t = 0
+ * and T-2, 1.5 for a = 0 at t = 0 and T-3 (bcs.cpp:89-116); 1 otherwise */
+static double rect_weight(int id, int a, int b)
+{
+	if (g_bc == BC_OPEN) return plaq_weight(id, a, b);
+	if (g_bc != BC_SF) return 1.0;
+	int t = crd[4 * id];
+	if ((t == 0 || t == L[0] - 2) && a >= 1 && a < b) return 0.0;
+	if ((t == 0 || t == L[0] - 3) && a == 0 && b >= 1) return 1.5;
+	return 1.0;
+}
+
+/* product of the links along a path from site *id: step +(d+1) takes the
+ * forward link U(s, d), step -(d+1) the backward link U(s-d, d)^+; *id ends
+ * at the path's last site */
+static gmat path_product(int *id, const int *steps, int n)
+{
+	gmat P;
+	memset(&P, 0, sizeof(P));
+	for (int i = 0; i < NC; i++) P.m[i][i] = 1.0;
+	int s = *id;
+	for (int k = 0; k < n; k++) {
+		int d = abs(steps[k]) - 1;
+		if (steps[k] > 0) {
+			P = gmul(P, LNK(s, d));
+			s = nb_fw[4 * s + d];
+		} else {
+			s = nb_bk[4 * s + d];
+			P = gmul(P, gdag(LNK(s, d)));
+		}
+	}
+	*id = s;
+	return P;
+}
+
+static int walk(int id, int d, int n)
+{
+	for (int k = 0; k < n; k++) id = d > 0 ? nb_fw[4 * id + d - 1] : nb_bk[4 * id - d - 1];
+	return id;
+}
+
+/* wilson_loop(id, mu, nu, h, w): Re Tr of the h x w loop (h links along mu,
+ * w along nu) based at id, with its plaquette or rectangle weight */
+static double wilson_loop(int id, int mu, int nu, int h, int w)
+{
+	int steps[64], n = 0;
+	for (int k = 0; k < h; k++) steps[n++] = mu + 1;
+	for (int k = 0; k < w; k++) steps[n++] = nu + 1;
+	for (int k = 0; k < h; k++) steps[n++] = -(mu + 1);
+	for (int k = 0; k < w; k++) steps[n++] = -(nu + 1);
+	int end = id;
+	gmat P = path_product(&end, steps, n);
+	double tr = 0;
+	for (int i = 0; i < NC; i++) tr += creal(P.m[i][i]);
+	if (h == 1 && w == 1) tr *= plaq_weight(id, mu, nu);
+	else if (h == 1 && w == 2) tr *= rect_weight(id, mu, nu);
+	else if (h == 2 && w == 1) tr *= rect_weight(id, nu, mu);
+	return tr;
+}
+
+/* avr_wilson(h, w): sum over the planes (mu > nu) / (NC 6 volume); h, w <= 16 */
+double or_avr_wilson(int h, int w)
+{
+	double sum = 0;
+#pragma omp parallel for reduction(+ : sum)
+	for (int id = 0; id < V; id++)
+		for (int mu = 1; mu < 4; mu++)
+			for (int nu = 0; nu < mu; nu++) sum += wilson_loop(id, mu, nu, h, w);
+	return sum / (NC * 6.0 * gauge_volume());
+}
+
+/* staples_symanzik(id, mu): the six 1 x 2 rectangles through the link
+ * (id, mu) for each nu != mu, each as the path from id + mu back to id,
+ * weighted by rect_weight of the rectangle's base corner */
+static gmat staple_symanzik(int id, int mu)
+{
+	gmat st;
+	memset(&st, 0, sizeof(st));
+	const int m = mu + 1;
+	for (int nu = 0; nu < 4; nu++) {
+		if (nu == mu) continue;
+		const int n = nu + 1;
+		const int paths[6][5] = {
+			{n, n, -m, -n, -n},  /* long side nu, +nu side */
+			{-n, -n, -m, n, n},  /* long side nu, -nu side */
+			{m, n, -m, -m, -n},  /* long side mu, link first, +nu */
+			{n, -m, -m, -n, m},  /* long side mu, link second, +nu */
+			{m, -n, -m, -m, n},  /* long side mu, link first, -nu */
+			{-n, -m, -m, n, m},  /* long side mu, link second, -nu */
+		};
+		const int base[6] = {id, walk(id, -n, 2), id, walk(id, -m, 1), walk(id, -n, 1), walk(walk(id, -m, 1), -n, 1)};
+		for (int k = 0; k < 6; k++) {
+			int s = nb_fw[4 * id + mu];
+			gmat P = path_product(&s, paths[k], 5);
+			double wgt = k < 2 ? rect_weight(base[k], mu, nu) : rect_weight(base[k], nu, mu);
+			st = gadd(st, gscale(P, wgt));
+		}
+	}
+	return st;
+}
+
+/* improved_gauge_force: momentum -= algebra_project(c0 U S_plaq + c1 U S_rect,
+ * dt beta / NC) */
+void or_improved_gauge_force(double dt, double beta, double c0, double c1)
+{
+	double coeff = dt * (beta / NC);
+#pragma omp parallel for
+	for (int id = 0; id < V; id++)
+		for (int mu = 0; mu < 4; mu++) {
+			gmat U = LNK(id, mu);
+			gmat F = gadd(gscale(gmul(U, staple_wilson(id, mu)), c0), gscale(gmul(U, staple_symanzik(id, mu)), c1));
+			algebra_project_g(F, coeff, mom + (4 * (size_t)id + mu) * NG, -1.0);
+		}
+}
 
 /* TF/2 sum |P|^2 after the momentum BCs (update.cpp:201-216) */
 double or_kinetic(void)

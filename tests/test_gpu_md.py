@@ -1,8 +1,8 @@
 """Gauge molecular dynamics on the device against the oracle
-(library/forces.cpp:262-283, update.cpp:201-320, wilsonloop.cpp:116-133,
-integrator.cpp:42-63): plaquette, Wilson gauge force, link update,
-reunitarisation, momentum energy, a whole o2mn trajectory, and the
-represented operator after the links moved."""
+(library/forces.cpp:262-310, update.cpp:201-320, wilsonloop.cpp:58-133,
+integrator.cpp:42-63): plaquette, Wilson loops, Wilson and improved gauge
+forces, link update, reunitarisation, momentum energy, whole o2mn
+trajectories, and the represented operator after the links moved."""
 import numpy as np
 import pytest
 
@@ -186,6 +186,82 @@ def test_open_boundary_o2mn_trajectory_matches_oracle():
             x.wilson_gauge_force((1 - 2 * a) * dt, beta)
             x.update_links(dt / 2)
         x.wilson_gauge_force(a * dt, beta)
+    dHd, dHo = ham(lat) - h0d, ham(o) - h0o
+    print(f"dH device {dHd:.6e} oracle {dHo:.6e}")
+    assert abs(dHd - dHo) < 1e-9 * abs(h0o)
+    assert np.abs(lat.get_gauge() - o.get_gauge()).max() < 1e-12
+    assert np.abs(lat.get_momenta() - o.get_momenta()).max() < 1e-11
+
+
+IMPROVED_CASES = [("su3_fund", 3, "fundamental", "antiperiodic"), ("su2_fund", 2, "fundamental", "periodic"),
+                  ("su4_asym", 4, "antisymmetric", "antiperiodic"), ("su3_fund", 3, "fundamental", "open"),
+                  ("su3_fund", 3, "fundamental", "sf"), ("su2_fund", 2, "fundamental", "sf")]
+C0_LW = 5.0 / 3.0  # tree-level Luscher-Weisz, c1 = -1/12
+
+
+@pytest.mark.parametrize("model,nc,rep,bc", IMPROVED_CASES)
+def test_improved_gauge_force_and_loops_match_oracle(model, nc, rep, bc):
+    """avr_wilson(h, w) (wilsonloop.cpp:58-133) and improved_gauge_force
+    (forces.cpp:285-310, staples.cpp:7-80) with the rectangle weights of the
+    open and SF boundaries (bcs.cpp:30-31, 72-120); the gauge monomial with
+    c0 != 1 (monomials/gauge.cpp) uses both."""
+    print(f"\n[case] improved gauge force {model} {bc}", flush=True)
+    from yahmc_amd.hmc import GaugeMonomial
+    if bc in ("open", "sf"):
+        o, lat, rng = setup_bc(model, nc, rep, bc, seed=29)
+    else:
+        o, lat, rng = setup(model, nc, rep, seed=29, bc=bc)
+    for h, w in ((1, 1), (1, 2), (2, 1), (2, 2), (3, 1)):
+        assert abs(lat.avr_wilson(h, w) - o.avr_wilson(h, w)) < 1e-14, (h, w)
+    assert abs(lat.avr_wilson(1, 1) - lat.plaquette()) < 1e-15
+    beta, c0 = 4.2, C0_LW
+    c1 = (1.0 - c0) / 8.0
+    P = rng.normal(size=(4 * o.V, nc * nc - 1))
+    o.set_momenta(P)
+    lat.set_momenta(P)
+    o.improved_gauge_force(0.37, beta, c0, c1)
+    lat.improved_gauge_force(0.37, beta, c0, c1)
+    want = o.get_momenta()
+    assert np.abs(lat.get_momenta() - want).max() < 1e-13 * np.abs(want).max()
+    # the monomial: action and update
+    mono = GaugeMonomial(beta, c0=c0)
+    assert mono.improved
+    assert abs(mono.action(lat) - o.improved_gauge_action(beta, c0)) < 1e-12 * abs(o.improved_gauge_action(beta, c0))
+    lat.set_momenta(P)
+    mono.update(lat, 0.37)
+    assert np.abs(lat.get_momenta() - want).max() < 1e-13 * np.abs(want).max()
+    with pytest.raises(yh.YahmcError):
+        lat.avr_wilson(0, 1)
+
+
+def test_improved_sf_o2mn_trajectory_matches_oracle():
+    """A five-step o2mn trajectory of the improved gauge action with SF
+    boundaries: dH, links and momenta agree with the oracle."""
+    print("\n[case] improved o2mn trajectory su3_fund sf", flush=True)
+    o, lat, rng = setup_bc("su3_fund", 3, "fundamental", "sf", seed=31)
+    beta, c0 = 4.2, C0_LW
+    c1 = (1.0 - c0) / 8.0
+    P = rng.normal(size=(4 * o.V, 8))
+    o.set_momenta(P)
+    lat.set_momenta(P)
+
+    def ham(x):
+        if x is lat:
+            v1 = lat.avr_wilson(2, 1) + lat.avr_wilson(1, 2)
+            return x.momenta_energy() - beta * (c0 * x.plaquette() + c1 * v1) * 6 * lat.gauge_volume
+        return x.kinetic() + x.improved_gauge_action(beta, c0)
+
+    h0d, h0o = ham(lat), ham(o)
+    assert abs(h0d - h0o) < 1e-10 * abs(h0o)
+    a, steps, length = 0.1931833275037836, 5, 0.5
+    dt = length / steps
+    for x in (o, lat):
+        for n in range(steps):
+            x.improved_gauge_force(a * dt if n == 0 else 2 * a * dt, beta, c0, c1)
+            x.update_links(dt / 2)
+            x.improved_gauge_force((1 - 2 * a) * dt, beta, c0, c1)
+            x.update_links(dt / 2)
+        x.improved_gauge_force(a * dt, beta, c0, c1)
     dHd, dHo = ham(lat) - h0d, ham(o) - h0o
     print(f"dH device {dHd:.6e} oracle {dHo:.6e}")
     assert abs(dHd - dHo) < 1e-9 * abs(h0o)

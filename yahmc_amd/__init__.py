@@ -136,6 +136,9 @@ _SIGS = {
     "yh_get_gauge": ([_VP, _D], ctypes.c_int),
     "yh_represent_gauge_field": ([_VP], ctypes.c_int),
     "yh_wilson_gauge_force": ([_VP, ctypes.c_double, ctypes.c_double], ctypes.c_int),
+    "yh_improved_gauge_force": ([_VP, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double],
+                                ctypes.c_int),
+    "yh_avr_wilson": ([_VP, ctypes.c_int, ctypes.c_int, _D], ctypes.c_int),
     "yh_update_links": ([_VP, ctypes.c_double], ctypes.c_int),
     "yh_reunitarize": ([_VP], ctypes.c_int),
     "yh_apply_bcs_on_gauge_field": ([_VP], ctypes.c_int),
@@ -608,6 +611,11 @@ class Lattice:
     def wilson_gauge_force(self, dt, beta):
         _check(lib().yh_wilson_gauge_force(self.ctx, dt, beta))
 
+    def improved_gauge_force(self, dt, beta, c0, c1):
+        """improved_gauge_force (library/forces.cpp:285-310): plaquette
+        staples times c0 plus the 1 x 2 rectangle staples times c1."""
+        _check(lib().yh_improved_gauge_force(self.ctx, dt, beta, c0, c1))
+
     def update_links(self, eps):
         _check(lib().yh_update_links(self.ctx, eps))
 
@@ -617,6 +625,12 @@ class Lattice:
     def plaquette(self):
         out = np.zeros(1)
         _check(lib().yh_plaquette(self.ctx, _dp(out)))
+        return float(out[0])
+
+    def avr_wilson(self, height, width):
+        """avr_wilson(height, width) (library/wilsonloop.cpp:116-133)."""
+        out = np.zeros(1)
+        _check(lib().yh_avr_wilson(self.ctx, height, width, _dp(out)))
         return float(out[0])
 
     def momenta_energy(self):

@@ -195,6 +195,8 @@ int clover_force_download(Ctx* c, double* host);
 // ---- gauge molecular dynamics ----------------------------------------------
 int md_init_constants(Ctx* c);
 int md_gauge_force(Ctx* c, double dt, double beta);
+int md_improved_gauge_force(Ctx* c, double dt, double beta, double c0, double c1);
+int md_avr_wilson(Ctx* c, int h, int w, double* out);
 int md_update_links(Ctx* c, double eps);
 int md_reunitarize(Ctx* c);
 int md_plaquette(Ctx* c, double* out);

@@ -5,6 +5,9 @@
 // update_links       : library/update.cpp:228-247 + suNa.cpp:113-134 (exp)
 // reunitarize        : library/suNg.cpp:171-359
 // plaquette          : library/wilsonloop.cpp:58-133 (avr_wilson(1,1))
+// improved_gauge_force: library/forces.cpp:285-310 + staples.cpp:7-80
+// avr_wilson(h, w)   : library/wilsonloop.cpp:58-133 (rectangle weights of
+//                      bcs.cpp:30-31, 72-120)
 // hamiltonian (P^2)  : library/update.cpp:201-216
 // Fundamental links stay in the reference layout (4*id+mu, re plane then im
 // plane); momenta are SoA mom[(mu * NG + a) * V + id].  Periodic, open and
@@ -265,6 +268,60 @@ __device__ __forceinline__ bool link_fixed(int t, int mu, const Geom& g) {
   return false;
 }
 
+// rect_weight(id, a, b) of the 1 x 2 rectangle based on time slice t with
+// its short side along a and long side along b: open boundaries use the
+// plaquette weights (bcs.cpp:31); SF weights 0 for the spatial a < b
+// rectangles on t = 0 and T-2 and 1.5 for a = 0 on t = 0 and T-3
+// (bcs.cpp:89-116)
+__device__ __forceinline__ double rect_w(int t, int a, int b, const Geom& g) {
+  if (g.bc == BC_OPEN) return plaq_w(t, a, b, g);
+  if (g.bc != BC_SF) return 1.0;
+  if ((t == 0 || t == g.L[0] - 2) && a >= 1 && a < b) return 0.0;
+  if ((t == 0 || t == g.L[0] - 3) && a == 0 && b >= 1) return 1.5;
+  return 1.0;
+}
+
+// one step of a path on coordinates c: forward (d, +1) multiplies P by
+// U(c, d) and moves to c + d; backward (d, -1) moves to c - d and multiplies
+// by U(c - d, d)^+
+template <int NC>
+__device__ __forceinline__ void path_step(M<NC>& P, const double* fund, int (&c)[4], int d, bool fw, const Geom& g) {
+  if (fw) {
+    P = mul<NC>(P, ld<NC>(fund, full_index(c[0], c[1], c[2], c[3], g), d));
+    c[d] = c[d] + 1 == g.L[d] ? 0 : c[d] + 1;
+  } else {
+    c[d] = c[d] == 0 ? g.L[d] - 1 : c[d] - 1;
+    P = muld<NC>(P, ld<NC>(fund, full_index(c[0], c[1], c[2], c[3], g), d));
+  }
+}
+
+template <int NC>
+__device__ __forceinline__ M<NC> unit_matrix() {
+  M<NC> r;
+#pragma unroll
+  for (int i = 0; i < NC; i++)
+#pragma unroll
+    for (int j = 0; j < NC; j++) r.m[i][j] = cd(i == j ? 1.0 : 0.0, 0.0);
+  return r;
+}
+
+// The six 1 x 2 rectangles through the link (x, mu) for one nu, as paths
+// from x + mu back to x (bits 2j, 2j+1 of the code: step j is 0 +mu, 1 -mu,
+// 2 +nu, 3 -nu), with the offset of the rectangle's base corner from x
+// (bits 10-11: -mu steps, 12-13: -nu steps) and whether its long side is
+// along nu (bit 14), which orders the arguments of its weight
+#define YH_R(s0, s1, s2, s3, s4, bm, bn, lnu) \
+  ((s0) | (s1) << 2 | (s2) << 4 | (s3) << 6 | (s4) << 8 | (bm) << 10 | (bn) << 12 | (lnu) << 14)
+__constant__ int c_rect_paths[6] = {
+    YH_R(2, 2, 1, 3, 3, 0, 0, 1),  // long side nu, +nu side
+    YH_R(3, 3, 1, 2, 2, 0, 2, 1),  // long side nu, -nu side
+    YH_R(0, 2, 1, 1, 3, 0, 0, 0),  // long side mu, the link first, +nu side
+    YH_R(2, 1, 1, 3, 0, 1, 0, 0),  // long side mu, the link second, +nu side
+    YH_R(0, 3, 1, 1, 2, 0, 1, 0),  // long side mu, the link first, -nu side
+    YH_R(3, 1, 1, 2, 0, 1, 1, 0),  // long side mu, the link second, -nu side
+};
+#undef YH_R
+
 // apply_bcs_on_momentum_field (bcs.cpp:166-199)
 __global__ void __launch_bounds__(256) k_mom_bcs(double* __restrict__ mom, int NG, Geom g) {
   int t = blockIdx.x * 256 + threadIdx.x;
@@ -300,10 +357,13 @@ __global__ void __launch_bounds__(256) k_gauge_bcs(double* __restrict__ fund, Ge
 }
 
 // Wilson gauge force: mom -= algebra_project(U staple, dt beta / NC), the
-// staples weighted by their plaquettes' weights (staples.cpp:82-107)
-template <int NC>
+// staples weighted by their plaquettes' weights (staples.cpp:82-107).  With
+// RECT the improved force: staple c0 S_plaq + c1 S_rect, S_rect the six
+// weighted rectangles through the link for each nu (staples.cpp:7-80)
+template <int NC, bool RECT>
 __global__ void __launch_bounds__(128) k_gauge_force(const double* __restrict__ fund, double* __restrict__ mom,
-                                                     double coeff, Geom g, const double2* __restrict__ itf) {
+                                                     double coeff, double c0, double c1, Geom g,
+                                                     const double2* __restrict__ itf) {
   int t = blockIdx.x * 128 + threadIdx.x;
   if (t >= 4 * g.V) return;
   int id = t >> 2, mu = t & 3;
@@ -327,11 +387,35 @@ __global__ void __launch_bounds__(128) k_gauge_force(const double* __restrict__ 
     int b2 = mshift(cb, mu, 1, g);
     M<NC> q = mul<NC>(ld<NC>(fund, b, mu), ld<NC>(fund, b2, nu));
     M<NC> s2 = dmul<NC>(q, ld<NC>(fund, b, nu));
-    const double w1 = plaq_w(c[0], mu, nu, g), w2 = plaq_w(cb[0], nu, mu, g);
+    double w1 = plaq_w(c[0], mu, nu, g), w2 = plaq_w(cb[0], nu, mu, g);
+    if constexpr (RECT) {
+      w1 *= c0;
+      w2 *= c0;
+    }
 #pragma unroll
     for (int i = 0; i < NC; i++)
 #pragma unroll
       for (int j = 0; j < NC; j++) S.m[i][j] = S.m[i][j] + w1 * s1.m[i][j] + w2 * s2.m[i][j];
+    if constexpr (RECT) {
+      for (int r = 0; r < 6; r++) {
+        const int code = c_rect_paths[r];
+        int p[4] = {c[0], c[1], c[2], c[3]};
+        p[mu] = p[mu] + 1 == g.L[mu] ? 0 : p[mu] + 1;
+        M<NC> P = unit_matrix<NC>();
+        for (int j = 0; j < 5; j++) {
+          const int sc = (code >> (2 * j)) & 3;
+          path_step<NC>(P, fund, p, sc >> 1 ? nu : mu, (sc & 1) == 0, g);
+        }
+        // time slice of the base corner: x - bm mu - bn nu
+        int tb = c[0] - (mu == 0 ? (code >> 10) & 3 : 0) - (nu == 0 ? (code >> 12) & 3 : 0);
+        tb = tb < 0 ? tb + g.L[0] : tb;
+        const double w = c1 * ((code >> 14) & 1 ? rect_w(tb, mu, nu, g) : rect_w(tb, nu, mu, g));
+#pragma unroll
+        for (int i = 0; i < NC; i++)
+#pragma unroll
+          for (int j = 0; j < NC; j++) S.m[i][j] = S.m[i][j] + w * P.m[i][j];
+      }
+    }
   }
   M<NC> F = mul<NC>(ld<NC>(fund, id, mu), S);
 #pragma unroll
@@ -415,6 +499,37 @@ __global__ void __launch_bounds__(256) k_plaquette(const double* __restrict__ fu
   block_reduce_finish<1, 256>(v, partials, gridDim.x, blockIdx.x, ticket, out);
 }
 
+// sum over planes mu > nu of the weighted Re Tr of the h x w loop based at
+// each site: h links along mu, w along nu, back along mu, back along nu
+// (wilson_loop, wilsonloop.cpp:58-113)
+template <int NC>
+__global__ void __launch_bounds__(256) k_wilson_loop(const double* __restrict__ fund, int h, int w, double* partials,
+                                                     unsigned* ticket, double* out, Geom g) {
+  int id = blockIdx.x * 256 + threadIdx.x;
+  double v[1] = {0.0};
+  if (id < g.V) {
+    int c[4];
+    full_coords(id, g, c);
+    for (int mu = 1; mu < 4; mu++)
+      for (int nu = 0; nu < mu; nu++) {
+        int p[4] = {c[0], c[1], c[2], c[3]};
+        M<NC> P = unit_matrix<NC>();
+        for (int k = 0; k < h; k++) path_step<NC>(P, fund, p, mu, true, g);
+        for (int k = 0; k < w; k++) path_step<NC>(P, fund, p, nu, true, g);
+        for (int k = 0; k < h; k++) path_step<NC>(P, fund, p, mu, false, g);
+        for (int k = 0; k < w; k++) path_step<NC>(P, fund, p, nu, false, g);
+        double tr = 0.0;
+#pragma unroll
+        for (int i = 0; i < NC; i++) tr += P.m[i][i].re;
+        if (h == 1 && w == 1) tr *= plaq_w(c[0], mu, nu, g);
+        else if (h == 1 && w == 2) tr *= rect_w(c[0], mu, nu, g);
+        else if (h == 2 && w == 1) tr *= rect_w(c[0], nu, mu, g);
+        v[0] += tr;
+      }
+  }
+  block_reduce_finish<1, 256>(v, partials, gridDim.x, blockIdx.x, ticket, out);
+}
+
 __global__ void __launch_bounds__(256) k_sumsq(const double* __restrict__ a, long n, double* partials, unsigned* ticket,
                                                double* out) {
   long i0 = (long)blockIdx.x * 256 + threadIdx.x;
@@ -463,7 +578,19 @@ int md_gauge_force(Ctx* c, double dt, double beta) {
   if ((rc = force_alloc(c))) return rc;
   double coeff = dt * beta / c->nc;
   dim3 grid((4 * c->g.V + 127) / 128);
-  YH_NC_SWITCH(hipLaunchKernelGGL((k_gauge_force<NC_>), grid, dim3(128), 0, c->stream, c->fund, c->mom, coeff, c->g, c->tab_itf));
+  YH_NC_SWITCH(hipLaunchKernelGGL((k_gauge_force<NC_, false>), grid, dim3(128), 0, c->stream, c->fund, c->mom, coeff,
+                                  1.0, 0.0, c->g, c->tab_itf));
+  YH_HIP(hipGetLastError());
+  return YH_OK;
+}
+
+int md_improved_gauge_force(Ctx* c, double dt, double beta, double c0, double c1) {
+  int rc;
+  if ((rc = force_alloc(c))) return rc;
+  double coeff = dt * beta / c->nc;
+  dim3 grid((4 * c->g.V + 127) / 128);
+  YH_NC_SWITCH(hipLaunchKernelGGL((k_gauge_force<NC_, true>), grid, dim3(128), 0, c->stream, c->fund, c->mom, coeff,
+                                  c0, c1, c->g, c->tab_itf));
   YH_HIP(hipGetLastError());
   return YH_OK;
 }
@@ -541,6 +668,19 @@ int md_plaquette(Ctx* c, double* out) {
   return YH_OK;
 }
 
+int md_avr_wilson(Ctx* c, int h, int w, double* out) {
+  dim3 grid((c->g.V + 255) / 256);
+  double* dst = &c->sc->red[0];
+  YH_NC_SWITCH(hipLaunchKernelGGL((k_wilson_loop<NC_>), grid, dim3(256), 0, c->stream, c->fund, h, w, c->partials,
+                                  c->ticket, dst, c->g));
+  YH_HIP(hipGetLastError());
+  double v;
+  YH_HIP(hipMemcpyAsync(&v, dst, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  YH_HIP(hipStreamSynchronize(c->stream));
+  *out = v / (c->nc * 6.0 * gauge_volume(c));
+  return YH_OK;
+}
+
 int md_kinetic(Ctx* c, double* out) {
   int rc;
   if ((rc = force_alloc(c))) return rc;
@@ -607,6 +747,20 @@ int yh_wilson_gauge_force(yh_ctx* c, double dt, double beta) {
   int rc = md_check(c);
   if (rc) return rc;
   return md_sync(c, md_gauge_force(c, dt, beta));
+}
+
+int yh_improved_gauge_force(yh_ctx* c, double dt, double beta, double c0, double c1) {
+  int rc = md_check(c);
+  if (rc) return rc;
+  return md_sync(c, md_improved_gauge_force(c, dt, beta, c0, c1));
+}
+
+int yh_avr_wilson(yh_ctx* c, int height, int width, double* out) {
+  if (!out) { set_error("null argument"); return YH_ERR_ARG; }
+  if (height < 1 || width < 1 || height > 16 || width > 16) { set_error("avr_wilson: loop sides must be 1..16"); return YH_ERR_ARG; }
+  int rc = md_check(c);
+  if (rc) return rc;
+  return md_avr_wilson(c, height, width, out);
 }
 
 int yh_update_links(yh_ctx* c, double eps) {

@@ -26,22 +26,32 @@ O4MN = (0.08398315262876693, 0.2539785108410595, 0.6822365335719091, -0.03230286
 
 
 class GaugeMonomial:
-    """Wilson plaquette action (MonomialGauge with c0 = 1, gauge.cpp)."""
+    """Wilson plaquette action, or with c0 != 1 the Symanzik-improved action
+    with rectangles weighted c1 = (1 - c0) / 8 (MonomialGauge, gauge.cpp)."""
 
-    def __init__(self, beta, level=0):
+    def __init__(self, beta, level=0, c0=1.0):
         if level != 0:
             raise ValueError("the gauge monomial must be on integrator level 0 (update.cpp)")
         self.beta = beta
         self.level = 0
+        self.c0 = c0
+        self.c1 = (1.0 - c0) / 8.0
+        self.improved = abs(self.c1) > 1e-8  # gauge.cpp:11
 
     def reset(self, lat, rng):
         pass
 
     def action(self, lat):
+        if self.improved:
+            v1 = lat.avr_wilson(2, 1) + lat.avr_wilson(1, 2)
+            return -self.beta * (self.c0 * lat.plaquette() + self.c1 * v1) * 6.0 * lat.gauge_volume
         return -self.beta * lat.plaquette() * 6.0 * lat.gauge_volume
 
     def update(self, lat, dt):
-        lat.wilson_gauge_force(dt, self.beta)
+        if self.improved:
+            lat.improved_gauge_force(dt, self.beta, self.c0, self.c1)
+        else:
+            lat.wilson_gauge_force(dt, self.beta)
 
 
 class FermionMonomial:

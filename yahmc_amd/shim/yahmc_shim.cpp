@@ -591,11 +591,11 @@ void wilson_gauge_force(double dt, double beta) {
   momenta_out(c);
 }
 
-// the Symanzik rectangle term is not part of the device build; c1 = 0 is the
-// Wilson action scaled by c0 (forces.cpp:287-309)
 void improved_gauge_force(double dt, double beta, double c0, double c1) {
-  if (c1 != 0.0) lprintf("YAHMC", CRITICAL, "improved_gauge_force: rectangle term (c1 = %g) not supported", c1);
-  wilson_gauge_force(dt, beta * c0);
+  yh_ctx* c = ready();
+  momenta_in(c);
+  check(yh_improved_gauge_force(c, dt, beta, c0, c1), "improved_gauge_force");
+  momenta_out(c);
 }
 
 // ---- rational.h -----------------------------------------------------------
