@@ -92,6 +92,24 @@ __device__ __forceinline__ Cx<R> unit_mul(Cx<R> z) {
   else return Cx<R>(z.im, -z.re);
 }
 
+// a += s * unit_mul<C>(z) as two FMAs (ns = -s)
+template <int C, typename R>
+__device__ __forceinline__ void unit_fma(Cx<R>& a, R s, R ns, Cx<R> z) {
+  if constexpr (C == 0) {
+    a.re = fma(s, z.re, a.re);
+    a.im = fma(s, z.im, a.im);
+  } else if constexpr (C == 1) {
+    a.re = fma(ns, z.re, a.re);
+    a.im = fma(ns, z.im, a.im);
+  } else if constexpr (C == 2) {
+    a.re = fma(ns, z.im, a.re);
+    a.im = fma(s, z.re, a.im);
+  } else {
+    a.re = fma(s, z.im, a.re);
+    a.im = fma(ns, z.re, a.im);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // geometry
 // ---------------------------------------------------------------------------
@@ -369,10 +387,23 @@ struct LinkStore<PREC_F32, RECON_12H, D> { typedef uint2 vec; static constexpr i
 
 // third row of an SU(3) matrix from the first two: conj(r0 x r1)
 template <typename R>
+__device__ __forceinline__ Cx<R> conj_cross(Cx<R> a, Cx<R> b, Cx<R> c, Cx<R> d) {
+  // conj(a b - c d) as two four-term FMA chains
+  R re = a.re * b.re;
+  re = fma(-a.im, b.im, re);
+  re = fma(-c.re, d.re, re);
+  re = fma(c.im, d.im, re);
+  R im = c.re * d.im;
+  im = fma(c.im, d.re, im);
+  im = fma(-a.re, b.im, im);
+  im = fma(-a.im, b.re, im);
+  return Cx<R>(re, im);
+}
+template <typename R>
 __device__ __forceinline__ void su3_third_row(Cx<R> (&u)[3][3]) {
-  u[2][0] = conj(u[0][1] * u[1][2] - u[0][2] * u[1][1]);
-  u[2][1] = conj(u[0][2] * u[1][0] - u[0][0] * u[1][2]);
-  u[2][2] = conj(u[0][0] * u[1][1] - u[0][1] * u[1][0]);
+  u[2][0] = conj_cross(u[0][1], u[1][2], u[0][2], u[1][1]);
+  u[2][1] = conj_cross(u[0][2], u[1][0], u[0][0], u[1][2]);
+  u[2][2] = conj_cross(u[0][0], u[1][1], u[0][1], u[1][0]);
 }
 
 // NT: non-temporal loads (each link is read once per hop)

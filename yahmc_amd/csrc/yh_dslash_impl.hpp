@@ -94,7 +94,12 @@ __device__ __forceinline__ void hop_dir_src(Cx<R> (&acc)[4][D], const HopArgs& a
   load_link<PREC, RECON, D, R, PREC == PREC_F64, PREC == PREC_F16>(u, lbase, lidx, a.g.VHpad);
   if constexpr (PREC == PREC_F16) sc *= (R)Q_UNIT;  // the links were loaded as 32768 U
   Cx<R> r[2][D];
-  const R sgn = flip ? -sc : sc;  // antiperiodic sign and 16-bit scales
+  // the antiperiodic sign and the 16-bit scales multiply the accumulation
+  // (a branch here splits the kernel into blocks and serialises the
+  // neighbour loads of the t direction); folded into its FMAs they cost no
+  // separate multiplies
+  const R sgn = flip ? -sc : sc;
+  const R nsgn = -sgn;
 #pragma unroll
   for (int i = 0; i < D; i++) {
     Cx<R> r0((R)0, (R)0), r1((R)0, (R)0);
@@ -108,17 +113,15 @@ __device__ __forceinline__ void hop_dir_src(Cx<R> (&acc)[4][D], const HopArgs& a
         cmac_conj(r1, u[j][i], h[1][j]);
       }
     }
-    // the antiperiodic sign as a multiply: a branch here splits the kernel
-    // into blocks and serialises the neighbour loads of the t direction
-    r[0][i] = sgn * r0;
-    r[1][i] = sgn * r1;
+    r[0][i] = r0;
+    r[1][i] = r1;
   }
 #pragma unroll
   for (int i = 0; i < D; i++) {
-    acc[2][i] = acc[2][i] + r[0][i];
-    acc[3][i] = acc[3][i] + r[1][i];
-    acc[0][i] = acc[0][i] + unit_mul<P::RC0>(r[P::RI0][i]);
-    acc[1][i] = acc[1][i] + unit_mul<P::RC1>(r[P::RI1][i]);
+    unit_fma<0>(acc[2][i], sgn, nsgn, r[0][i]);
+    unit_fma<0>(acc[3][i], sgn, nsgn, r[1][i]);
+    unit_fma<P::RC0>(acc[0][i], sgn, nsgn, r[P::RI0][i]);
+    unit_fma<P::RC1>(acc[1][i], sgn, nsgn, r[P::RI1][i]);
   }
 }
 
