@@ -88,9 +88,22 @@ def cpu_baseline(threads, budget_s=12.0):
     dt = (time.perf_counter() - t0) / n
     vh = o.VH
     return {"value": round(BYTES_PER_SITE[("f64", 0)] * vh / dt / 1e9, 3), "unit": "GB/s", "cores": threads,
-            "kind": "port",
+            "kind": "port", "cpu": cpu_model(), "nproc": os.cpu_count(),
             "sample": f"oracle Dphi_eo (FP64, 18-real links) on a 16^4 lattice, {n} applications in {budget_s:.0f} s, "
                       f"{dt * 1e3:.2f} ms each; same compulsory-bytes model as the GPU value"}
+
+
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or the platform's."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or platform.machine()
 
 
 def true_residual(lat, fx, fb, mass=MASS):
