@@ -251,7 +251,12 @@ int yh_cg_mixed(yh_ctx* cc, const yh_inv_par* par, yh_field* dptr, const yh_fiel
   // by an update) up to two items ahead of the one whose scalar snapshot it
   // reads; items queued after the solve finished, or sloppy items queued
   // after the recurrence stopped on its own flag, return at once.  An early
-  // stop moves the next update forward.
+  // stop moves the next update forward.  Once a segment has ended on the
+  // delta flag, later segments are planned to that length: the chunk that
+  // reaches it ends there, and after it the host queues two iterations at a
+  // time, so that a stop leaves at most a few queued iterations idle instead
+  // of a whole chunk (a light system that reaches delta in ~16 iterations
+  // otherwise queued twice the iterations it ran).
   const double delta = (par->delta > 0.0 && par->delta < 1.0) ? par->delta : 0.1;
   SolverScalars* h = c->sc_host;
   memset(h, 0, sizeof(SolverScalars));
@@ -274,6 +279,8 @@ int yh_cg_mixed(yh_ctx* cc, const yh_inv_par* par, yh_field* dptr, const yh_fiel
   long planned = 0;     // sloppy iterations planned in total
   bool ru_next = false; // the recurrence stopped early: update next
   bool plan_done = false;
+  int est = 0;          // length of the last segment that ended on the delta flag (0: none yet)
+  int seg_start = 0;    // iterations done when the current segment started
   SolverScalars v;
   memset(&v, 0, sizeof(v));
   auto enqueue = [&](Item it) -> int {
@@ -308,6 +315,7 @@ int yh_cg_mixed(yh_ctx* cc, const yh_inv_par* par, yh_field* dptr, const yh_fiel
         if (capped) plan_done = true;  // the last update at the iteration cap
       } else {
         it.n = std::min(chunk, K - seg);
+        if (est > 0) it.n = std::min(it.n, std::max(2, est - seg));
         if (par->max_iter > 0) it.n = (int)std::min<long>(it.n, par->max_iter - planned);
         seg += it.n;
         planned += it.n;
@@ -332,10 +340,12 @@ int yh_cg_mixed(yh_ctx* cc, const yh_inv_par* par, yh_field* dptr, const yh_fiel
       plan_done = true;
       continue;
     }
+    if (it.ru) seg_start = v.iters;
     if (!it.ru && v.converged) {
       // the recurrence stopped inside this item: the sloppy items queued
       // after it did nothing, and the update comes next unless queued
       planned = v.iters;
+      if (v.reason == 2 && v.iters > seg_start) est = v.iters - seg_start;
       if (inflight.empty() || !inflight.front().ru) ru_next = true;
     }
   }
