@@ -515,6 +515,45 @@ __device__ __forceinline__ void load_clover_block(R (&c)[(2 * D) * (2 * D)], con
   }
 }
 
+// 16-bit block as loaded: the packed integers (4 per uint2) and the block's
+// scale; herm_mv_q decodes each entry where it is used, so the registers
+// hold the 16-bit words rather than n*n floats
+template <int D>
+__device__ __forceinline__ float load_clover_block_q(uint2 (&cq)[(2 * D) * (2 * D) / 4], const void* base, int blk,
+                                                     int site, int stride, const float* cnrm) {
+  constexpr int n2 = (2 * D) * (2 * D);
+  const uint2* p = (const uint2*)base + (size_t)blk * (n2 / 4) * stride;
+#pragma unroll
+  for (int k = 0; k < n2 / 4; k++) cq[k] = p[(size_t)k * stride + site];
+  return cnrm[(size_t)blk * stride + site] * Q_UNIT;
+}
+
+template <int N>
+__device__ __forceinline__ float clq(const uint2 (&cq)[N * N / 4], int k) {
+  const unsigned w = (k & 2) ? cq[k >> 2].y : cq[k >> 2].x;
+  return (k & 1) ? q2r_hi(w) : q2r_lo(w);
+}
+
+// y = sc H x for the packed Hermitian block held as 16-bit integers
+template <int N, typename R>
+__device__ __forceinline__ void herm_mv_q(Cx<R> (&y)[N], const uint2 (&cq)[N * N / 4], R sc, const Cx<R> (&x)[N]) {
+#pragma unroll
+  for (int i = 0; i < N; i++) y[i] = clq<N>(cq, i) * x[i];
+  int o = N;
+#pragma unroll
+  for (int i = 1; i < N; i++) {
+#pragma unroll
+    for (int j = 0; j < i; j++) {
+      Cx<R> h(clq<N>(cq, o), clq<N>(cq, o + 1));
+      o += 2;
+      cmac(y[i], h, x[j]);
+      cmac_conj(y[j], h, x[i]);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < N; i++) y[i] = sc * y[i];
+}
+
 // y = H x for the packed Hermitian block
 template <int N, typename R>
 __device__ __forceinline__ void herm_mv(Cx<R> (&y)[N], const R (&c)[N * N], const Cx<R> (&x)[N]) {

@@ -390,12 +390,18 @@ __global__ void __launch_bounds__(YH_HOP_BLOCK, hop_min_waves<PREC>()) k_hop(Hop
       int kb = ke;
 #pragma unroll
       for (int blk2 = 0; blk2 < 2; blk2++) {
-        R cl[N * N];
-        load_clover_block<PREC, D, R>(cl, a.clov, blk2, kb, g.VHpad, a.clov_nrm);
         Cx<R> xin[N], yout[N];
 #pragma unroll
         for (int i = 0; i < D; i++) { xin[i] = acc[2 * blk2][i]; xin[D + i] = acc[2 * blk2 + 1][i]; }
-        herm_mv<N, R>(yout, cl, xin);
+        if constexpr (PREC == PREC_F16) {
+          uint2 cq[N * N / 4];
+          const R cs = (R)load_clover_block_q<D>(cq, a.clov, blk2, kb, g.VHpad, a.clov_nrm);
+          herm_mv_q<N, R>(yout, cq, cs, xin);
+        } else {
+          R cl[N * N];
+          load_clover_block<PREC, D, R>(cl, a.clov, blk2, kb, g.VHpad, a.clov_nrm);
+          herm_mv<N, R>(yout, cl, xin);
+        }
 #pragma unroll
         for (int i = 0; i < D; i++) { o[2 * blk2][i] = yout[i]; o[2 * blk2 + 1][i] = yout[D + i]; }
         kb = ke + hop_group_dep<PREC, RECON, 5, D, R, STG>(o);
@@ -410,15 +416,21 @@ __global__ void __launch_bounds__(YH_HOP_BLOCK, hop_min_waves<PREC>()) k_hop(Hop
       int kb = ke;
 #pragma unroll
       for (int blk2 = 0; blk2 < 2; blk2++) {
-        R cl[N * N];
-        load_clover_block<PREC, D, R>(cl, a.clov, blk2, kb, g.VHpad, a.clov_nrm);
         Cx<R> xin[N], yout[N];
 #pragma unroll
         for (int i = 0; i < D; i++) {
           xin[i] = zs ? Cx<R>((R)0, (R)0) : xs[2 * blk2][i];
           xin[D + i] = zs ? Cx<R>((R)0, (R)0) : xs[2 * blk2 + 1][i];
         }
-        herm_mv<N, R>(yout, cl, xin);
+        if constexpr (PREC == PREC_F16) {
+          uint2 cq[N * N / 4];
+          const R cs = (R)load_clover_block_q<D>(cq, a.clov, blk2, kb, g.VHpad, a.clov_nrm);
+          herm_mv_q<N, R>(yout, cq, cs, xin);
+        } else {
+          R cl[N * N];
+          load_clover_block<PREC, D, R>(cl, a.clov, blk2, kb, g.VHpad, a.clov_nrm);
+          herm_mv<N, R>(yout, cl, xin);
+        }
 #pragma unroll
         for (int i = 0; i < D; i++) {
           acc[2 * blk2][i] = acc[2 * blk2][i] + yout[i] + dg * xin[i];
