@@ -554,6 +554,41 @@ __device__ __forceinline__ void herm_mv_q(Cx<R> (&y)[N], const uint2 (&cq)[N * N
   for (int i = 0; i < N; i++) y[i] = sc * y[i];
 }
 
+// y = H x for a packed FP64 block of D_R >= 6 (2 D_R^2 doubles per block)
+// read from memory as it is used: after every ~24 complex entries the next
+// loads wait (opaque zero) for the products so far, so at most that many
+// entries are live.  The products are those of herm_mv, in its order.
+template <int N>
+__device__ __forceinline__ void herm_mv_f64_stream(Cx<double> (&y)[N], const void* base, int blk, int site,
+                                                   int stride, const Cx<double> (&x)[N]) {
+  const double2* p = (const double2*)base + (size_t)blk * (N * N / 2) * stride;
+  int dep = 0;
+#pragma unroll
+  for (int k = 0; k < N / 2; k++) {
+    const double2 v = p[(size_t)k * stride + site];
+    y[2 * k] = v.x * x[2 * k];
+    y[2 * k + 1] = v.y * x[2 * k + 1];
+  }
+  int cnt = 0;
+#pragma unroll
+  for (int i = 1; i < N; i++) {
+    if (cnt >= 24) {
+#pragma unroll
+      for (int q = 0; q < N; q++) __asm__ volatile("" : "+v"(y[q].re), "+v"(y[q].im));
+      __asm__ volatile("v_mov_b32 %0, 0" : "=v"(dep));
+      cnt = 0;
+    }
+#pragma unroll
+    for (int j = 0; j < i; j++) {
+      const double2 v = p[(size_t)(N / 2 + i * (i - 1) / 2 + j) * stride + site + dep];
+      const Cx<double> h(v.x, v.y);
+      cmac(y[i], h, x[j]);
+      cmac_conj(y[j], h, x[i]);
+    }
+    cnt += i;
+  }
+}
+
 // y = H x for the packed Hermitian block
 template <int N, typename R>
 __device__ __forceinline__ void herm_mv(Cx<R> (&y)[N], const R (&c)[N * N], const Cx<R> (&x)[N]) {

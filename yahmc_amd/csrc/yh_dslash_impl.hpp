@@ -86,36 +86,95 @@ __device__ __forceinline__ void hop_dir_src(Cx<R> (&acc)[4][D], const HopArgs& a
     h[1][i] = s[3][i] + unit_mul<P::PC1>(s[P::PI1][i]);
   }
   const LV* lbase = (const LV*)(FWD ? a.lf : a.lb) + (size_t)MU * LNV * a.g.VHpad;
-  Cx<R> u[D][D];
-  // FP64 hops stream the links (each is read once per hop) with non-temporal
-  // loads and write their output with non-temporal stores: measured faster
-  // for FP64 (326 -> 301 us, 18 reals, 32^4), neutral for FP32 and slower
-  // for FP16, which keep ordinary accesses
-  load_link<PREC, RECON, D, R, PREC == PREC_F64, PREC == PREC_F16>(u, lbase, lidx, a.g.VHpad);
-  if constexpr (PREC == PREC_F16) sc *= (R)Q_UNIT;  // the links were loaded as 32768 U
   Cx<R> r[2][D];
+  if constexpr (PREC == PREC_F64 && RECON == RECON_18 && D >= 6) {
+    // a D_R x D_R FP64 link is 2 D_R^2 doubles (144 registers at D_R = 6,
+    // 256 at 8): it is read in NP parts of RP rows (forward: U h) or columns
+    // (backward: U^+ h), each part giving RP complete output elements that go
+    // straight into the accumulator; the next part's loads wait (opaque
+    // zero) for that, so one part is live at a time.  Every output element
+    // sums over j in the order of the one-piece product: bit-identical.
+    constexpr int NP = D >= 8 ? 4 : 3, RP = D / NP;
+    const R sgn = flip ? -(R)1 : (R)1, nsgn = -sgn;
+    // the first part waits for the projected half spinor, so the full
+    // neighbour spinor is not live beside the link
+#pragma unroll
+    for (int i = 0; i < D; i++) {
+      __asm__ volatile("" : "+v"(h[0][i].re), "+v"(h[0][i].im));
+      __asm__ volatile("" : "+v"(h[1][i].re), "+v"(h[1][i].im));
+    }
+    int dep;
+    __asm__ volatile("v_mov_b32 %0, 0" : "=v"(dep));
+#pragma unroll
+    for (int part = 0; part < NP; part++) {
+      Cx<R> u[RP][D];
+#pragma unroll
+      for (int k = 0; k < RP * D; k++) {
+        const int o = part * RP + k / D, j = k % D;  // output element o, summed index j
+        const double2 v = ld_stream(&lbase[(size_t)(FWD ? o * D + j : j * D + o) * a.g.VHpad + lidx + dep]);
+        u[k / D][j] = Cx<R>((R)v.x, (R)v.y);
+      }
+#pragma unroll
+      for (int l = 0; l < RP; l++) {
+        const int o = part * RP + l;
+        Cx<R> r0((R)0, (R)0), r1((R)0, (R)0);
+#pragma unroll
+        for (int j = 0; j < D; j++) {
+          if constexpr (FWD) {
+            cmac(r0, u[l][j], h[0][j]);
+            cmac(r1, u[l][j], h[1][j]);
+          } else {
+            cmac_conj(r0, u[l][j], h[0][j]);
+            cmac_conj(r1, u[l][j], h[1][j]);
+          }
+        }
+        r[0][o] = r0;
+        r[1][o] = r1;
+        unit_fma<0>(acc[2][o], sgn, nsgn, r0);
+        unit_fma<0>(acc[3][o], sgn, nsgn, r1);
+        unit_fma<P::RC0>(acc[0][o], sgn, nsgn, r[P::RI0][o]);
+        unit_fma<P::RC1>(acc[1][o], sgn, nsgn, r[P::RI1][o]);
+      }
+      if (part + 1 < NP) {
+#pragma unroll
+        for (int l = 0; l < RP; l++)
+#pragma unroll
+          for (int sp = 0; sp < 4; sp++) __asm__ volatile("" : "+v"(acc[sp][part * RP + l].re), "+v"(acc[sp][part * RP + l].im));
+        __asm__ volatile("v_mov_b32 %0, 0" : "=v"(dep));
+      }
+    }
+    return;
+  } else {
+    Cx<R> u[D][D];
+    // FP64 hops stream the links (each is read once per hop) with
+    // non-temporal loads and write their output with non-temporal stores:
+    // measured faster for FP64 (326 -> 301 us, 18 reals, 32^4), neutral for
+    // FP32 and slower for FP16, which keep ordinary accesses
+    load_link<PREC, RECON, D, R, PREC == PREC_F64, PREC == PREC_F16>(u, lbase, lidx, a.g.VHpad);
+    if constexpr (PREC == PREC_F16) sc *= (R)Q_UNIT;  // the links were loaded as 32768 U
+#pragma unroll
+    for (int i = 0; i < D; i++) {
+      Cx<R> r0((R)0, (R)0), r1((R)0, (R)0);
+#pragma unroll
+      for (int j = 0; j < D; j++) {
+        if constexpr (FWD) {
+          cmac(r0, u[i][j], h[0][j]);
+          cmac(r1, u[i][j], h[1][j]);
+        } else {
+          cmac_conj(r0, u[j][i], h[0][j]);
+          cmac_conj(r1, u[j][i], h[1][j]);
+        }
+      }
+      r[0][i] = r0;
+      r[1][i] = r1;
+    }
+  }
   // the antiperiodic sign and the 16-bit scales multiply the accumulation
   // (a branch here splits the kernel into blocks and serialises the
   // neighbour loads of the t direction); folded into its FMAs they cost no
   // separate multiplies
   const R sgn = flip ? -sc : sc;
   const R nsgn = -sgn;
-#pragma unroll
-  for (int i = 0; i < D; i++) {
-    Cx<R> r0((R)0, (R)0), r1((R)0, (R)0);
-#pragma unroll
-    for (int j = 0; j < D; j++) {
-      if constexpr (FWD) {
-        cmac(r0, u[i][j], h[0][j]);
-        cmac(r1, u[i][j], h[1][j]);
-      } else {
-        cmac_conj(r0, u[j][i], h[0][j]);
-        cmac_conj(r1, u[j][i], h[1][j]);
-      }
-    }
-    r[0][i] = r0;
-    r[1][i] = r1;
-  }
 #pragma unroll
   for (int i = 0; i < D; i++) {
     unit_fma<0>(acc[2][i], sgn, nsgn, r[0][i]);
@@ -397,6 +456,8 @@ __global__ void __launch_bounds__(YH_HOP_BLOCK, hop_min_waves<PREC>()) k_hop(Hop
           uint2 cq[N * N / 4];
           const R cs = (R)load_clover_block_q<D>(cq, a.clov, blk2, kb, g.VHpad, a.clov_nrm);
           herm_mv_q<N, R>(yout, cq, cs, xin);
+        } else if constexpr (PREC == PREC_F64 && D >= 6) {
+          herm_mv_f64_stream<N>(yout, a.clov, blk2, kb, g.VHpad, xin);
         } else {
           R cl[N * N];
           load_clover_block<PREC, D, R>(cl, a.clov, blk2, kb, g.VHpad, a.clov_nrm);
@@ -409,14 +470,25 @@ __global__ void __launch_bounds__(YH_HOP_BLOCK, hop_min_waves<PREC>()) k_hop(Hop
       store_spinor<PREC, D, R, PREC == PREC_F64>(a.out, a.out_nrm, k, g.VHpad, o);
     } else {
       constexpr int N = 2 * D;
+      // FP64 at D_R >= 6: each block's two spins of the own spinor are read
+      // with the block (the whole spinor is 8 D_R doubles)
+      constexpr bool SPLIT = PREC == PREC_F64 && D >= 6;
       Cx<R> xs[4][D];
-      load_spinor<PREC, D, R>(xs, a.self, a.self_nrm, ke, g.VHpad);
+      if constexpr (!SPLIT) load_spinor<PREC, D, R>(xs, a.self, a.self_nrm, ke, g.VHpad);
       const R dg = (R)(a.diag + bc_cf_shift(t, g));
       const bool zs = a.zero_self_bc && bz;
       int kb = ke;
 #pragma unroll
       for (int blk2 = 0; blk2 < 2; blk2++) {
         Cx<R> xin[N], yout[N];
+        if constexpr (SPLIT) {
+          const double2* sp = (const double2*)a.self;
+#pragma unroll
+          for (int c = 0; c < 2 * D; c++) {
+            const double2 v = sp[(size_t)(2 * blk2 * D + c) * g.VHpad + kb];
+            xs[2 * blk2 + c / D][c % D] = Cx<R>((R)v.x, (R)v.y);
+          }
+        }
 #pragma unroll
         for (int i = 0; i < D; i++) {
           xin[i] = zs ? Cx<R>((R)0, (R)0) : xs[2 * blk2][i];
@@ -426,6 +498,8 @@ __global__ void __launch_bounds__(YH_HOP_BLOCK, hop_min_waves<PREC>()) k_hop(Hop
           uint2 cq[N * N / 4];
           const R cs = (R)load_clover_block_q<D>(cq, a.clov, blk2, kb, g.VHpad, a.clov_nrm);
           herm_mv_q<N, R>(yout, cq, cs, xin);
+        } else if constexpr (PREC == PREC_F64 && D >= 6) {
+          herm_mv_f64_stream<N>(yout, a.clov, blk2, kb, g.VHpad, xin);
         } else {
           R cl[N * N];
           load_clover_block<PREC, D, R>(cl, a.clov, blk2, kb, g.VHpad, a.clov_nrm);
