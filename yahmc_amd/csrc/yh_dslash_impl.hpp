@@ -90,11 +90,11 @@ __device__ __forceinline__ void hop_dir_src(Cx<R> (&acc)[4][D], const HopArgs& a
   if constexpr (PREC == PREC_F64 && RECON == RECON_18 && D >= 6) {
     // a D_R x D_R FP64 link is 2 D_R^2 doubles (144 registers at D_R = 6,
     // 256 at 8): it is read in NP parts of RP rows (forward: U h) or columns
-    // (three at D_R = 6, two at 8)
-    // (backward: U^+ h), each part giving RP complete output elements that go
-    // straight into the accumulator; the next part's loads wait (opaque
-    // zero) for that, so one part is live at a time.  Every output element
-    // sums over j in the order of the one-piece product: bit-identical.
+    // (backward: U^+ h), three at D_R = 6 and two at 8, each part giving RP
+    // complete output elements that go straight into the accumulator; the
+    // next part's loads wait (opaque zero) for that, so one part is live at
+    // a time.  Every output element sums over j in the order of the
+    // one-piece product: bit-identical.
     constexpr int NP = D >= 8 ? 4 : 2, RP = D / NP;
     const R sgn = flip ? -(R)1 : (R)1, nsgn = -sgn;
     // the first part waits for the projected half spinor, so the full
