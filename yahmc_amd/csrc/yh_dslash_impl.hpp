@@ -292,7 +292,10 @@ constexpr int hop_min_waves() {
 // of a finished group are free again.  Defaults from a sweep of all masks on
 // MI355X (SU(3), 32^4, GB/s by algorithmic bytes): FP64-18 t|xy|z 5763
 // (one group: 2670, spilling), FP64-12 t|x|y|z 5374 (4771), FP32-12 t|xyz
-// 5174 (5111), FP16-12 txy|z 4302 (3998).
+// 5174 (5111), FP16-12 txy|z 4302 (3998).  The 16-bit kernels were swept
+// again by the time of the 16-bit mixed CG at 32^4, which runs the
+// clover-epilogue variants: t x|y|z 0.2203-0.2210 s against 0.2221-0.2226 s
+// for txy|z (one group 0.2394, t|x y|z 0.2251, one per direction 0.2246).
 #ifndef YH_SCHED_F64_18
 #define YH_SCHED_F64_18 5
 #endif
@@ -303,7 +306,7 @@ constexpr int hop_min_waves() {
 #define YH_SCHED_F32 1
 #endif
 #ifndef YH_SCHED_F16
-#define YH_SCHED_F16 4
+#define YH_SCHED_F16 6
 #endif
 // representations with D >= 4: bit 3 also splits each direction between its
 // forward and backward hop (one neighbour spinor and link in flight)
